@@ -1,0 +1,234 @@
+/*
+ * vortexlab-mi355x: C ABI of the MI355X-native D2Q9 lattice-Boltzmann time step.
+ *
+ * The reference (a Rust crate) has no FFI layer: its boundary is the public Rust API
+ * Builder -> Sim -> Channel.  This header is that API restated as plain C, so a Rust
+ * `extern "C"` binding (or ctypes, JNI, ...) can drive the GPU solver with the same
+ * call sequence.  Each declaration cites the reference item it replaces
+ * (path:line into the reference crate).
+ *
+ * Conventions
+ *  - Every fallible call returns vl_status (0 = VL_OK).  The reference panics on misuse
+ *    (src/sim.rs:85, 101, 122-124, 222, 261, 303); here those become VL_EINVAL with a
+ *    message in vl_last_error() (thread-local).
+ *  - Divergence (the reference's `!cd.is_finite()` check, src/cases/tunnel.rs:78-84,
+ *    src/cases/cylinder.rs:179-186) is VL_ENONFINITE; the step index is in the message.
+ *  - Handles are single-caller: one thread at a time per handle.  Each vl_sim owns one
+ *    HIP stream; all work is stream-ordered and only readbacks synchronise.
+ *  - Arrays handed in are copied; the library owns all device memory.
+ *  - Populations use the reference layout in every host array: f[i * nx * ny + y * nx + x],
+ *    i = 0..8 in the reference's direction numbering (src/lattice.rs:3-24).
+ */
+#ifndef VORTEXLAB_H
+#define VORTEXLAB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int vl_status;
+#define VL_OK 0
+#define VL_EINVAL 1      /* bad argument or builder misuse (reference: panic/assert) */
+#define VL_ENOMEM 2      /* host or device allocation failed */
+#define VL_EDEVICE 3     /* a HIP call failed (no GPU, launch failure, ...) */
+#define VL_ENONFINITE 4  /* the run diverged (non-finite force coefficient) */
+
+#define VL_ABI_VERSION 1
+
+/* Storage precision of the populations.  F64 stores f_i; F32/F16 store the deviation
+ * f_i - w_i (F16 scaled by 2^10) and collide in FP32.  Forces, the outlet controller and
+ * all statistics stay FP64 for every precision. */
+#define VL_F64 0
+#define VL_F32 1
+#define VL_F16 2
+
+/* Collision models: src/lattice.rs:28-45. */
+#define VL_BGK 0
+#define VL_TRT 1
+
+/* Last error message of the calling thread ("" if none). */
+const char* vl_last_error(void);
+int vl_abi_version(void);
+/* Number of HIP devices visible (0 without a GPU); never fails. */
+int vl_device_count(void);
+
+/* ------------------------------------------------------------------------------------
+ * Shapes: src/geometry.rs:12-95, 187-279.  A shape only answers "is (x, y) inside?".
+ * Composite constructors take ownership of `inner`.
+ * ---------------------------------------------------------------------------------- */
+typedef struct vl_shape vl_shape;
+typedef int (*vl_inside_fn)(void* ctx, double x, double y);
+
+vl_shape* vl_shape_circle(double cx, double cy, double r);                 /* geometry.rs:16-27 */
+vl_shape* vl_shape_polygon(const double* xy, size_t npts);                 /* geometry.rs:30-49 */
+vl_shape* vl_shape_rounded_square(double cx, double cy, double half, double r); /* :52-66 */
+vl_shape* vl_shape_rotated(vl_shape* inner, double cx, double cy, double angle); /* :69-82 */
+vl_shape* vl_shape_translated(vl_shape* inner, double dx, double dy);      /* geometry.rs:85-95 */
+/* Square section of side d with a corner treatment (geometry.rs:187-261):
+ * corner 0 sharp, 1 chamfered(b), 2 rounded(b), 3 recessed(b), 4 double-recessed(b). */
+vl_shape* vl_shape_section(int corner, double b, double cx, double cy, double d, double angle);
+/* "Everything outside the band y0 < y < y1" (src/cases/poiseuille.rs:12-21). */
+vl_shape* vl_shape_outside_band(double y0, double y1);
+/* Caller-supplied predicate (the Rust `Box<dyn Shape>`); called only at build time. */
+vl_shape* vl_shape_callback(vl_inside_fn fn, void* ctx);
+int vl_shape_inside(const vl_shape* s, double x, double y);
+/* Fraction of the link (x,y)->(x+dx,y+dy) in the fluid: geometry.rs:268-279. */
+double vl_cut_fraction(const vl_shape* s, double x, double y, double dx, double dy);
+void vl_shape_free(vl_shape* s);
+
+/* ------------------------------------------------------------------------------------
+ * Wall velocity functions (the reference's VelocityFn closures, src/sim.rs:33-34),
+ * evaluated only at build time at the wall point of each link.
+ * ---------------------------------------------------------------------------------- */
+typedef void (*vl_velocity_fn)(void* ctx, double x, double y, double* out2);
+#define VL_VEL_REST 0      /* wall at rest (velocity: None) */
+#define VL_VEL_CONST 1     /* [p0, p1] */
+#define VL_VEL_PARABOLA 2  /* channel inlet profile, u_mean = p0, h = p1 (cases/channel.rs:95-103) */
+#define VL_VEL_SPIN 3      /* rotation about (p0, p1): [-(y - p1), x - p0] (cases/channel.rs:122) */
+#define VL_VEL_CALLBACK 4  /* fn(ctx, x, y, out) */
+typedef struct {
+  int kind;
+  double p[4];
+  vl_velocity_fn fn;
+  void* ctx;
+} vl_velocity;
+
+/* ------------------------------------------------------------------------------------
+ * Builder: src/sim.rs:70-137.
+ * ---------------------------------------------------------------------------------- */
+typedef struct vl_builder vl_builder;
+vl_builder* vl_builder_new(size_t nx, size_t ny);                          /* sim.rs:84-97 */
+/* pub fields nx, ny, periodic_x/y, tau, collision, accel (sim.rs:70-81). */
+vl_status vl_builder_config(vl_builder* b, int periodic_x, int periodic_y, double tau,
+                            int collision, double magic, double ax, double ay);
+/* Kind::Wall(Wall{velocity, vgroup, fgroup, shape}) (sim.rs:36-54, 100-104).  Returns the
+ * kind id (>0) or -1.  fgroup < 0 means "no force bookkeeping"; `shape` is consumed. */
+int vl_builder_add_wall(vl_builder* b, const vl_velocity* vel, int vgroup, int fgroup,
+                        vl_shape* shape);
+int vl_builder_add_pressure(vl_builder* b, double rho);                    /* Kind::Pressure */
+int vl_builder_add_radiating(vl_builder* b, double rho);                   /* Kind::Radiating */
+vl_status vl_builder_fill_rect(vl_builder* b, size_t x0, size_t x1, size_t y0, size_t y1,
+                               int kind);                                  /* sim.rs:111-118 */
+vl_status vl_builder_fill_shape(vl_builder* b, int kind);                  /* sim.rs:120-132 */
+void vl_builder_free(vl_builder* b);
+
+/* ------------------------------------------------------------------------------------
+ * Sim: src/sim.rs:180-673 on one GPU.
+ * ---------------------------------------------------------------------------------- */
+typedef struct vl_sim vl_sim;
+
+typedef struct {
+  int64_t nx, ny;
+  int64_t fluid_nodes;   /* sim.rs:377-379 */
+  int64_t link_count;    /* sim.rs:386-388 */
+  int64_t bnode_count;
+  int64_t pnode_count;
+  int64_t force_groups;  /* len of Sim::force */
+  int64_t t;             /* Sim::t */
+  int32_t precision;
+  int32_t collision;
+  double omega_plus, omega_minus;
+  double outlet_density_offset; /* Sim::outlet_density_offset (last step) */
+  int64_t device_bytes;
+} vl_sim_info;
+
+/* Per-step host controls (what Channel::step sets before Sim::step, cases/channel.rs:136-147).
+ * target is used only when has_target != 0 (Sim::outflow_target = Some(target)). */
+typedef struct {
+  double vscale[4];
+  double target;
+  int32_t has_target;
+  int32_t pad_;
+} vl_ctl;
+
+/* Builder::build (sim.rs:134-136, from_builder :219-354).  The builder is consumed
+ * (freed) whatever the outcome.  State is initialised to rest (rho 1, u 0). */
+vl_status vl_sim_create(vl_builder* b, int precision, vl_sim** out);
+void vl_sim_destroy(vl_sim* s);
+vl_status vl_sim_info_get(vl_sim* s, vl_sim_info* out);
+/* Sim::init (sim.rs:357-371): equilibrium of per-node (rho, ux, uy), arrays of nx*ny;
+ * non-fluid cells get (1, 0, 0).  Resets t to 0. */
+vl_status vl_sim_init_macro(vl_sim* s, const double* rho, const double* ux, const double* uy);
+/* Sim::vscale and Sim::outflow_target (sim.rs:195-208). */
+vl_status vl_sim_set_vscale(vl_sim* s, const double* vscale4);
+vl_status vl_sim_set_outflow_target(vl_sim* s, int enabled, double target);
+/* nsteps x Sim::step (sim.rs:391-420) with the current vscale / outflow_target. */
+vl_status vl_sim_step(vl_sim* s, int64_t nsteps);
+/* nsteps steps with per-step controls ctl[k]; if force_hist is non-NULL it receives
+ * Sim::force after every step: force_hist[(k * force_groups + g) * 2 + {0,1}]. */
+vl_status vl_sim_run(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist);
+/* Sim::force after the last step (force_groups * 2 doubles). */
+vl_status vl_sim_get_force(vl_sim* s, double* out);
+/* Populations of A (the current state) in the reference layout, as f_i in FP64. */
+vl_status vl_sim_read(vl_sim* s, double* f);
+vl_status vl_sim_write(vl_sim* s, const double* f);
+/* Sim::macros for every node (sim.rs:527-534): rho, ux, uy arrays of nx*ny. */
+vl_status vl_sim_macros(vl_sim* s, double* rho, double* ux, double* uy);
+vl_status vl_sim_state_hash(vl_sim* s, uint64_t* out);          /* sim.rs:654-668 */
+vl_status vl_sim_total_mass(vl_sim* s, double* out);            /* sim.rs:538-548 */
+vl_status vl_sim_total_momentum(vl_sim* s, double* out2);       /* sim.rs:551-563 */
+vl_status vl_sim_max_speed(vl_sim* s, double* out);             /* sim.rs:566-579 */
+
+/* ------------------------------------------------------------------------------------
+ * Channel: src/cases/channel.rs:40-155 (a body in a channel; the per-step driver).
+ * ---------------------------------------------------------------------------------- */
+#define VL_INFLOW_PARABOLIC 0
+#define VL_INFLOW_UNIFORM 1
+#define VL_INFLOW_UNIFORM_OPEN 2
+typedef struct {
+  double cx, cy, omega;
+  uint64_t start, duration;
+} vl_kick;
+typedef struct {
+  int64_t nx, ny;
+  int32_t inflow;      /* VL_INFLOW_* */
+  int32_t collision;   /* VL_BGK / VL_TRT */
+  double u;            /* u_mean (parabolic) or u (uniform) */
+  double nu;           /* lattice viscosity */
+  double magic;        /* TRT Lambda */
+  uint64_t ramp_steps;
+  int32_t has_kick;
+  int32_t pad_;
+  vl_kick kick;
+} vl_channel_spec;
+
+typedef struct vl_channel vl_channel;
+/* Channel::new (channel.rs:89-134); `body` is consumed. */
+vl_status vl_channel_create(const vl_channel_spec* spec, vl_shape* body, int precision,
+                            vl_channel** out);
+void vl_channel_destroy(vl_channel* c);
+vl_sim* vl_channel_sim(vl_channel* c);
+/* Channel::absorbing_outlet (channel.rs:84-85). */
+vl_status vl_channel_set_absorbing(vl_channel* c, int absorbing);
+/* nsteps x Channel::step (channel.rs:136-148).  If coef is non-NULL it receives the drag
+ * and lift coefficients 2F/(u_ref^2 d) after every step (coef[2k], coef[2k+1]), as
+ * Channel::coefficients (channel.rs:151-154).  A non-finite drag coefficient returns
+ * VL_ENONFINITE (the history up to that step is still written). */
+vl_status vl_channel_run(vl_channel* c, int64_t nsteps, double d, double* coef);
+vl_status vl_channel_coefficients(vl_channel* c, double d, double* cd, double* cl);
+
+/* ------------------------------------------------------------------------------------
+ * Force-history statistics: src/cases/channel.rs:157-207 and src/signal.rs (host, FP64).
+ * ---------------------------------------------------------------------------------- */
+typedef struct {
+  double cd_mean, cd_rms, cl_mean, cl_rms, st, st_crossings;
+  int32_t periodic;
+  int32_t pad_;
+} vl_stats;
+/* Series::stats (channel.rs:182-206) of a history sampled every dt, from time `from`. */
+vl_status vl_series_stats(const double* cd, const double* cl, size_t n, double dt,
+                          double from, vl_stats* out);
+/* signal::dominant_frequency (signal.rs:92-107): out2 = (frequency, amplitude). */
+vl_status vl_dominant_frequency(const double* x, size_t n, double dt, double* out2);
+/* signal::crossing_frequency (signal.rs:127-140); returns 0 when fewer than two crossings. */
+vl_status vl_crossing_frequency(const double* x, size_t n, double dt, double* out, int* found);
+/* In-place radix-2 FFT (signal.rs:6-43); n must be a power of two. */
+vl_status vl_fft(double* re, double* im, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VORTEXLAB_H */
