@@ -171,6 +171,13 @@ vl_status vl_sim_state_hash(vl_sim* s, uint64_t* out);          /* sim.rs:654-66
 vl_status vl_sim_total_mass(vl_sim* s, double* out);            /* sim.rs:538-548 */
 vl_status vl_sim_total_momentum(vl_sim* s, double* out2);       /* sim.rs:551-563 */
 vl_status vl_sim_max_speed(vl_sim* s, double* out);             /* sim.rs:566-579 */
+/* Kernel timing with HIP events on the sim's own stream (no reference counterpart; the
+ * reference times wall clock only, src/cases/cylinder.rs:224-232).  Switching it on resets
+ * the counters; bulk_ms sums the duration of every bulk stream-collide launch, total_ms the
+ * duration of whole steps (every kernel of the step). */
+vl_status vl_sim_set_timing(vl_sim* s, int on);
+vl_status vl_sim_get_timing(vl_sim* s, double* bulk_ms, int64_t* bulk_launches, double* total_ms,
+                            int64_t* steps);
 
 /* ------------------------------------------------------------------------------------
  * Channel: src/cases/channel.rs:40-155 (a body in a channel; the per-step driver).

@@ -1,0 +1,610 @@
+"""Python mirror of the reference's solver API (Builder -> Sim -> Channel) over a C ABI.
+
+The classes follow the reference crate's public Rust API (src/sim.rs:36-673,
+src/geometry.rs, src/cases/channel.rs) and call a shared library through ctypes.  They are
+written against a *backend*: the MI355X library ``libvortexlab.so`` (prefix ``vl_``) or, in
+tests only, the CPU oracle (prefix ``or_``), which exports the same entry points.  That lets a
+test state one case once and run it on both.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+# ---------------------------------------------------------------------------------------
+# constants (include/vortexlab.h)
+# ---------------------------------------------------------------------------------------
+OK, EINVAL, ENOMEM, EDEVICE, ENONFINITE = 0, 1, 2, 3, 4
+F64, F32, F16 = 0, 1, 2
+BGK, TRT = 0, 1
+VEL_REST, VEL_CONST, VEL_PARABOLA, VEL_SPIN, VEL_CALLBACK = 0, 1, 2, 3, 4
+INFLOW_PARABOLIC, INFLOW_UNIFORM, INFLOW_UNIFORM_OPEN = 0, 1, 2
+CORNER = {"square": 0, "sharp": 0, "chamfered": 1, "rounded": 2, "recessed": 3, "double-recessed": 4}
+Q = 9
+CX = (0, 1, 0, -1, 0, 1, -1, -1, 1)
+CY = (0, 0, 1, 0, -1, 1, 1, -1, -1)
+W = (4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0)
+CS2 = 1.0 / 3.0
+
+INSIDE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_double)
+VELOCITY_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double))
+
+
+class VelocityDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p", C.c_double * 4), ("fn", VELOCITY_FN), ("ctx", C.c_void_p)]
+
+
+class SimInfo(C.Structure):
+    _fields_ = [("nx", C.c_int64), ("ny", C.c_int64), ("fluid_nodes", C.c_int64), ("link_count", C.c_int64),
+                ("bnode_count", C.c_int64), ("pnode_count", C.c_int64), ("force_groups", C.c_int64),
+                ("t", C.c_int64), ("precision", C.c_int32), ("collision", C.c_int32),
+                ("omega_plus", C.c_double), ("omega_minus", C.c_double),
+                ("outlet_density_offset", C.c_double), ("device_bytes", C.c_int64)]
+
+
+class Ctl(C.Structure):
+    _fields_ = [("vscale", C.c_double * 4), ("target", C.c_double), ("has_target", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
+class KickDesc(C.Structure):
+    _fields_ = [("cx", C.c_double), ("cy", C.c_double), ("omega", C.c_double), ("start", C.c_uint64),
+                ("duration", C.c_uint64)]
+
+
+class ChannelSpecDesc(C.Structure):
+    _fields_ = [("nx", C.c_int64), ("ny", C.c_int64), ("inflow", C.c_int32), ("collision", C.c_int32),
+                ("u", C.c_double), ("nu", C.c_double), ("magic", C.c_double), ("ramp_steps", C.c_uint64),
+                ("has_kick", C.c_int32), ("pad_", C.c_int32), ("kick", KickDesc)]
+
+
+class StatsDesc(C.Structure):
+    _fields_ = [("cd_mean", C.c_double), ("cd_rms", C.c_double), ("cl_mean", C.c_double),
+                ("cl_rms", C.c_double), ("st", C.c_double), ("st_crossings", C.c_double),
+                ("periodic", C.c_int32), ("pad_", C.c_int32)]
+
+
+class VortexError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[status {status}] {message}")
+        self.status = status
+
+
+class Diverged(VortexError):
+    pass
+
+
+_D = C.POINTER(C.c_double)
+_VP = C.c_void_p
+# (name, restype, argtypes); the prefix (vl_ / or_) is added per backend.
+_SIGS = [
+    ("last_error", C.c_char_p, []),
+    ("shape_circle", _VP, [C.c_double] * 3),
+    ("shape_polygon", _VP, [_D, C.c_size_t]),
+    ("shape_rounded_square", _VP, [C.c_double] * 4),
+    ("shape_rotated", _VP, [_VP, C.c_double, C.c_double, C.c_double]),
+    ("shape_translated", _VP, [_VP, C.c_double, C.c_double]),
+    ("shape_section", _VP, [C.c_int] + [C.c_double] * 5),
+    ("shape_outside_band", _VP, [C.c_double, C.c_double]),
+    ("shape_callback", _VP, [INSIDE_FN, _VP]),
+    ("shape_inside", C.c_int, [_VP, C.c_double, C.c_double]),
+    ("cut_fraction", C.c_double, [_VP] + [C.c_double] * 4),
+    ("shape_free", None, [_VP]),
+    ("builder_new", _VP, [C.c_size_t, C.c_size_t]),
+    ("builder_config", C.c_int, [_VP, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
+                                 C.c_double]),
+    ("builder_add_wall", C.c_int, [_VP, C.POINTER(VelocityDesc), C.c_int, C.c_int, _VP]),
+    ("builder_add_pressure", C.c_int, [_VP, C.c_double]),
+    ("builder_add_radiating", C.c_int, [_VP, C.c_double]),
+    ("builder_fill_rect", C.c_int, [_VP, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int]),
+    ("builder_fill_shape", C.c_int, [_VP, C.c_int]),
+    ("builder_free", None, [_VP]),
+    ("sim_create", C.c_int, [_VP, C.c_int, C.POINTER(_VP)]),
+    ("sim_destroy", None, [_VP]),
+    ("sim_info_get", C.c_int, [_VP, C.POINTER(SimInfo)]),
+    ("sim_init_macro", C.c_int, [_VP, _D, _D, _D]),
+    ("sim_set_vscale", C.c_int, [_VP, _D]),
+    ("sim_set_outflow_target", C.c_int, [_VP, C.c_int, C.c_double]),
+    ("sim_step", C.c_int, [_VP, C.c_int64]),
+    ("sim_run", C.c_int, [_VP, C.c_int64, C.POINTER(Ctl), _D]),
+    ("sim_get_force", C.c_int, [_VP, _D]),
+    ("sim_read", C.c_int, [_VP, _D]),
+    ("sim_write", C.c_int, [_VP, _D]),
+    ("sim_macros", C.c_int, [_VP, _D, _D, _D]),
+    ("sim_state_hash", C.c_int, [_VP, C.POINTER(C.c_uint64)]),
+    ("sim_total_mass", C.c_int, [_VP, _D]),
+    ("sim_total_momentum", C.c_int, [_VP, _D]),
+    ("sim_max_speed", C.c_int, [_VP, _D]),
+    ("sim_set_timing", C.c_int, [_VP, C.c_int]),
+    ("sim_get_timing", C.c_int, [_VP, _D, C.POINTER(C.c_int64), _D, C.POINTER(C.c_int64)]),
+    ("channel_create", C.c_int, [C.POINTER(ChannelSpecDesc), _VP, C.c_int, C.POINTER(_VP)]),
+    ("channel_destroy", None, [_VP]),
+    ("channel_sim", _VP, [_VP]),
+    ("channel_set_absorbing", C.c_int, [_VP, C.c_int]),
+    ("channel_run", C.c_int, [_VP, C.c_int64, C.c_double, _D]),
+    ("channel_coefficients", C.c_int, [_VP, C.c_double, _D, _D]),
+    ("series_stats", C.c_int, [_D, _D, C.c_size_t, C.c_double, C.c_double, C.POINTER(StatsDesc)]),
+    ("dominant_frequency", C.c_int, [_D, C.c_size_t, C.c_double, _D]),
+    ("crossing_frequency", C.c_int, [_D, C.c_size_t, C.c_double, _D, C.POINTER(C.c_int)]),
+    ("fft", C.c_int, [_D, _D, C.c_size_t]),
+]
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_D)
+
+
+class Backend:
+    """A loaded shared library exposing the solver entry points under one prefix.
+
+    ``create_arg`` names what the second argument of ``sim_create`` means for this backend:
+    the storage precision for the GPU library, the worker-thread count for the oracle.
+    """
+
+    def __init__(self, lib: C.CDLL, prefix: str, create_arg: str):
+        self.lib = lib
+        self.prefix = prefix
+        self.create_arg = create_arg
+        for name, res, args in _SIGS:
+            if not hasattr(lib, prefix + name):
+                continue  # the oracle has no device-timing entry points
+            fn = getattr(lib, prefix + name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name, fn)
+
+    def check(self, status: int):
+        if status != OK:
+            msg = self.last_error().decode(errors="replace")
+            if status == ENONFINITE:
+                raise Diverged(status, msg)
+            raise VortexError(status, msg)
+
+
+# ---------------------------------------------------------------------------------------
+# Shapes (src/geometry.rs)
+# ---------------------------------------------------------------------------------------
+@dataclass
+class ShapeSpec:
+    """A shape description; materialised into a library handle when used."""
+
+    kind: str
+    args: tuple
+    inner: Optional["ShapeSpec"] = None
+
+    def make(self, be: Backend):
+        k, a = self.kind, self.args
+        if k == "circle":
+            return be.shape_circle(*a)
+        if k == "polygon":
+            pts = np.ascontiguousarray(np.asarray(a[0], dtype=np.float64).reshape(-1, 2))
+            return be.shape_polygon(_dptr(pts), pts.shape[0])
+        if k == "rounded_square":
+            return be.shape_rounded_square(*a)
+        if k == "rotated":
+            return be.shape_rotated(self.inner.make(be), *a)
+        if k == "translated":
+            return be.shape_translated(self.inner.make(be), *a)
+        if k == "section":
+            return be.shape_section(*a)
+        if k == "outside_band":
+            return be.shape_outside_band(*a)
+        if k == "callback":
+            fn = a[0]
+            cb = INSIDE_FN(lambda _ctx, x, y: 1 if fn(x, y) else 0)
+            _KEEP.append(cb)
+            return be.shape_callback(cb, None)
+        raise ValueError(k)
+
+
+_KEEP: list = []  # ctypes callbacks must outlive the handles that call them
+
+
+def circle(cx, cy, r):
+    return ShapeSpec("circle", (float(cx), float(cy), float(r)))
+
+
+def polygon(pts):
+    return ShapeSpec("polygon", (np.asarray(pts, dtype=np.float64),))
+
+
+def rounded_square(cx, cy, half, r):
+    return ShapeSpec("rounded_square", (float(cx), float(cy), float(half), float(r)))
+
+
+def rotated(inner: ShapeSpec, cx, cy, angle):
+    return ShapeSpec("rotated", (float(cx), float(cy), float(angle)), inner)
+
+
+def translated(inner: ShapeSpec, dx, dy):
+    return ShapeSpec("translated", (float(dx), float(dy)), inner)
+
+
+def section(corner: str, b: float, cx, cy, d, angle=0.0):
+    return ShapeSpec("section", (CORNER[corner], float(b), float(cx), float(cy), float(d), float(angle)))
+
+
+def outside_band(y0, y1):
+    return ShapeSpec("outside_band", (float(y0), float(y1)))
+
+
+def shape_from(fn: Callable[[float, float], bool]):
+    return ShapeSpec("callback", (fn,))
+
+
+# ---------------------------------------------------------------------------------------
+# Wall velocities (the reference's VelocityFn closures)
+# ---------------------------------------------------------------------------------------
+def vel_const(ux, uy):
+    return ("const", float(ux), float(uy))
+
+
+def vel_parabola(u_mean, h):
+    return ("parabola", float(u_mean), float(h))
+
+
+def vel_spin(cx, cy):
+    return ("spin", float(cx), float(cy))
+
+
+def vel_fn(fn: Callable[[float, float], Sequence[float]]):
+    return ("fn", fn)
+
+
+def _velocity_desc(v) -> Optional[VelocityDesc]:
+    if v is None:
+        return None
+    d = VelocityDesc()
+    if v[0] == "const":
+        d.kind, d.p[0], d.p[1] = VEL_CONST, v[1], v[2]
+    elif v[0] == "parabola":
+        d.kind, d.p[0], d.p[1] = VEL_PARABOLA, v[1], v[2]
+    elif v[0] == "spin":
+        d.kind, d.p[0], d.p[1] = VEL_SPIN, v[1], v[2]
+    elif v[0] == "fn":
+        fn = v[1]
+
+        def tramp(_ctx, x, y, out):
+            u = fn(x, y)
+            out[0], out[1] = float(u[0]), float(u[1])
+
+        cb = VELOCITY_FN(tramp)
+        _KEEP.append(cb)
+        d.kind, d.fn = VEL_CALLBACK, cb
+    else:
+        raise ValueError(v)
+    return d
+
+
+# ---------------------------------------------------------------------------------------
+# Builder / Sim (src/sim.rs)
+# ---------------------------------------------------------------------------------------
+class Builder:
+    """Describes a domain before the solver is built (src/sim.rs:70-137)."""
+
+    def __init__(self, be: Backend, nx: int, ny: int):
+        self.be = be
+        self.h = be.builder_new(nx, ny)
+        if not self.h:
+            be.check(EINVAL)
+        self.nx, self.ny = nx, ny
+        self.periodic_x = False
+        self.periodic_y = False
+        self.tau = 1.0
+        self.collision = TRT
+        self.magic = 3.0 / 16.0
+        self.accel = (0.0, 0.0)
+
+    def _kind(self, k: int) -> int:
+        if k < 0:
+            self.be.check(EINVAL)
+        return k
+
+    def add_wall(self, velocity=None, vgroup: int = 0, fgroup: Optional[int] = None,
+                 shape: Optional[ShapeSpec] = None) -> int:
+        d = _velocity_desc(velocity)
+        sh = shape.make(self.be) if shape is not None else None
+        return self._kind(self.be.builder_add_wall(self.h, C.byref(d) if d is not None else None, vgroup,
+                                                   -1 if fgroup is None else fgroup, sh))
+
+    def add_static_wall(self) -> int:
+        return self.add_wall()
+
+    def add_pressure(self, rho: float) -> int:
+        return self._kind(self.be.builder_add_pressure(self.h, rho))
+
+    def add_radiating(self, rho: float) -> int:
+        return self._kind(self.be.builder_add_radiating(self.h, rho))
+
+    def fill_rect(self, x0, x1, y0, y1, kind):
+        self.be.check(self.be.builder_fill_rect(self.h, x0, x1, y0, y1, kind))
+
+    def fill_shape(self, kind):
+        self.be.check(self.be.builder_fill_shape(self.h, kind))
+
+    def build(self, arg: int = 0) -> "Sim":
+        """Build the solver.  ``arg`` is the precision (GPU library) or thread count (oracle)."""
+        be = self.be
+        be.check(be.builder_config(self.h, int(self.periodic_x), int(self.periodic_y), self.tau, self.collision,
+                                   self.magic, self.accel[0], self.accel[1]))
+        out = C.c_void_p()
+        h, self.h = self.h, None
+        be.check(be.sim_create(h, arg, C.byref(out)))
+        return Sim(be, out.value)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.be.builder_free(self.h)
+            self.h = None
+
+
+class Sim:
+    """A built domain: populations, boundary tables and the step (src/sim.rs:180-673)."""
+
+    def __init__(self, be: Backend, handle, owned: bool = True):
+        self.be, self.h, self.owned = be, handle, owned
+        self.vscale = [1.0, 1.0, 1.0, 1.0]
+        self.outflow_target: Optional[float] = None
+        i = self.info()
+        self.nx, self.ny = int(i.nx), int(i.ny)
+        self.n = self.nx * self.ny
+
+    def info(self) -> SimInfo:
+        i = SimInfo()
+        self.be.check(self.be.sim_info_get(self.h, C.byref(i)))
+        return i
+
+    @property
+    def t(self) -> int:
+        return int(self.info().t)
+
+    @property
+    def fluid_nodes(self) -> int:
+        return int(self.info().fluid_nodes)
+
+    @property
+    def link_count(self) -> int:
+        return int(self.info().link_count)
+
+    @property
+    def outlet_density_offset(self) -> float:
+        return float(self.info().outlet_density_offset)
+
+    def _push_controls(self):
+        v = np.asarray(self.vscale, dtype=np.float64)
+        self.be.check(self.be.sim_set_vscale(self.h, _dptr(v)))
+        tgt = self.outflow_target
+        self.be.check(self.be.sim_set_outflow_target(self.h, int(tgt is not None), float(tgt or 0.0)))
+
+    def step(self, n: int = 1):
+        """``n`` time steps with the current ``vscale`` and ``outflow_target``."""
+        self._push_controls()
+        self.be.check(self.be.sim_step(self.h, n))
+
+    def run(self, ctl: np.ndarray, want_force: bool = True) -> Optional[np.ndarray]:
+        """Steps with per-step controls: ``ctl`` is a structured array of ``Ctl`` records."""
+        k = len(ctl)
+        ng = int(self.info().force_groups)
+        hist = np.zeros((k, max(ng, 1), 2)) if want_force else None
+        arr = (Ctl * k)(*ctl)
+        self.be.check(self.be.sim_run(self.h, k, arr, _dptr(hist) if want_force else None))
+        return hist[:, :ng] if want_force else None
+
+    def init(self, field: Callable[[int, int], tuple]):
+        """Sim::init: equilibrium of field(x, y) -> (rho, ux, uy) on every fluid node."""
+        rho = np.ones(self.n)
+        ux = np.zeros(self.n)
+        uy = np.zeros(self.n)
+        for y in range(self.ny):
+            for x in range(self.nx):
+                r, u, v = field(x, y)
+                k = y * self.nx + x
+                rho[k], ux[k], uy[k] = r, u, v
+        self.init_arrays(rho, ux, uy)
+
+    def init_arrays(self, rho, ux, uy):
+        rho, ux, uy = (np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), (self.ny, self.nx)))
+                       .reshape(-1).copy() for a in (rho, ux, uy))
+        self.be.check(self.be.sim_init_macro(self.h, _dptr(rho), _dptr(ux), _dptr(uy)))
+
+    @property
+    def force(self) -> np.ndarray:
+        ng = int(self.info().force_groups)
+        out = np.zeros((max(ng, 1), 2))
+        self.be.check(self.be.sim_get_force(self.h, _dptr(out)))
+        return out[:ng]
+
+    def read(self) -> np.ndarray:
+        """Populations as an array [9, ny, nx] (reference layout f[i*n + y*nx + x])."""
+        out = np.zeros((Q, self.ny, self.nx))
+        self.be.check(self.be.sim_read(self.h, _dptr(out)))
+        return out
+
+    def write(self, f: np.ndarray):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(Q, self.ny, self.nx)
+        self.be.check(self.be.sim_write(self.h, _dptr(f)))
+
+    def macros_all(self):
+        rho, ux, uy = (np.zeros((self.ny, self.nx)) for _ in range(3))
+        self.be.check(self.be.sim_macros(self.h, _dptr(rho), _dptr(ux), _dptr(uy)))
+        return rho, ux, uy
+
+    def macros(self, x: int, y: int):
+        rho, ux, uy = self.macros_all()
+        return float(rho[y, x]), float(ux[y, x]), float(uy[y, x])
+
+    def state_hash(self) -> int:
+        h = C.c_uint64()
+        self.be.check(self.be.sim_state_hash(self.h, C.byref(h)))
+        return int(h.value)
+
+    def total_mass(self) -> float:
+        v = C.c_double()
+        self.be.check(self.be.sim_total_mass(self.h, C.byref(v)))
+        return v.value
+
+    def total_momentum(self):
+        v = np.zeros(2)
+        self.be.check(self.be.sim_total_momentum(self.h, _dptr(v)))
+        return float(v[0]), float(v[1])
+
+    def max_speed(self) -> float:
+        v = C.c_double()
+        self.be.check(self.be.sim_max_speed(self.h, C.byref(v)))
+        return v.value
+
+    def set_timing(self, on: bool = True):
+        self.be.check(self.be.sim_set_timing(self.h, int(on)))
+
+    def timing(self):
+        """(bulk_ms, bulk_launches, total_ms, steps) since set_timing(True)."""
+        bms, tms = C.c_double(), C.c_double()
+        bn, st = C.c_int64(), C.c_int64()
+        self.be.check(self.be.sim_get_timing(self.h, C.byref(bms), C.byref(bn), C.byref(tms), C.byref(st)))
+        return bms.value, bn.value, tms.value, st.value
+
+    def close(self):
+        if self.h and self.owned:
+            self.be.sim_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_ctl(vscale_rows, targets=None) -> list:
+    """Per-step controls: vscale_rows [k][4]; targets [k] (None / NaN = no outflow target)."""
+    out = []
+    for k, v in enumerate(vscale_rows):
+        c = Ctl()
+        for g in range(4):
+            c.vscale[g] = float(v[g])
+        t = None if targets is None else targets[k]
+        if t is None or (isinstance(t, float) and math.isnan(t)):
+            c.has_target, c.target = 0, 0.0
+        else:
+            c.has_target, c.target = 1, float(t)
+        out.append(c)
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+# Channel (src/cases/channel.rs)
+# ---------------------------------------------------------------------------------------
+@dataclass
+class Kick:
+    cx: float
+    cy: float
+    omega: float
+    start: int
+    duration: int
+
+
+@dataclass
+class ChannelSpec:
+    nx: int
+    ny: int
+    inflow: int
+    u: float
+    nu: float
+    body: ShapeSpec
+    collision: int = TRT
+    magic: float = 3.0 / 16.0
+    ramp_steps: int = 0
+    kick: Optional[Kick] = None
+
+
+class Channel:
+    """A body in a channel with its per-step driver (src/cases/channel.rs:78-155)."""
+
+    def __init__(self, be: Backend, spec: ChannelSpec, arg: int = 0):
+        self.be = be
+        d = ChannelSpecDesc()
+        d.nx, d.ny, d.inflow, d.collision = spec.nx, spec.ny, spec.inflow, spec.collision
+        d.u, d.nu, d.magic, d.ramp_steps = spec.u, spec.nu, spec.magic, spec.ramp_steps
+        if spec.kick is not None:
+            d.has_kick = 1
+            k = spec.kick
+            d.kick.cx, d.kick.cy, d.kick.omega, d.kick.start, d.kick.duration = (k.cx, k.cy, k.omega, k.start,
+                                                                                 k.duration)
+        out = C.c_void_p()
+        be.check(be.channel_create(C.byref(d), spec.body.make(be), arg, C.byref(out)))
+        self.h = out.value
+        self.u_ref = spec.u
+        self.sim = Sim(be, be.channel_sim(self.h), owned=False)
+
+    def set_absorbing(self, on: bool):
+        self.be.check(self.be.channel_set_absorbing(self.h, int(on)))
+
+    def run(self, steps: int, d: float) -> np.ndarray:
+        """``steps`` x Channel::step; returns the (cd, cl) history [steps, 2]."""
+        coef = np.zeros((steps, 2))
+        self.be.check(self.be.channel_run(self.h, steps, d, _dptr(coef)))
+        return coef
+
+    def coefficients(self, d: float):
+        cd, cl = C.c_double(), C.c_double()
+        self.be.check(self.be.channel_coefficients(self.h, d, C.byref(cd), C.byref(cl)))
+        return cd.value, cl.value
+
+    def close(self):
+        if self.h:
+            self.sim.h = None
+            self.be.channel_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------------------
+# Statistics (src/cases/channel.rs:157-207, src/signal.rs)
+# ---------------------------------------------------------------------------------------
+@dataclass
+class Stats:
+    cd_mean: float
+    cd_rms: float
+    cl_mean: float
+    cl_rms: float
+    st: float
+    st_crossings: float
+    periodic: bool
+
+
+def series_stats(be: Backend, cd, cl, dt: float, start: float) -> Stats:
+    cd = np.ascontiguousarray(cd, dtype=np.float64)
+    cl = np.ascontiguousarray(cl, dtype=np.float64)
+    s = StatsDesc()
+    be.check(be.series_stats(_dptr(cd), _dptr(cl), len(cd), dt, start, C.byref(s)))
+    return Stats(s.cd_mean, s.cd_rms, s.cl_mean, s.cl_rms, s.st, s.st_crossings, bool(s.periodic))
+
+
+def dominant_frequency(be: Backend, x, dt: float):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(2)
+    be.check(be.dominant_frequency(_dptr(x), len(x), dt, _dptr(out)))
+    return float(out[0]), float(out[1])
+
+
+def crossing_frequency(be: Backend, x, dt: float) -> Optional[float]:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    f, found = C.c_double(), C.c_int()
+    be.check(be.crossing_frequency(_dptr(x), len(x), dt, C.byref(f), C.byref(found)))
+    return f.value if found.value else None
+
+
+def fft(be: Backend, re, im):
+    re = np.array(re, dtype=np.float64)
+    im = np.array(im, dtype=np.float64)
+    be.check(be.fft(_dptr(re), _dptr(im), len(re)))
+    return re, im

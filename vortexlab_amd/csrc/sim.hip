@@ -1,0 +1,891 @@
+// One D2Q9 time step on the GPU, and the vl_sim C ABI.
+//
+// Per step (reference: Sim::step, src/sim.rs:391-420), in stream order:
+//   1. k_outlet  -- outlet-controller density offset from A (src/sim.rs:443-455).
+//   2. k_bulk    -- pull-stream + collide of every interior fluid node, A -> B
+//                   (src/sim.rs:700-735 with src/lattice.rs:80-156).
+//   3. k_links   -- boundary nodes: pull, apply the link rules, collide, A -> B; each force
+//                   link writes its momentum-exchange term (src/sim.rs:456-507).
+//   4. k_force   -- fixed-order FP64 tree sum of the force terms per force group.
+// then A and B swap.  Kernels 2 and 3 write disjoint nodes of B and read only A, exactly the
+// independence the reference's pull scheme guarantees (src/sim.rs:5-9).
+//
+// Storage classes: F64 keeps f_i and uses the reference's operation order, so with
+// -ffp-contract=off the populations are bit-identical to the reference.  F32 / F16 keep the
+// deviation d_i = f_i - w_i (F16 scaled by 2^10) and collide in FP32.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "lbm_math.h"
+#include "sim_internal.h"
+
+namespace vl {
+
+// ------------------------------------------------------------------------------------------
+// Storage classes
+// ------------------------------------------------------------------------------------------
+struct StoreF64 {
+  using T = double;
+  using C = double;
+  static constexpr bool exact = true;
+  __device__ static inline C ld(const T* p) { return *p; }
+  __device__ static inline void st(T* p, C v) { *p = v; }
+};
+struct StoreF32 {
+  using T = float;
+  using C = float;
+  static constexpr bool exact = false;
+  __device__ static inline C ld(const T* p) { return *p; }
+  __device__ static inline void st(T* p, C v) { *p = v; }
+};
+// FP16 deviations scaled by 2^10: keeps |d| ~ 1e-6..1e-1 inside the normal half range.
+constexpr float F16_SCALE = 1024.0f;
+constexpr float F16_INV = 1.0f / 1024.0f;
+struct StoreF16 {
+  using T = _Float16;
+  using C = float;
+  static constexpr bool exact = false;
+  __device__ static inline C ld(const T* p) { return (float)*p * F16_INV; }
+  __device__ static inline void st(T* p, C v) { *p = (_Float16)(v * F16_SCALE); }
+};
+
+template <class S, int M>
+__device__ inline void collide(typename S::C* f, const Coll& c) {
+  if constexpr (S::exact)
+    Exact::collide<M>(f, c.op, c.om, c.gx, c.gy);
+  else
+    Dev::collide<M>(f, c.opf, c.omf, c.gxf, c.gyf);
+}
+
+__device__ inline int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// Pull the nine populations of node (x, y) from their upstream neighbours in A.
+template <class S>
+__device__ inline void pull(const typename S::T* __restrict__ A, const Geom& g, int x, int y,
+                            typename S::C* f) {
+  const int xm = wrapi(x - 1, g.nx), xp = wrapi(x + 1, g.nx);
+  const int ym = wrapi(y - 1, g.ny), yp = wrapi(y + 1, g.ny);
+  const int64_t r0 = (int64_t)y * g.pitch, rm = (int64_t)ym * g.pitch, rp = (int64_t)yp * g.pitch;
+  const int64_t P = g.plane;
+  f[0] = S::ld(A + r0 + x);
+  f[1] = S::ld(A + P + r0 + xm);
+  f[2] = S::ld(A + 2 * P + rm + x);
+  f[3] = S::ld(A + 3 * P + r0 + xp);
+  f[4] = S::ld(A + 4 * P + rp + x);
+  f[5] = S::ld(A + 5 * P + rm + xm);
+  f[6] = S::ld(A + 6 * P + rm + xp);
+  f[7] = S::ld(A + 7 * P + rp + xp);
+  f[8] = S::ld(A + 8 * P + rp + xm);
+}
+
+// ------------------------------------------------------------------------------------------
+// Bulk: one thread per node, rows of 256 nodes per workgroup.
+// ------------------------------------------------------------------------------------------
+constexpr int BULK_X = 256;
+
+template <class S, int M>
+__global__ __launch_bounds__(BULK_X) void k_bulk(const typename S::T* __restrict__ A,
+                                                 typename S::T* __restrict__ B,
+                                                 const uint8_t* __restrict__ flag, Geom g, Coll c,
+                                                 int xblocks) {
+  const int y = blockIdx.x / xblocks;
+  const int x = (blockIdx.x - y * xblocks) * BULK_X + threadIdx.x;
+  if (x >= g.nx) return;
+  const int64_t idx = (int64_t)y * g.pitch + x;
+  if (flag[idx] != 0) return;
+  typename S::C f[Q];
+  pull<S>(A, g, x, y, f);
+  collide<S, M>(f, c);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) S::st(B + i * g.plane + idx, f[i]);
+}
+
+// ------------------------------------------------------------------------------------------
+// Outlet controller (src/sim.rs:443-455).  FP64 storage: the reference's serial left-to-right
+// sum in pressure-node creation order (it feeds back into every later step, so the order
+// matters for bit equality).  Deviation storage: a fixed-order tree (reproducible run to run).
+// ------------------------------------------------------------------------------------------
+constexpr int OUT_T = 256;
+
+template <class S>
+__device__ inline double normal_velocity(const typename S::T* __restrict__ A, const Geom& g,
+                                         const Coll& c, int x, int y, int nx_, int ny_) {
+  const int64_t idx = (int64_t)y * g.pitch + x;
+  if constexpr (S::exact) {
+    double f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = A[i * g.plane + idx];
+    double rho, jx, jy;
+    Exact::moments(f, rho, jx, jy);
+    const double ux = jx / rho - 0.5 * c.gx, uy = jy / rho - 0.5 * c.gy;
+    return (double)nx_ * ux + (double)ny_ * uy;
+  } else {
+    float d[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) d[i] = S::ld(A + i * g.plane + idx);
+    float drho, jx, jy;
+    Dev::moments(d, drho, jx, jy);
+    const double rho = 1.0 + (double)drho;
+    const double ux = (double)jx / rho - 0.5 * c.gx, uy = (double)jy / rho - 0.5 * c.gy;
+    return (double)nx_ * ux + (double)ny_ * uy;
+  }
+}
+
+template <class S>
+__global__ __launch_bounds__(OUT_T) void k_outlet(const typename S::T* __restrict__ A, Geom g, Coll c,
+                                                  const int32_t* __restrict__ px,
+                                                  const int32_t* __restrict__ py,
+                                                  const int8_t* __restrict__ pnx,
+                                                  const int8_t* __restrict__ pny, int np,
+                                                  double target, double sqrt_cs2,
+                                                  double* __restrict__ drho_out) {
+  __shared__ double v[OUT_T];
+  double sum = 0.0;  // running serial sum (exact) or this thread's strided partial (tree)
+  for (int base = 0; base < np; base += OUT_T) {
+    const int k = base + threadIdx.x;
+    const double val = k < np ? normal_velocity<S>(A, g, c, px[k], py[k], pnx[k], pny[k]) : 0.0;
+    if constexpr (S::exact) {
+      v[threadIdx.x] = val;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const int m = min(OUT_T, np - base);
+        for (int i = 0; i < m; ++i) sum += v[i];
+      }
+      __syncthreads();
+    } else {
+      sum += val;
+    }
+  }
+  if constexpr (!S::exact) {
+    v[threadIdx.x] = sum;
+    __syncthreads();
+    for (int s = OUT_T / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) v[threadIdx.x] += v[threadIdx.x + s];
+      __syncthreads();
+    }
+    sum = v[0];
+  }
+  if (threadIdx.x == 0) *drho_out = (sum / (double)np - target) / sqrt_cs2;
+}
+
+// ------------------------------------------------------------------------------------------
+// Boundary nodes (src/sim.rs:456-507): one thread per node, its links in link order.
+// ------------------------------------------------------------------------------------------
+constexpr int LINK_T = 128;
+
+template <class S, int M>
+__global__ __launch_bounds__(LINK_T) void k_links(const typename S::T* __restrict__ A,
+                                                  typename S::T* __restrict__ B, Geom g, Coll c,
+                                                  DevTables t, StepCtl ctl, double sqrt_cs2) {
+  using C = typename S::C;
+  const int b = blockIdx.x * LINK_T + threadIdx.x;
+  if (b >= t.nb) return;
+  const int x = t.bx[b], y = t.by[b];
+  const int64_t idx = (int64_t)y * g.pitch + x;
+  C f[Q];
+  pull<S>(A, g, x, y, f);
+  const double drho = ctl.has_target ? *t.drho : 0.0;
+  // Local velocity at the node for anti-bounce-back links, computed once (same value the
+  // reference recomputes per link).
+  bool have_u = false;
+  double ux = 0.0, uy = 0.0;
+  C dr_local = 0;
+  const int l1 = t.bfirst[b + 1];
+  for (int l = t.bfirst[b]; l < l1; ++l) {
+    const int j = t.dir[l];
+    const int i = opp(j);
+    const C out = S::ld(A + j * g.plane + idx);
+    const double scale = ctl.vscale[t.vgroup[l]];
+    C back;
+    if (t.kind[l] == LINK_WALL) {
+      const double q = t.q[l];
+      if constexpr (S::exact) {
+        const double corr = t.cu6[l] * scale;
+        if (q >= 0.5) {
+          const double k = 0.5 / q;
+          back = k * (out - corr) + (1.0 - k) * A[i * g.plane + idx];
+        } else if (t.lflags[l] & 2) {
+          const int bxn = wrapi(x - cx(j), g.nx), byn = wrapi(y - cy(j), g.ny);
+          back = 2.0 * q * out + (1.0 - 2.0 * q) * A[j * g.plane + (int64_t)byn * g.pitch + bxn] - corr;
+        } else {
+          back = out - corr;
+        }
+      } else {
+        // The wall rule is affine with weights summing to one, so it holds unchanged for
+        // deviations (the w_i terms cancel).
+        const float corr = (float)(t.cu6[l] * scale);
+        const float qf = (float)q;
+        if (q >= 0.5) {
+          const float k = (float)(0.5 / q);
+          back = k * (out - corr) + (1.0f - k) * S::ld(A + i * g.plane + idx);
+        } else if (t.lflags[l] & 2) {
+          const int bxn = wrapi(x - cx(j), g.nx), byn = wrapi(y - cy(j), g.ny);
+          back = 2.0f * qf * out +
+                 (1.0f - 2.0f * qf) * S::ld(A + j * g.plane + (int64_t)byn * g.pitch + bxn) - corr;
+        } else {
+          back = out - corr;
+        }
+      }
+    } else {
+      if (!have_u) {
+        have_u = true;
+        if constexpr (S::exact) {
+          double fa[Q];
+#pragma unroll
+          for (int k = 0; k < Q; ++k) fa[k] = A[k * g.plane + idx];
+          double rho, jx, jy;
+          Exact::moments(fa, rho, jx, jy);
+          ux = jx / rho - 0.5 * c.gx;
+          uy = jy / rho - 0.5 * c.gy;
+        } else {
+          float da[Q];
+#pragma unroll
+          for (int k = 0; k < Q; ++k) da[k] = S::ld(A + k * g.plane + idx);
+          float dr, jx, jy;
+          Dev::moments(da, dr, jx, jy);
+          dr_local = dr;
+          const double rho = 1.0 + (double)dr;
+          ux = (double)jx / rho - 0.5 * c.gx;
+          uy = (double)jy / rho - 0.5 * c.gy;
+        }
+      }
+      const double cu = 3.0 * ((double)cx(j) * ux + (double)cy(j) * uy);
+      const double rw_off = (t.lflags[l] & 1) ? ((double)t.lnx[l] * ux + (double)t.lny[l] * uy) / sqrt_cs2 : drho;
+      if constexpr (S::exact) {
+        const double rho_w = t.rho[l] + rw_off;
+        back = -out + 2.0 * wgt(j) * rho_w * (1.0 + 0.5 * cu * cu - 1.5 * (ux * ux + uy * uy));
+      } else {
+        // back - w = -d_out + 2 w (drho_w + rho_w (cu^2/2 - 1.5 u^2))
+        const double drw = (t.rho[l] - 1.0) + rw_off;
+        const double rho_w = 1.0 + drw;
+        back = (C)(-(double)out + 2.0 * wgt(j) * (drw + rho_w * (0.5 * cu * cu - 1.5 * (ux * ux + uy * uy))));
+      }
+    }
+    f[i] = back;
+    const int slot = t.fslot[l];
+    if (slot >= 0) {
+      const double wx = t.uwx[l] * scale, wy = t.uwy[l] * scale;
+      double o = (double)out, bk = (double)back;
+      if constexpr (!S::exact) {
+        o += wgt(j);
+        bk += wgt(j);
+      }
+      t.fcontrib[2 * slot] = (double)cx(j) * (o + bk) - wx * (o - bk);
+      t.fcontrib[2 * slot + 1] = (double)cy(j) * (o + bk) - wy * (o - bk);
+    }
+  }
+  (void)dr_local;
+  collide<S, M>(f, c);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) S::st(B + i * g.plane + idx, f[i]);
+}
+
+// ------------------------------------------------------------------------------------------
+// Force groups: fixed-order FP64 tree.  Thread k sums slots k, k+T, k+2T, ... in order, then a
+// fixed pairwise tree over the threads: the same association every step and every run.
+// ------------------------------------------------------------------------------------------
+constexpr int FORCE_T = 256;
+
+__global__ __launch_bounds__(FORCE_T) void k_force(const double* __restrict__ contrib,
+                                                   const int32_t* __restrict__ gfirst,
+                                                   double* __restrict__ out) {
+  __shared__ double sx[FORCE_T], sy[FORCE_T];
+  const int grp = blockIdx.x;
+  const int s0 = gfirst[grp], s1 = gfirst[grp + 1];
+  double ax = 0.0, ay = 0.0;
+  for (int s = s0 + (int)threadIdx.x; s < s1; s += FORCE_T) {
+    ax += contrib[2 * s];
+    ay += contrib[2 * s + 1];
+  }
+  sx[threadIdx.x] = ax;
+  sy[threadIdx.x] = ay;
+  __syncthreads();
+  for (int w = FORCE_T / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sx[threadIdx.x] += sx[threadIdx.x + w];
+      sy[threadIdx.x] += sy[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * grp] = sx[0];
+    out[2 * grp + 1] = sy[0];
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------------
+static const double SQRT_CS2 = std::sqrt(1.0 / 3.0);
+
+template <class S, int M>
+static vl_status launch_step(vl_sim* s, const StepCtl& ctl, double* force_out) {
+  using T = typename S::T;
+  const T* A = (const T*)s->buf[s->cur];
+  T* B = (T*)s->buf[s->cur ^ 1];
+  const Geom& g = s->g;
+  DevTables& t = s->dt;
+  if (ctl.has_target && t.np > 0) {
+    hipLaunchKernelGGL(k_outlet<S>, dim3(1), dim3(OUT_T), 0, s->stream, A, g, s->coll, t.px, t.py, t.pnx,
+                       t.pny, t.np, ctl.target, SQRT_CS2, t.drho);
+  }
+  const int xblocks = (g.nx + BULK_X - 1) / BULK_X;
+  const bool timed = s->timing && s->ev_used + 2 <= (int64_t)s->ev.size();
+  if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
+  hipLaunchKernelGGL((k_bulk<S, M>), dim3((unsigned)(xblocks * g.ny)), dim3(BULK_X), 0, s->stream, A, B,
+                     s->dflag, g, s->coll, xblocks);
+  if (timed) {
+    VL_HIP(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
+    s->ev_used += 2;
+  }
+  if (t.nb > 0) {
+    StepCtl c2 = ctl;
+    if (t.np == 0) c2.has_target = 0;
+    hipLaunchKernelGGL((k_links<S, M>), dim3((unsigned)((t.nb + LINK_T - 1) / LINK_T)), dim3(LINK_T), 0,
+                       s->stream, A, B, g, s->coll, t, c2, SQRT_CS2);
+  }
+  if (t.ngroups > 0 && force_out) {
+    hipLaunchKernelGGL(k_force, dim3((unsigned)t.ngroups), dim3(FORCE_T), 0, s->stream, t.fcontrib, t.gfirst,
+                       force_out);
+  }
+  VL_HIP(hipGetLastError());
+  s->cur ^= 1;
+  s->t += 1;
+  return VL_OK;
+}
+
+template <class S>
+static vl_status step_dispatch(vl_sim* s, const StepCtl& ctl, double* force_out) {
+  switch (s->coll.model) {
+    case MODEL_FORCED: return launch_step<S, MODEL_FORCED>(s, ctl, force_out);
+    case MODEL_BGK: return launch_step<S, MODEL_BGK>(s, ctl, force_out);
+    default: return launch_step<S, MODEL_TRT>(s, ctl, force_out);
+  }
+}
+
+static vl_status one_step(vl_sim* s, const StepCtl& ctl, double* force_out) {
+  switch (s->precision) {
+    case VL_F32: return step_dispatch<StoreF32>(s, ctl, force_out);
+    case VL_F16: return step_dispatch<StoreF16>(s, ctl, force_out);
+    default: return step_dispatch<StoreF64>(s, ctl, force_out);
+  }
+}
+
+template <class V>
+static vl_status upload(vl_sim* s, V*& dst, const std::vector<V>& src) {
+  if (src.empty()) return VL_OK;
+  const size_t bytes = src.size() * sizeof(V);
+  VL_HIP(hipMalloc((void**)&dst, bytes));
+  s->device_bytes += (int64_t)bytes;
+  VL_HIP(hipMemcpy(dst, src.data(), bytes, hipMemcpyHostToDevice));
+  return VL_OK;
+}
+
+#define VL_TRY(x)                 \
+  do {                            \
+    vl_status st_ = (x);          \
+    if (st_ != VL_OK) return st_; \
+  } while (0)
+
+static vl_status ensure_hist(vl_sim* s, int64_t steps) {
+  if (steps <= s->hist_cap) return VL_OK;
+  if (s->dhist) VL_HIP(hipFree(s->dhist));
+  s->dhist = nullptr;
+  const int64_t ng = std::max<int64_t>(s->dt.ngroups, 1);
+  VL_HIP(hipMalloc((void**)&s->dhist, (size_t)(steps * ng * 2) * sizeof(double)));
+  s->hist_cap = steps;
+  return VL_OK;
+}
+
+// Equilibrium populations of (rho, ux, uy) stored in the sim's precision at node (x, y).
+static void host_store_eq(vl_sim* s, std::vector<char>& host, int64_t x, int64_t y, double rho, double ux,
+                          double uy) {
+  double feq[Q];
+  Exact::equilibrium(rho, ux, uy, feq);
+  const int64_t off = y * s->g.pitch + x;
+  for (int i = 0; i < Q; ++i) {
+    const int64_t e = i * s->g.plane + off;
+    if (s->precision == VL_F64)
+      ((double*)host.data())[e] = feq[i];
+    else if (s->precision == VL_F32)
+      ((float*)host.data())[e] = (float)(feq[i] - wgt(i));
+    else
+      ((_Float16*)host.data())[e] = (_Float16)((float)(feq[i] - wgt(i)) * F16_SCALE);
+  }
+}
+
+static double host_load(const vl_sim* s, const std::vector<char>& host, int i, int64_t off) {
+  const int64_t e = i * s->g.plane + off;
+  if (s->precision == VL_F64) return ((const double*)host.data())[e];
+  if (s->precision == VL_F32) return wgt(i) + (double)((const float*)host.data())[e];
+  return wgt(i) + (double)((float)((const _Float16*)host.data())[e] * F16_INV);
+}
+
+static vl_status read_raw(vl_sim* s, std::vector<char>& host) {
+  host.resize(s->pop_bytes);
+  VL_HIP(hipStreamSynchronize(s->stream));
+  VL_HIP(hipMemcpy(host.data(), s->buf[s->cur], s->pop_bytes, hipMemcpyDeviceToHost));
+  return VL_OK;
+}
+
+static vl_status write_raw(vl_sim* s, const std::vector<char>& host) {
+  VL_HIP(hipStreamSynchronize(s->stream));
+  VL_HIP(hipMemcpy(s->buf[0], host.data(), s->pop_bytes, hipMemcpyHostToDevice));
+  VL_HIP(hipMemcpy(s->buf[1], host.data(), s->pop_bytes, hipMemcpyHostToDevice));
+  return VL_OK;
+}
+
+static void free_sim(vl_sim* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(s->buf[0]);
+  f(s->buf[1]);
+  f(s->dflag);
+  DevTables& t = s->dt;
+  for (void* p : {(void*)t.bx, (void*)t.by, (void*)t.bfirst, (void*)t.dir, (void*)t.kind, (void*)t.vgroup,
+                  (void*)t.lflags, (void*)t.lnx, (void*)t.lny, (void*)t.fslot, (void*)t.q, (void*)t.uwx,
+                  (void*)t.uwy, (void*)t.cu6, (void*)t.rho, (void*)t.px, (void*)t.py, (void*)t.pnx,
+                  (void*)t.pny, (void*)t.gfirst, (void*)t.fcontrib, (void*)t.drho})
+    f(p);
+  f(s->dhist);
+  for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : s->ev_chunk)
+    if (e) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
+  if (precision != VL_F64 && precision != VL_F32 && precision != VL_F16)
+    return set_error(VL_EINVAL, "precision must be VL_F64, VL_F32 or VL_F16");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(VL_EDEVICE, "no HIP device");
+  auto* s = new vl_sim;
+  std::unique_ptr<vl_sim, void (*)(vl_sim*)> guard(s, free_sim);
+  VL_TRY(build_tables(d, s->tab));
+  s->precision = precision;
+  s->collision = d.collision;
+  s->accel[0] = d.accel[0];
+  s->accel[1] = d.accel[1];
+  double op, om;
+  omegas(d.collision, d.magic, d.tau, op, om);
+  const bool forced = d.accel[0] != 0.0 || d.accel[1] != 0.0;  // src/sim.rs:395
+  s->coll = Coll{op, om, d.accel[0], d.accel[1], (float)op, (float)om, (float)d.accel[0], (float)d.accel[1],
+                 forced ? MODEL_FORCED : (d.collision == VL_BGK ? MODEL_BGK : MODEL_TRT)};
+  s->elem = precision == VL_F64 ? 8 : (precision == VL_F32 ? 4 : 2);
+  s->g.nx = (int32_t)d.nx;
+  s->g.ny = (int32_t)d.ny;
+  s->g.pitch = (int32_t)round_up(d.nx, 64);
+  s->g.periodic_x = d.periodic_x;
+  s->g.periodic_y = d.periodic_y;
+  // A gap of 5 x 512 B between planes so the nine streams do not start on the same HBM channel.
+  s->g.plane = round_up((int64_t)s->g.pitch * d.ny, 256) + 2560 / (int64_t)s->elem;
+  s->pop_bytes = (size_t)(Q * s->g.plane) * s->elem;
+  VL_HIP(hipGetDevice(&s->device));
+  VL_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  VL_HIP(hipMalloc(&s->buf[0], s->pop_bytes));
+  VL_HIP(hipMalloc(&s->buf[1], s->pop_bytes));
+  s->device_bytes += 2 * (int64_t)s->pop_bytes;
+  // flags in pitch layout (padding columns are "not fluid")
+  std::vector<uint8_t> pflag((size_t)s->g.pitch * d.ny, 2);
+  for (int64_t y = 0; y < d.ny; ++y)
+    std::memcpy(&pflag[y * s->g.pitch], &s->tab.flag[y * d.nx], (size_t)d.nx);
+  VL_TRY(upload(s, s->dflag, pflag));
+  const Tables& tb = s->tab;
+  DevTables& t = s->dt;
+  t.nb = (int32_t)tb.bnode_x.size();
+  t.nl = (int32_t)tb.l_dir.size();
+  t.np = (int32_t)tb.p_x.size();
+  t.nslots = (int32_t)tb.f_link.size();
+  t.ngroups = tb.force_groups;
+  VL_TRY(upload(s, t.bx, tb.bnode_x));
+  VL_TRY(upload(s, t.by, tb.bnode_y));
+  VL_TRY(upload(s, t.bfirst, tb.bnode_first));
+  VL_TRY(upload(s, t.dir, tb.l_dir));
+  VL_TRY(upload(s, t.kind, tb.l_kind));
+  VL_TRY(upload(s, t.vgroup, tb.l_vgroup));
+  VL_TRY(upload(s, t.lflags, tb.l_flags));
+  VL_TRY(upload(s, t.lnx, tb.l_nx));
+  VL_TRY(upload(s, t.lny, tb.l_ny));
+  VL_TRY(upload(s, t.fslot, tb.l_fslot));
+  VL_TRY(upload(s, t.q, tb.l_q));
+  VL_TRY(upload(s, t.uwx, tb.l_uwx));
+  VL_TRY(upload(s, t.uwy, tb.l_uwy));
+  VL_TRY(upload(s, t.cu6, tb.l_cu6));
+  VL_TRY(upload(s, t.rho, tb.l_rho));
+  VL_TRY(upload(s, t.px, tb.p_x));
+  VL_TRY(upload(s, t.py, tb.p_y));
+  VL_TRY(upload(s, t.pnx, tb.p_nx));
+  VL_TRY(upload(s, t.pny, tb.p_ny));
+  VL_TRY(upload(s, t.gfirst, tb.f_group_first));
+  if (t.nslots > 0) {
+    VL_HIP(hipMalloc((void**)&t.fcontrib, (size_t)t.nslots * 2 * sizeof(double)));
+    VL_HIP(hipMemset(t.fcontrib, 0, (size_t)t.nslots * 2 * sizeof(double)));
+  }
+  VL_HIP(hipMalloc((void**)&t.drho, sizeof(double)));
+  VL_HIP(hipMemset(t.drho, 0, sizeof(double)));
+  s->last_force.assign((size_t)t.ngroups * 2, 0.0);
+  VL_TRY(ensure_hist(s, 1));
+  // rest state, as Sim::from_builder leaves it (src/sim.rs:352)
+  std::vector<char> host(s->pop_bytes, 0);
+  for (int64_t y = 0; y < d.ny; ++y)
+    for (int64_t x = 0; x < d.nx; ++x) host_store_eq(s, host, x, y, 1.0, 0.0, 0.0);
+  VL_TRY(write_raw(s, host));
+  s->t = 0;
+  *out = guard.release();
+  return VL_OK;
+}
+
+static constexpr int64_t RUN_CHUNK = 4096;
+
+static vl_status ensure_events(vl_sim* s) {
+  if (!s->ev_chunk[0]) {
+    VL_HIP(hipEventCreate(&s->ev_chunk[0]));
+    VL_HIP(hipEventCreate(&s->ev_chunk[1]));
+  }
+  while ((int64_t)s->ev.size() < 2 * RUN_CHUNK) {
+    hipEvent_t e;
+    VL_HIP(hipEventCreate(&e));
+    s->ev.push_back(e);
+  }
+  return VL_OK;
+}
+
+// Adds up the event pairs recorded in the chunk that just completed.
+static vl_status collect_times(vl_sim* s, int64_t steps) {
+  for (int64_t k = 0; k + 1 < s->ev_used; k += 2) {
+    float ms = 0.0f;
+    VL_HIP(hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]));
+    s->bulk_ms += ms;
+    s->bulk_launches += 1;
+  }
+  float ms = 0.0f;
+  VL_HIP(hipEventElapsedTime(&ms, s->ev_chunk[0], s->ev_chunk[1]));
+  s->total_ms += ms;
+  s->timed_steps += steps;
+  s->ev_used = 0;
+  return VL_OK;
+}
+
+static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist) {
+  if (nsteps < 0) return set_error(VL_EINVAL, "negative step count");
+  const int64_t ng = s->dt.ngroups;
+  const int64_t chunk = RUN_CHUNK;
+  VL_TRY(ensure_hist(s, std::min<int64_t>(std::max<int64_t>(nsteps, 1), chunk)));
+  if (s->timing) VL_TRY(ensure_events(s));
+  for (int64_t k0 = 0; k0 < nsteps; k0 += chunk) {
+    const int64_t m = std::min(chunk, nsteps - k0);
+    if (s->timing) {
+      s->ev_used = 0;
+      VL_HIP(hipEventRecord(s->ev_chunk[0], s->stream));
+    }
+    for (int64_t k = 0; k < m; ++k) {
+      StepCtl c;
+      if (ctl) {
+        for (int g = 0; g < 4; ++g) c.vscale[g] = ctl[k0 + k].vscale[g];
+        c.target = ctl[k0 + k].target;
+        c.has_target = ctl[k0 + k].has_target;
+      } else {
+        for (int g = 0; g < 4; ++g) c.vscale[g] = s->vscale[g];
+        c.target = s->target;
+        c.has_target = s->has_target;
+      }
+      VL_TRY(one_step(s, c, ng > 0 ? s->dhist + k * ng * 2 : nullptr));
+      if (k0 + k == nsteps - 1) {
+        s->has_target = c.has_target;
+        s->target = c.target;
+        for (int g = 0; g < 4; ++g) s->vscale[g] = c.vscale[g];
+      }
+    }
+    if (s->timing) VL_HIP(hipEventRecord(s->ev_chunk[1], s->stream));
+    if (ng > 0) {
+      if (force_hist)
+        VL_HIP(hipMemcpyAsync(force_hist + k0 * ng * 2, s->dhist, (size_t)(m * ng * 2) * sizeof(double),
+                              hipMemcpyDeviceToHost, s->stream));
+      VL_HIP(hipMemcpyAsync(s->last_force.data(), s->dhist + (m - 1) * ng * 2, (size_t)(ng * 2) * sizeof(double),
+                            hipMemcpyDeviceToHost, s->stream));
+    }
+    VL_HIP(hipStreamSynchronize(s->stream));
+    if (s->timing) VL_TRY(collect_times(s, m));
+  }
+  if (nsteps > 0 && s->has_target && s->dt.np > 0)
+    VL_HIP(hipMemcpy(&s->outlet_density_offset, s->dt.drho, sizeof(double), hipMemcpyDeviceToHost));
+  else if (nsteps > 0)
+    s->outlet_density_offset = 0.0;
+  return VL_OK;
+}
+
+}  // namespace vl
+
+using namespace vl;
+
+// ==========================================================================================
+// C ABI
+// ==========================================================================================
+extern "C" {
+
+int vl_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+vl_status vl_sim_create(vl_builder* b, int precision, vl_sim** out) {
+  if (!b || !out) {
+    if (b) vl_builder_free(b);
+    return set_error(VL_EINVAL, "null argument");
+  }
+  *out = nullptr;
+  vl_status st = create(b->d, precision, out);
+  vl_builder_free(b);
+  return st;
+}
+
+void vl_sim_destroy(vl_sim* s) { free_sim(s); }
+
+vl_status vl_sim_info_get(vl_sim* s, vl_sim_info* o) {
+  if (!s || !o) return set_error(VL_EINVAL, "null argument");
+  std::memset(o, 0, sizeof(*o));
+  o->nx = s->g.nx;
+  o->ny = s->g.ny;
+  o->fluid_nodes = s->tab.fluid_nodes;
+  o->link_count = s->dt.nl;
+  o->bnode_count = s->dt.nb;
+  o->pnode_count = s->dt.np;
+  o->force_groups = s->dt.ngroups;
+  o->t = s->t;
+  o->precision = s->precision;
+  o->collision = s->collision;
+  o->omega_plus = s->coll.op;
+  o->omega_minus = s->coll.om;
+  o->outlet_density_offset = s->outlet_density_offset;
+  o->device_bytes = s->device_bytes;
+  return VL_OK;
+}
+
+vl_status vl_sim_init_macro(vl_sim* s, const double* rho, const double* ux, const double* uy) {
+  if (!s || !rho || !ux || !uy) return set_error(VL_EINVAL, "null argument");
+  std::vector<char> host(s->pop_bytes, 0);
+  const int64_t nx = s->g.nx;
+  for (int64_t y = 0; y < s->g.ny; ++y)
+    for (int64_t x = 0; x < nx; ++x) {
+      const int64_t k = y * nx + x;
+      if (s->tab.flag[k] == 2)
+        host_store_eq(s, host, x, y, 1.0, 0.0, 0.0);
+      else
+        host_store_eq(s, host, x, y, rho[k], ux[k], uy[k]);
+    }
+  VL_TRY(write_raw(s, host));
+  s->t = 0;
+  return VL_OK;
+}
+
+vl_status vl_sim_set_vscale(vl_sim* s, const double* v) {
+  if (!s || !v) return set_error(VL_EINVAL, "null argument");
+  for (int g = 0; g < 4; ++g) s->vscale[g] = v[g];
+  return VL_OK;
+}
+
+vl_status vl_sim_set_outflow_target(vl_sim* s, int enabled, double target) {
+  if (!s) return set_error(VL_EINVAL, "null argument");
+  s->has_target = enabled != 0;
+  s->target = target;
+  return VL_OK;
+}
+
+vl_status vl_sim_step(vl_sim* s, int64_t nsteps) {
+  if (!s) return set_error(VL_EINVAL, "null argument");
+  return run_steps(s, nsteps, nullptr, nullptr);
+}
+
+vl_status vl_sim_run(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist) {
+  if (!s || (!ctl && nsteps > 0)) return set_error(VL_EINVAL, "null argument");
+  return run_steps(s, nsteps, ctl, force_hist);
+}
+
+vl_status vl_sim_set_timing(vl_sim* s, int on) {
+  if (!s) return set_error(VL_EINVAL, "null argument");
+  s->timing = on != 0;
+  s->bulk_ms = s->total_ms = 0.0;
+  s->bulk_launches = s->timed_steps = 0;
+  s->ev_used = 0;
+  return VL_OK;
+}
+
+vl_status vl_sim_get_timing(vl_sim* s, double* bulk_ms, int64_t* bulk_launches, double* total_ms,
+                            int64_t* steps) {
+  if (!s || !bulk_ms || !bulk_launches || !total_ms || !steps) return set_error(VL_EINVAL, "null argument");
+  *bulk_ms = s->bulk_ms;
+  *bulk_launches = s->bulk_launches;
+  *total_ms = s->total_ms;
+  *steps = s->timed_steps;
+  return VL_OK;
+}
+
+vl_status vl_sim_get_force(vl_sim* s, double* out) {
+  if (!s || !out) return set_error(VL_EINVAL, "null argument");
+  std::copy(s->last_force.begin(), s->last_force.end(), out);
+  return VL_OK;
+}
+
+vl_status vl_sim_read(vl_sim* s, double* f) {
+  if (!s || !f) return set_error(VL_EINVAL, "null argument");
+  std::vector<char> host;
+  VL_TRY(read_raw(s, host));
+  const int64_t nx = s->g.nx, ny = s->g.ny, n = nx * ny;
+  for (int i = 0; i < Q; ++i)
+    for (int64_t y = 0; y < ny; ++y)
+      for (int64_t x = 0; x < nx; ++x) f[i * n + y * nx + x] = host_load(s, host, i, y * s->g.pitch + x);
+  return VL_OK;
+}
+
+vl_status vl_sim_write(vl_sim* s, const double* f) {
+  if (!s || !f) return set_error(VL_EINVAL, "null argument");
+  std::vector<char> host(s->pop_bytes, 0);
+  const int64_t nx = s->g.nx, ny = s->g.ny, n = nx * ny;
+  for (int i = 0; i < Q; ++i)
+    for (int64_t y = 0; y < ny; ++y)
+      for (int64_t x = 0; x < nx; ++x) {
+        const int64_t e = i * s->g.plane + y * s->g.pitch + x;
+        const double v = f[i * n + y * nx + x];
+        if (s->precision == VL_F64)
+          ((double*)host.data())[e] = v;
+        else if (s->precision == VL_F32)
+          ((float*)host.data())[e] = (float)(v - wgt(i));
+        else
+          ((_Float16*)host.data())[e] = (_Float16)((float)(v - wgt(i)) * F16_SCALE);
+      }
+  return write_raw(s, host);
+}
+
+// Sim::macros (src/sim.rs:527-534) for every node.
+vl_status vl_sim_macros(vl_sim* s, double* rho, double* ux, double* uy) {
+  if (!s || !rho || !ux || !uy) return set_error(VL_EINVAL, "null argument");
+  std::vector<char> host;
+  VL_TRY(read_raw(s, host));
+  const int64_t nx = s->g.nx, ny = s->g.ny;
+  for (int64_t y = 0; y < ny; ++y)
+    for (int64_t x = 0; x < nx; ++x) {
+      const int64_t k = y * nx + x;
+      if (s->tab.flag[k] == 2) {
+        rho[k] = 1.0;
+        ux[k] = 0.0;
+        uy[k] = 0.0;
+        continue;
+      }
+      double f[Q];
+      for (int i = 0; i < Q; ++i) f[i] = host_load(s, host, i, y * s->g.pitch + x);
+      double r, jx, jy;
+      Exact::moments(f, r, jx, jy);
+      rho[k] = r;
+      ux[k] = jx / r - 0.5 * s->accel[0];
+      uy[k] = jy / r - 0.5 * s->accel[1];
+    }
+  return VL_OK;
+}
+
+// FNV-1a over the little-endian bytes of every fluid population, node-major, direction-minor
+// (src/sim.rs:654-668).
+vl_status vl_sim_state_hash(vl_sim* s, uint64_t* out) {
+  if (!s || !out) return set_error(VL_EINVAL, "null argument");
+  std::vector<char> host;
+  VL_TRY(read_raw(s, host));
+  uint64_t h = 0xcbf29ce484222325ull;
+  const int64_t nx = s->g.nx, ny = s->g.ny;
+  for (int64_t y = 0; y < ny; ++y)
+    for (int64_t x = 0; x < nx; ++x) {
+      if (s->tab.flag[y * nx + x] == 2) continue;
+      for (int i = 0; i < Q; ++i) {
+        const double v = host_load(s, host, i, y * s->g.pitch + x);
+        uint64_t bits;
+        std::memcpy(&bits, &v, 8);
+        for (int k = 0; k < 8; ++k) {
+          h ^= (bits >> (8 * k)) & 0xffu;
+          h *= 0x100000001b3ull;
+        }
+      }
+    }
+  *out = h;
+  return VL_OK;
+}
+
+namespace {
+struct Compensated {  // Kahan-Babuska-Neumaier (the reference's diagnostic sum, src/sim.rs:676-691)
+  double sum = 0.0, c = 0.0;
+  void add(double v) {
+    const double t = sum + v;
+    c += std::fabs(sum) >= std::fabs(v) ? (sum - t) + v : (v - t) + sum;
+    sum = t;
+  }
+  double value() const { return sum + c; }
+};
+}  // namespace
+
+vl_status vl_sim_total_mass(vl_sim* s, double* out) {
+  if (!s || !out) return set_error(VL_EINVAL, "null argument");
+  std::vector<char> host;
+  VL_TRY(read_raw(s, host));
+  Compensated m;
+  const int64_t nx = s->g.nx, ny = s->g.ny;
+  for (int64_t y = 0; y < ny; ++y)
+    for (int64_t x = 0; x < nx; ++x)
+      if (s->tab.flag[y * nx + x] != 2)
+        for (int i = 0; i < Q; ++i) m.add(host_load(s, host, i, y * s->g.pitch + x));
+  *out = m.value();
+  return VL_OK;
+}
+
+vl_status vl_sim_total_momentum(vl_sim* s, double* out) {
+  if (!s || !out) return set_error(VL_EINVAL, "null argument");
+  std::vector<char> host;
+  VL_TRY(read_raw(s, host));
+  Compensated px, py;
+  const int64_t nx = s->g.nx, ny = s->g.ny;
+  for (int64_t y = 0; y < ny; ++y)
+    for (int64_t x = 0; x < nx; ++x)
+      if (s->tab.flag[y * nx + x] != 2)
+        for (int i = 1; i < Q; ++i) {
+          const double f = host_load(s, host, i, y * s->g.pitch + x);
+          px.add((double)cx(i) * f);
+          py.add((double)cy(i) * f);
+        }
+  out[0] = px.value();
+  out[1] = py.value();
+  return VL_OK;
+}
+
+vl_status vl_sim_max_speed(vl_sim* s, double* out) {
+  if (!s || !out) return set_error(VL_EINVAL, "null argument");
+  std::vector<char> host;
+  VL_TRY(read_raw(s, host));
+  double m = 0.0;
+  const int64_t nx = s->g.nx, ny = s->g.ny;
+  for (int64_t y = 0; y < ny; ++y)
+    for (int64_t x = 0; x < nx; ++x) {
+      if (s->tab.flag[y * nx + x] == 2) continue;
+      double f[Q];
+      for (int i = 0; i < Q; ++i) f[i] = host_load(s, host, i, y * s->g.pitch + x);
+      double r, jx, jy;
+      Exact::moments(f, r, jx, jy);
+      const double ux = jx / r - 0.5 * s->accel[0], uy = jy / r - 0.5 * s->accel[1];
+      const double sp = std::sqrt(ux * ux + uy * uy);
+      if (std::isnan(sp)) {
+        *out = NAN;
+        return VL_OK;
+      }
+      m = std::max(m, sp);
+    }
+  *out = m;
+  return VL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Device-side simulation state shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "setup.h"
+
+namespace vl {
+
+#define VL_HIP(call)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return ::vl::set_error(VL_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Per-step controls as the kernels see them (a copy of vl_ctl).
+struct StepCtl {
+  double vscale[4];
+  double target;
+  int has_target;
+};
+
+// Collision parameters.
+struct Coll {
+  double op, om, gx, gy;
+  float opf, omf, gxf, gyf;
+  int model;  // Model
+};
+
+// Geometry of the device arrays: population i of node (x, y) lives at
+// base[i * plane + y * pitch + x]; pitch >= nx is padded for aligned rows and `plane`
+// leaves a gap between planes so the nine streams do not alias in the HBM channels.
+struct Geom {
+  int32_t nx, ny, pitch;
+  int32_t periodic_x, periodic_y;
+  int64_t plane;
+};
+
+struct DevTables {
+  int32_t nb = 0, nl = 0, np = 0, nslots = 0, ngroups = 0;
+  int32_t *bx = nullptr, *by = nullptr, *bfirst = nullptr;
+  uint8_t *dir = nullptr, *kind = nullptr, *vgroup = nullptr, *lflags = nullptr;
+  int8_t *lnx = nullptr, *lny = nullptr;
+  int32_t* fslot = nullptr;
+  double *q = nullptr, *uwx = nullptr, *uwy = nullptr, *cu6 = nullptr, *rho = nullptr;
+  int32_t *px = nullptr, *py = nullptr;
+  int8_t *pnx = nullptr, *pny = nullptr;
+  int32_t* gfirst = nullptr;   // ngroups+1 slot offsets
+  double* fcontrib = nullptr;  // nslots*2
+  double* drho = nullptr;      // 1 value: outlet density offset of the current step
+};
+
+}  // namespace vl
+
+struct vl_sim {
+  int precision = VL_F64;
+  vl::Tables tab;          // host copy (flags for readbacks, counts)
+  vl::Geom g{};
+  vl::Coll coll{};
+  int collision = VL_TRT;
+  double accel[2] = {0.0, 0.0};
+  double vscale[4] = {1.0, 1.0, 1.0, 1.0};
+  bool has_target = false;
+  double target = 0.0;
+  double outlet_density_offset = 0.0;
+  int64_t t = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* buf[2] = {nullptr, nullptr};  // population arrays; buf[cur] is A
+  int cur = 0;
+  size_t elem = 8;
+  size_t pop_bytes = 0;  // bytes of one population array
+  uint8_t* dflag = nullptr;
+  vl::DevTables dt;
+  double* dhist = nullptr;  // device force history (hist_cap steps)
+  int64_t hist_cap = 0;
+  std::vector<double> last_force;  // Sim::force after the last step
+  int64_t device_bytes = 0;
+  // Optional HIP-event timing of the bulk kernel and of whole steps (vl_sim_set_timing).
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // per step in a chunk: [bulk start, bulk end]
+  hipEvent_t ev_chunk[2] = {nullptr, nullptr};
+  int64_t ev_used = 0;
+  double bulk_ms = 0.0, total_ms = 0.0;
+  int64_t bulk_launches = 0, timed_steps = 0;
+};
