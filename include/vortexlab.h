@@ -176,6 +176,9 @@ vl_status vl_sim_max_speed(vl_sim* s, double* out);             /* sim.rs:566-57
  * the counters; bulk_ms sums the duration of every bulk stream-collide launch, total_ms the
  * duration of whole steps (every kernel of the step). */
 vl_status vl_sim_set_timing(vl_sim* s, int on);
+/* Best-of-`iters` bandwidth of a plain device-to-device copy kernel of `bytes` (read + write
+ * bytes counted): the achievable-HBM yardstick reported beside the kernel figures. */
+vl_status vl_device_copy_bandwidth(size_t bytes, int iters, double* gbs);
 vl_status vl_sim_get_timing(vl_sim* s, double* bulk_ms, int64_t* bulk_launches, double* total_ms,
                             int64_t* steps);
 

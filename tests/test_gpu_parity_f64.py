@@ -29,11 +29,13 @@ def assert_pops(g, o, label, exact=True):
     assert rel <= POP_RTOL, f"{label}: max rel diff {rel:.3e}"
 
 
-def assert_force(g, o, label, rtol=1e-9, atol=1e-13):
+def assert_force(g, o, label, rtol=1e-9):
+    """Forces within 1e-9 of the force scale of the run (the largest |F| in the history):
+    the tree and the serial sum differ only by rounding, which near a zero force is absolute."""
     g, o = np.asarray(g), np.asarray(o)
     err = np.abs(g - o)
-    lim = rtol * np.maximum(np.abs(o), 0.0) + atol
-    assert np.all(err <= lim), f"{label}: force {g} vs oracle {o} (err {err.max():.3e})"
+    scale = max(float(np.max(np.abs(o))), 1e-300)
+    assert np.all(err <= rtol * scale), f"{label}: max err {err.max():.3e} vs scale {scale:.3e}"
 
 
 @pytest.mark.parametrize("n,steps,golden,fluid", [(40, 3000, 0x6B6814B976BBBAFF, 143056),

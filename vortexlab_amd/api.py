@@ -120,6 +120,7 @@ _SIGS = [
     ("sim_total_momentum", C.c_int, [_VP, _D]),
     ("sim_max_speed", C.c_int, [_VP, _D]),
     ("sim_set_timing", C.c_int, [_VP, C.c_int]),
+    ("device_copy_bandwidth", C.c_int, [C.c_size_t, C.c_int, _D]),
     ("sim_get_timing", C.c_int, [_VP, _D, C.POINTER(C.c_int64), _D, C.POINTER(C.c_int64)]),
     ("channel_create", C.c_int, [C.POINTER(ChannelSpecDesc), _VP, C.c_int, C.POINTER(_VP)]),
     ("channel_destroy", None, [_VP]),

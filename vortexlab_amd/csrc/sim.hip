@@ -1,13 +1,14 @@
 // One D2Q9 time step on the GPU, and the vl_sim C ABI.
 //
-// Per step (reference: Sim::step, src/sim.rs:391-420), in stream order:
-//   1. k_outlet  -- outlet-controller density offset from A (src/sim.rs:443-455).
-//   2. k_bulk    -- pull-stream + collide of every interior fluid node, A -> B
-//                   (src/sim.rs:700-735 with src/lattice.rs:80-156).
-//   3. k_links   -- boundary nodes: pull, apply the link rules, collide, A -> B; each force
-//                   link writes its momentum-exchange term (src/sim.rs:456-507).
-//   4. k_force   -- fixed-order FP64 tree sum of the force terms per force group.
-// then A and B swap.  Kernels 2 and 3 write disjoint nodes of B and read only A, exactly the
+// A step (reference: Sim::step, src/sim.rs:391-420) is ONE launch of k_step:
+//   * boundary workgroups (lowest ids, dispatched first): the outlet-controller density
+//     offset from A (src/sim.rs:443-455), then one boundary node per thread -- pull, link
+//     rules, collide, A -> B, and each force link's momentum-exchange term
+//     (src/sim.rs:456-507); workgroup 0 also reduces the previous step's force terms with a
+//     fixed-order FP64 tree (double-buffered, so no cross-workgroup synchronisation);
+//   * bulk workgroups: pull-stream + collide of every interior fluid node, A -> B
+//     (src/sim.rs:700-735 with src/lattice.rs:80-156);
+// then A and B swap.  Both halves write disjoint nodes of B and read only A, exactly the
 // independence the reference's pull scheme guarantees (src/sim.rs:5-9).
 //
 // Storage classes: F64 keeps f_i and uses the reference's operation order, so with
@@ -17,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -83,37 +85,32 @@ __device__ inline void pull(const typename S::T* __restrict__ A, const Geom& g, 
 }
 
 // ------------------------------------------------------------------------------------------
-// Bulk: one thread per node, rows of 256 nodes per workgroup.
+// The fused step kernel.  One launch per time step; workgroups [0, nlb) handle the boundary
+// nodes (and the outlet controller and force reduction), the rest stream-collide the
+// interior.  Boundary workgroups have the lowest ids so they are dispatched first and finish
+// under the shadow of the bulk.
 // ------------------------------------------------------------------------------------------
-constexpr int BULK_X = 256;
+constexpr int STEP_T = 256;
 
-template <class S, int M>
-__global__ __launch_bounds__(BULK_X) void k_bulk(const typename S::T* __restrict__ A,
-                                                 typename S::T* __restrict__ B,
-                                                 const uint8_t* __restrict__ flag, Geom g, Coll c,
-                                                 int xblocks) {
-  const int y = blockIdx.x / xblocks;
-  const int x = (blockIdx.x - y * xblocks) * BULK_X + threadIdx.x;
-  if (x >= g.nx) return;
-  const int64_t idx = (int64_t)y * g.pitch + x;
-  if (flag[idx] != 0) return;
-  typename S::C f[Q];
-  pull<S>(A, g, x, y, f);
-  collide<S, M>(f, c);
-#pragma unroll
-  for (int i = 0; i < Q; ++i) S::st(B + i * g.plane + idx, f[i]);
-}
+struct StepArgs {
+  int nlb;        // boundary workgroups (a multiple of 8 when the XCD remap is on)
+  int xblocks;    // bulk workgroups per row
+  int nbulk;      // bulk workgroups that carry work
+  int per_xcd;    // bulk workgroups per XCD (remap on), else 0
+  double sqrt_cs2;
+  double* fc_cur;        // force terms written this step (per force slot: x, y)
+  const double* fc_prev; // force terms of the previous step
+  double* prev_out;      // where the previous step's group forces go (null: not wanted)
+};
 
-// ------------------------------------------------------------------------------------------
-// Outlet controller (src/sim.rs:443-455).  FP64 storage: the reference's serial left-to-right
-// sum in pressure-node creation order (it feeds back into every later step, so the order
-// matters for bit equality).  Deviation storage: a fixed-order tree (reproducible run to run).
-// ------------------------------------------------------------------------------------------
-constexpr int OUT_T = 256;
-
+// Outlet controller (src/sim.rs:443-455): the mean outward normal velocity over the
+// pressure-boundary nodes, as a density offset.  FP64 storage: the reference's serial
+// left-to-right sum in node creation order (it feeds back into every later step, so the
+// order matters for bit equality).  Deviation storage: a fixed-order tree.  Every thread of
+// the workgroup must call this; all get the result.
 template <class S>
-__device__ inline double normal_velocity(const typename S::T* __restrict__ A, const Geom& g,
-                                         const Coll& c, int x, int y, int nx_, int ny_) {
+__device__ inline double normal_velocity(const typename S::T* __restrict__ A, const Geom& g, const Coll& c,
+                                         int x, int y, int nx_, int ny_) {
   const int64_t idx = (int64_t)y * g.pitch + x;
   if constexpr (S::exact) {
     double f[Q];
@@ -136,64 +133,78 @@ __device__ inline double normal_velocity(const typename S::T* __restrict__ A, co
 }
 
 template <class S>
-__global__ __launch_bounds__(OUT_T) void k_outlet(const typename S::T* __restrict__ A, Geom g, Coll c,
-                                                  const int32_t* __restrict__ px,
-                                                  const int32_t* __restrict__ py,
-                                                  const int8_t* __restrict__ pnx,
-                                                  const int8_t* __restrict__ pny, int np,
-                                                  double target, double sqrt_cs2,
-                                                  double* __restrict__ drho_out) {
-  __shared__ double v[OUT_T];
-  double sum = 0.0;  // running serial sum (exact) or this thread's strided partial (tree)
-  for (int base = 0; base < np; base += OUT_T) {
-    const int k = base + threadIdx.x;
-    const double val = k < np ? normal_velocity<S>(A, g, c, px[k], py[k], pnx[k], pny[k]) : 0.0;
+__device__ double outlet_offset(const typename S::T* __restrict__ A, const Geom& g, const Coll& c,
+                                const DevTables& t, double target, double sqrt_cs2, double* red) {
+  double sum = 0.0;  // serial running sum (exact) or this thread's strided partial (tree)
+  for (int base = 0; base < t.np; base += STEP_T) {
+    const int k = base + (int)threadIdx.x;
+    const double v = k < t.np ? normal_velocity<S>(A, g, c, t.px[k], t.py[k], t.pnx[k], t.pny[k]) : 0.0;
     if constexpr (S::exact) {
-      v[threadIdx.x] = val;
+      red[threadIdx.x] = v;
       __syncthreads();
       if (threadIdx.x == 0) {
-        const int m = min(OUT_T, np - base);
-        for (int i = 0; i < m; ++i) sum += v[i];
+        const int m = min(STEP_T, t.np - base);
+        for (int i = 0; i < m; ++i) sum += red[i];
       }
       __syncthreads();
     } else {
-      sum += val;
+      sum += v;
     }
   }
   if constexpr (!S::exact) {
-    v[threadIdx.x] = sum;
+    red[threadIdx.x] = sum;
     __syncthreads();
-    for (int s = OUT_T / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) v[threadIdx.x] += v[threadIdx.x + s];
+    for (int w = STEP_T / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
       __syncthreads();
     }
-    sum = v[0];
+    sum = red[0];
+    __syncthreads();
   }
-  if (threadIdx.x == 0) *drho_out = (sum / (double)np - target) / sqrt_cs2;
+  if (threadIdx.x == 0) red[0] = (sum / (double)t.np - target) / sqrt_cs2;
+  __syncthreads();
+  const double drho = red[0];
+  __syncthreads();
+  return drho;
 }
 
-// ------------------------------------------------------------------------------------------
-// Boundary nodes (src/sim.rs:456-507): one thread per node, its links in link order.
-// ------------------------------------------------------------------------------------------
-constexpr int LINK_T = 128;
+// Force groups: fixed-order FP64 tree.  Thread k sums slots k, k+T, k+2T, ... in order, then
+// a fixed pairwise tree over the threads: the same association every step and every run.
+__device__ void reduce_forces(const double* __restrict__ fc, const int32_t* __restrict__ gfirst, int ngroups,
+                              double* __restrict__ out, double* red) {
+  for (int grp = 0; grp < ngroups; ++grp) {
+    const int s0 = gfirst[grp], s1 = gfirst[grp + 1];
+    for (int comp = 0; comp < 2; ++comp) {
+      double a = 0.0;
+      for (int s = s0 + (int)threadIdx.x; s < s1; s += STEP_T) a += fc[2 * s + comp];
+      red[threadIdx.x] = a;
+      __syncthreads();
+      for (int w = STEP_T / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) out[2 * grp + comp] = red[0];
+      __syncthreads();
+    }
+  }
+}
 
+// One boundary node (src/sim.rs:456-507): pull, replace the populations that would come out
+// of non-fluid cells by the link rules in link order, collide, write B; each force link
+// stores its momentum-exchange term.
 template <class S, int M>
-__global__ __launch_bounds__(LINK_T) void k_links(const typename S::T* __restrict__ A,
-                                                  typename S::T* __restrict__ B, Geom g, Coll c,
-                                                  DevTables t, StepCtl ctl, double sqrt_cs2) {
+__device__ void boundary_node(const typename S::T* __restrict__ A, typename S::T* __restrict__ B, const Geom& g,
+                              const Coll& c, const DevTables& t, const StepCtl& ctl, double drho, double sqrt_cs2,
+                              int b, double* __restrict__ fc) {
   using C = typename S::C;
-  const int b = blockIdx.x * LINK_T + threadIdx.x;
-  if (b >= t.nb) return;
   const int x = t.bx[b], y = t.by[b];
   const int64_t idx = (int64_t)y * g.pitch + x;
   C f[Q];
   pull<S>(A, g, x, y, f);
-  const double drho = ctl.has_target ? *t.drho : 0.0;
-  // Local velocity at the node for anti-bounce-back links, computed once (same value the
-  // reference recomputes per link).
+  // Local velocity for anti-bounce-back links, computed once (the reference recomputes the
+  // same value per link).
   bool have_u = false;
   double ux = 0.0, uy = 0.0;
-  C dr_local = 0;
   const int l1 = t.bfirst[b + 1];
   for (int l = t.bfirst[b]; l < l1; ++l) {
     const int j = t.dir[l];
@@ -224,8 +235,8 @@ __global__ __launch_bounds__(LINK_T) void k_links(const typename S::T* __restric
           back = k * (out - corr) + (1.0f - k) * S::ld(A + i * g.plane + idx);
         } else if (t.lflags[l] & 2) {
           const int bxn = wrapi(x - cx(j), g.nx), byn = wrapi(y - cy(j), g.ny);
-          back = 2.0f * qf * out +
-                 (1.0f - 2.0f * qf) * S::ld(A + j * g.plane + (int64_t)byn * g.pitch + bxn) - corr;
+          back = 2.0f * qf * out + (1.0f - 2.0f * qf) * S::ld(A + j * g.plane + (int64_t)byn * g.pitch + bxn) -
+                 corr;
         } else {
           back = out - corr;
         }
@@ -247,20 +258,19 @@ __global__ __launch_bounds__(LINK_T) void k_links(const typename S::T* __restric
           for (int k = 0; k < Q; ++k) da[k] = S::ld(A + k * g.plane + idx);
           float dr, jx, jy;
           Dev::moments(da, dr, jx, jy);
-          dr_local = dr;
           const double rho = 1.0 + (double)dr;
           ux = (double)jx / rho - 0.5 * c.gx;
           uy = (double)jy / rho - 0.5 * c.gy;
         }
       }
       const double cu = 3.0 * ((double)cx(j) * ux + (double)cy(j) * uy);
-      const double rw_off = (t.lflags[l] & 1) ? ((double)t.lnx[l] * ux + (double)t.lny[l] * uy) / sqrt_cs2 : drho;
+      const double off = (t.lflags[l] & 1) ? ((double)t.lnx[l] * ux + (double)t.lny[l] * uy) / sqrt_cs2 : drho;
       if constexpr (S::exact) {
-        const double rho_w = t.rho[l] + rw_off;
+        const double rho_w = t.rho[l] + off;
         back = -out + 2.0 * wgt(j) * rho_w * (1.0 + 0.5 * cu * cu - 1.5 * (ux * ux + uy * uy));
       } else {
         // back - w = -d_out + 2 w (drho_w + rho_w (cu^2/2 - 1.5 u^2))
-        const double drw = (t.rho[l] - 1.0) + rw_off;
+        const double drw = (t.rho[l] - 1.0) + off;
         const double rho_w = 1.0 + drw;
         back = (C)(-(double)out + 2.0 * wgt(j) * (drw + rho_w * (0.5 * cu * cu - 1.5 * (ux * ux + uy * uy))));
       }
@@ -274,47 +284,59 @@ __global__ __launch_bounds__(LINK_T) void k_links(const typename S::T* __restric
         o += wgt(j);
         bk += wgt(j);
       }
-      t.fcontrib[2 * slot] = (double)cx(j) * (o + bk) - wx * (o - bk);
-      t.fcontrib[2 * slot + 1] = (double)cy(j) * (o + bk) - wy * (o - bk);
+      fc[2 * slot] = (double)cx(j) * (o + bk) - wx * (o - bk);
+      fc[2 * slot + 1] = (double)cy(j) * (o + bk) - wy * (o - bk);
     }
   }
-  (void)dr_local;
   collide<S, M>(f, c);
 #pragma unroll
   for (int i = 0; i < Q; ++i) S::st(B + i * g.plane + idx, f[i]);
 }
 
-// ------------------------------------------------------------------------------------------
-// Force groups: fixed-order FP64 tree.  Thread k sums slots k, k+T, k+2T, ... in order, then a
-// fixed pairwise tree over the threads: the same association every step and every run.
-// ------------------------------------------------------------------------------------------
-constexpr int FORCE_T = 256;
-
-__global__ __launch_bounds__(FORCE_T) void k_force(const double* __restrict__ contrib,
-                                                   const int32_t* __restrict__ gfirst,
-                                                   double* __restrict__ out) {
-  __shared__ double sx[FORCE_T], sy[FORCE_T];
-  const int grp = blockIdx.x;
-  const int s0 = gfirst[grp], s1 = gfirst[grp + 1];
-  double ax = 0.0, ay = 0.0;
-  for (int s = s0 + (int)threadIdx.x; s < s1; s += FORCE_T) {
-    ax += contrib[2 * s];
-    ay += contrib[2 * s + 1];
-  }
-  sx[threadIdx.x] = ax;
-  sy[threadIdx.x] = ay;
-  __syncthreads();
-  for (int w = FORCE_T / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      sx[threadIdx.x] += sx[threadIdx.x + w];
-      sy[threadIdx.x] += sy[threadIdx.x + w];
+template <class S, int M>
+__global__ __launch_bounds__(STEP_T) void k_step(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
+                                                 const uint8_t* __restrict__ flag, Geom g, Coll c, DevTables t,
+                                                 StepCtl ctl, StepArgs sa) {
+  __shared__ double red[STEP_T];
+  const int blk = (int)blockIdx.x;
+  if (blk < sa.nlb) {
+    if (blk == 0 && sa.prev_out) reduce_forces(sa.fc_prev, t.gfirst, t.ngroups, sa.prev_out, red);
+    double drho = 0.0;
+    if (ctl.has_target && t.np > 0 && blk * STEP_T < t.npb) {
+      drho = outlet_offset<S>(A, g, c, t, ctl.target, sa.sqrt_cs2, red);
+      if (blk == 0 && threadIdx.x == 0) *t.drho = drho;
     }
-    __syncthreads();
+    const int b = blk * STEP_T + (int)threadIdx.x;
+    if (b < t.nb) boundary_node<S, M>(A, B, g, c, t, ctl, drho, sa.sqrt_cs2, b, sa.fc_cur);
+    return;
   }
-  if (threadIdx.x == 0) {
-    out[2 * grp] = sx[0];
-    out[2 * grp + 1] = sy[0];
+  int tile = blk - sa.nlb;
+  if (sa.per_xcd) {  // consecutive tiles on one XCD: its L2 sees whole row segments
+    tile = (tile & 7) * sa.per_xcd + (tile >> 3);
+    if (tile >= sa.nbulk) return;
   }
+  const int y = tile / sa.xblocks;
+  const int x = (tile - y * sa.xblocks) * STEP_T + (int)threadIdx.x;
+  if (x >= g.nx) return;
+  const int64_t idx = (int64_t)y * g.pitch + x;
+  if (flag[idx] != 0) return;
+  typename S::C f[Q];
+  pull<S>(A, g, x, y, f);
+  collide<S, M>(f, c);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) S::st(B + i * g.plane + idx, f[i]);
+}
+
+// Group forces of the last step of a run (the in-step reduction covers the ones before).
+__global__ __launch_bounds__(STEP_T) void k_force(const double* __restrict__ fc, const int32_t* __restrict__ gfirst,
+                                                  int ngroups, double* __restrict__ out) {
+  __shared__ double red[STEP_T];
+  reduce_forces(fc, gfirst, ngroups, out, red);
+}
+
+// Plain device copy, the bandwidth yardstick reported beside the kernel's own figure.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -322,35 +344,34 @@ __global__ __launch_bounds__(FORCE_T) void k_force(const double* __restrict__ co
 // ------------------------------------------------------------------------------------------
 static const double SQRT_CS2 = std::sqrt(1.0 / 3.0);
 
+// Launch one step.  `prev_out`: where the previous step's group forces go (or null).
 template <class S, int M>
-static vl_status launch_step(vl_sim* s, const StepCtl& ctl, double* force_out) {
+static vl_status launch_step(vl_sim* s, const StepCtl& ctl, double* prev_out) {
   using T = typename S::T;
   const T* A = (const T*)s->buf[s->cur];
   T* B = (T*)s->buf[s->cur ^ 1];
   const Geom& g = s->g;
   DevTables& t = s->dt;
-  if (ctl.has_target && t.np > 0) {
-    hipLaunchKernelGGL(k_outlet<S>, dim3(1), dim3(OUT_T), 0, s->stream, A, g, s->coll, t.px, t.py, t.pnx,
-                       t.pny, t.np, ctl.target, SQRT_CS2, t.drho);
-  }
-  const int xblocks = (g.nx + BULK_X - 1) / BULK_X;
+  StepArgs sa;
+  sa.nlb = s->nlb;
+  sa.xblocks = (g.nx + STEP_T - 1) / STEP_T;
+  sa.nbulk = sa.xblocks * g.ny;
+  sa.per_xcd = s->xcd_remap ? (sa.nbulk + 7) / 8 : 0;
+  sa.sqrt_cs2 = SQRT_CS2;
+  const int64_t ns2 = (int64_t)std::max(t.nslots, 1) * 2;
+  sa.fc_cur = t.fcontrib + (s->t & 1) * ns2;
+  sa.fc_prev = t.fcontrib + ((s->t + 1) & 1) * ns2;
+  sa.prev_out = (t.ngroups > 0) ? prev_out : nullptr;
+  StepCtl c2 = ctl;
+  if (t.np == 0) c2.has_target = 0;
+  const int nbulk_launch = sa.per_xcd ? sa.per_xcd * 8 : sa.nbulk;
   const bool timed = s->timing && s->ev_used + 2 <= (int64_t)s->ev.size();
   if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
-  hipLaunchKernelGGL((k_bulk<S, M>), dim3((unsigned)(xblocks * g.ny)), dim3(BULK_X), 0, s->stream, A, B,
-                     s->dflag, g, s->coll, xblocks);
+  hipLaunchKernelGGL((k_step<S, M>), dim3((unsigned)(sa.nlb + nbulk_launch)), dim3(STEP_T), 0, s->stream, A, B,
+                     s->dflag, g, s->coll, t, c2, sa);
   if (timed) {
     VL_HIP(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
     s->ev_used += 2;
-  }
-  if (t.nb > 0) {
-    StepCtl c2 = ctl;
-    if (t.np == 0) c2.has_target = 0;
-    hipLaunchKernelGGL((k_links<S, M>), dim3((unsigned)((t.nb + LINK_T - 1) / LINK_T)), dim3(LINK_T), 0,
-                       s->stream, A, B, g, s->coll, t, c2, SQRT_CS2);
-  }
-  if (t.ngroups > 0 && force_out) {
-    hipLaunchKernelGGL(k_force, dim3((unsigned)t.ngroups), dim3(FORCE_T), 0, s->stream, t.fcontrib, t.gfirst,
-                       force_out);
   }
   VL_HIP(hipGetLastError());
   s->cur ^= 1;
@@ -359,20 +380,31 @@ static vl_status launch_step(vl_sim* s, const StepCtl& ctl, double* force_out) {
 }
 
 template <class S>
-static vl_status step_dispatch(vl_sim* s, const StepCtl& ctl, double* force_out) {
+static vl_status step_dispatch(vl_sim* s, const StepCtl& ctl, double* prev_out) {
   switch (s->coll.model) {
-    case MODEL_FORCED: return launch_step<S, MODEL_FORCED>(s, ctl, force_out);
-    case MODEL_BGK: return launch_step<S, MODEL_BGK>(s, ctl, force_out);
-    default: return launch_step<S, MODEL_TRT>(s, ctl, force_out);
+    case MODEL_FORCED: return launch_step<S, MODEL_FORCED>(s, ctl, prev_out);
+    case MODEL_BGK: return launch_step<S, MODEL_BGK>(s, ctl, prev_out);
+    default: return launch_step<S, MODEL_TRT>(s, ctl, prev_out);
   }
 }
 
-static vl_status one_step(vl_sim* s, const StepCtl& ctl, double* force_out) {
+static vl_status one_step(vl_sim* s, const StepCtl& ctl, double* prev_out) {
   switch (s->precision) {
-    case VL_F32: return step_dispatch<StoreF32>(s, ctl, force_out);
-    case VL_F16: return step_dispatch<StoreF16>(s, ctl, force_out);
-    default: return step_dispatch<StoreF64>(s, ctl, force_out);
+    case VL_F32: return step_dispatch<StoreF32>(s, ctl, prev_out);
+    case VL_F16: return step_dispatch<StoreF16>(s, ctl, prev_out);
+    default: return step_dispatch<StoreF64>(s, ctl, prev_out);
   }
+}
+
+// Group forces of the step that just ran (time s->t - 1) into `out`.
+static vl_status finish_forces(vl_sim* s, double* out) {
+  DevTables& t = s->dt;
+  if (t.ngroups == 0) return VL_OK;
+  const int64_t ns2 = (int64_t)std::max(t.nslots, 1) * 2;
+  const double* fc = t.fcontrib + ((s->t - 1) & 1) * ns2;
+  hipLaunchKernelGGL(k_force, dim3(1), dim3(STEP_T), 0, s->stream, fc, t.gfirst, t.ngroups, out);
+  VL_HIP(hipGetLastError());
+  return VL_OK;
 }
 
 template <class V>
@@ -500,7 +532,51 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   for (int64_t y = 0; y < d.ny; ++y)
     std::memcpy(&pflag[y * s->g.pitch], &s->tab.flag[y * d.nx], (size_t)d.nx);
   VL_TRY(upload(s, s->dflag, pflag));
-  const Tables& tb = s->tab;
+  // Boundary nodes whose links need the outlet controller go first, so only the leading
+  // boundary workgroups compute it.  Link order inside each node, and the force-slot order,
+  // are unchanged, so results do not depend on this permutation.
+  Tables tb = s->tab;
+  {
+    const size_t nb = tb.bnode_x.size();
+    std::vector<int32_t> order;
+    order.reserve(nb);
+    std::vector<char> needs(nb, 0);
+    for (size_t b = 0; b < nb; ++b)
+      for (int32_t l = tb.bnode_first[b]; l < tb.bnode_first[b + 1]; ++l)
+        if (tb.l_kind[l] == LINK_PRESSURE && !(tb.l_flags[l] & 1)) needs[b] = 1;
+    for (size_t b = 0; b < nb; ++b)
+      if (needs[b]) order.push_back((int32_t)b);
+    s->dt.npb = (int32_t)order.size();
+    for (size_t b = 0; b < nb; ++b)
+      if (!needs[b]) order.push_back((int32_t)b);
+    Tables r = tb;
+    r.bnode_x.clear();
+    r.bnode_y.clear();
+    r.bnode_first.assign(1, 0);
+    auto clear_links = [&](auto&... v) { (v.clear(), ...); };
+    clear_links(r.l_dir, r.l_kind, r.l_vgroup, r.l_flags, r.l_nx, r.l_ny, r.l_fslot, r.l_q, r.l_uwx, r.l_uwy,
+                r.l_cu6, r.l_rho);
+    for (int32_t b : order) {
+      r.bnode_x.push_back(tb.bnode_x[b]);
+      r.bnode_y.push_back(tb.bnode_y[b]);
+      for (int32_t l = tb.bnode_first[b]; l < tb.bnode_first[b + 1]; ++l) {
+        r.l_dir.push_back(tb.l_dir[l]);
+        r.l_kind.push_back(tb.l_kind[l]);
+        r.l_vgroup.push_back(tb.l_vgroup[l]);
+        r.l_flags.push_back(tb.l_flags[l]);
+        r.l_nx.push_back(tb.l_nx[l]);
+        r.l_ny.push_back(tb.l_ny[l]);
+        r.l_fslot.push_back(tb.l_fslot[l]);
+        r.l_q.push_back(tb.l_q[l]);
+        r.l_uwx.push_back(tb.l_uwx[l]);
+        r.l_uwy.push_back(tb.l_uwy[l]);
+        r.l_cu6.push_back(tb.l_cu6[l]);
+        r.l_rho.push_back(tb.l_rho[l]);
+      }
+      r.bnode_first.push_back((int32_t)r.l_dir.size());
+    }
+    tb = std::move(r);
+  }
   DevTables& t = s->dt;
   t.nb = (int32_t)tb.bnode_x.size();
   t.nl = (int32_t)tb.l_dir.size();
@@ -527,10 +603,16 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   VL_TRY(upload(s, t.pnx, tb.p_nx));
   VL_TRY(upload(s, t.pny, tb.p_ny));
   VL_TRY(upload(s, t.gfirst, tb.f_group_first));
-  if (t.nslots > 0) {
-    VL_HIP(hipMalloc((void**)&t.fcontrib, (size_t)t.nslots * 2 * sizeof(double)));
-    VL_HIP(hipMemset(t.fcontrib, 0, (size_t)t.nslots * 2 * sizeof(double)));
+  {
+    const size_t fcb = (size_t)std::max(t.nslots, 1) * 4 * sizeof(double);  // two step buffers
+    VL_HIP(hipMalloc((void**)&t.fcontrib, fcb));
+    VL_HIP(hipMemset(t.fcontrib, 0, fcb));
   }
+  s->xcd_remap = true;
+  if (const char* e = std::getenv("VL_XCD_REMAP")) s->xcd_remap = e[0] != '0';
+  s->nlb = (t.nb + STEP_T - 1) / STEP_T;
+  if (s->nlb == 0 && t.ngroups > 0) s->nlb = 1;
+  if (s->xcd_remap) s->nlb = (s->nlb + 7) / 8 * 8;
   VL_HIP(hipMalloc((void**)&t.drho, sizeof(double)));
   VL_HIP(hipMemset(t.drho, 0, sizeof(double)));
   s->last_force.assign((size_t)t.ngroups * 2, 0.0);
@@ -599,13 +681,15 @@ static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double*
         c.target = s->target;
         c.has_target = s->has_target;
       }
-      VL_TRY(one_step(s, c, ng > 0 ? s->dhist + k * ng * 2 : nullptr));
+      // step k also reduces the force terms of step k-1 into its history slot
+      VL_TRY(one_step(s, c, (ng > 0 && k > 0) ? s->dhist + (k - 1) * ng * 2 : nullptr));
       if (k0 + k == nsteps - 1) {
         s->has_target = c.has_target;
         s->target = c.target;
         for (int g = 0; g < 4; ++g) s->vscale[g] = c.vscale[g];
       }
     }
+    if (ng > 0) VL_TRY(finish_forces(s, s->dhist + (m - 1) * ng * 2));
     if (s->timing) VL_HIP(hipEventRecord(s->ev_chunk[1], s->stream));
     if (ng > 0) {
       if (force_hist)
@@ -622,6 +706,45 @@ static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double*
   else if (nsteps > 0)
     s->outlet_density_offset = 0.0;
   return VL_OK;
+}
+
+// Bandwidth of a plain device copy of `bytes` (read + write counted), best of `iters`.
+static vl_status copy_bandwidth(size_t bytes, int iters, double* gbs) {
+  const size_t n = bytes / 16;
+  if (n == 0 || iters < 1) return set_error(VL_EINVAL, "copy size and iterations must be positive");
+  float4 *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  vl_status rc = VL_OK;
+  auto run = [&]() -> vl_status {
+    VL_HIP(hipMalloc((void**)&a, n * 16));
+    VL_HIP(hipMalloc((void**)&b, n * 16));
+    VL_HIP(hipMemset(a, 0, n * 16));
+    VL_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    VL_HIP(hipEventCreate(&e0));
+    VL_HIP(hipEventCreate(&e1));
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    float best = 1e30f;
+    for (int it = 0; it <= iters; ++it) {  // first launch warms up
+      VL_HIP(hipEventRecord(e0, st));
+      hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, st, a, b, n);
+      VL_HIP(hipGetLastError());
+      VL_HIP(hipEventRecord(e1, st));
+      VL_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      VL_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (it > 0) best = std::min(best, ms);
+    }
+    *gbs = 2.0 * (double)(n * 16) / (best * 1e-3) / 1e9;
+    return VL_OK;
+  };
+  rc = run();
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  return rc;
 }
 
 }  // namespace vl
@@ -710,6 +833,11 @@ vl_status vl_sim_step(vl_sim* s, int64_t nsteps) {
 vl_status vl_sim_run(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist) {
   if (!s || (!ctl && nsteps > 0)) return set_error(VL_EINVAL, "null argument");
   return run_steps(s, nsteps, ctl, force_hist);
+}
+
+vl_status vl_device_copy_bandwidth(size_t bytes, int iters, double* gbs) {
+  if (!gbs) return set_error(VL_EINVAL, "null argument");
+  return copy_bandwidth(bytes, iters, gbs);
 }
 
 vl_status vl_sim_set_timing(vl_sim* s, int on) {

@@ -43,6 +43,7 @@ struct Geom {
 
 struct DevTables {
   int32_t nb = 0, nl = 0, np = 0, nslots = 0, ngroups = 0;
+  int32_t npb = 0;  // boundary nodes (listed first) whose links use the outlet controller
   int32_t *bx = nullptr, *by = nullptr, *bfirst = nullptr;
   uint8_t *dir = nullptr, *kind = nullptr, *vgroup = nullptr, *lflags = nullptr;
   int8_t *lnx = nullptr, *lny = nullptr;
@@ -51,7 +52,7 @@ struct DevTables {
   int32_t *px = nullptr, *py = nullptr;
   int8_t *pnx = nullptr, *pny = nullptr;
   int32_t* gfirst = nullptr;   // ngroups+1 slot offsets
-  double* fcontrib = nullptr;  // nslots*2
+  double* fcontrib = nullptr;  // 2 steps x nslots x (x, y)
   double* drho = nullptr;      // 1 value: outlet density offset of the current step
 };
 
@@ -70,6 +71,8 @@ struct vl_sim {
   double outlet_density_offset = 0.0;
   int64_t t = 0;
   int device = 0;
+  int nlb = 0;            // boundary workgroups of the step kernel
+  bool xcd_remap = true;  // bulk tiles in contiguous runs per XCD
   hipStream_t stream = nullptr;
   void* buf[2] = {nullptr, nullptr};  // population arrays; buf[cur] is A
   int cur = 0;
