@@ -36,6 +36,7 @@ struct StoreF64 {
   static constexpr bool exact = true;
   __device__ static inline C ld(const T* p) { return *p; }
   __device__ static inline void st(T* p, C v) { *p = v; }
+  __device__ static inline C round(C v) { return v; }
 };
 struct StoreF32 {
   using T = float;
@@ -43,6 +44,7 @@ struct StoreF32 {
   static constexpr bool exact = false;
   __device__ static inline C ld(const T* p) { return *p; }
   __device__ static inline void st(T* p, C v) { *p = v; }
+  __device__ static inline C round(C v) { return v; }
 };
 // FP16 deviations scaled by 2^10: keeps |d| ~ 1e-6..1e-1 inside the normal half range.
 constexpr float F16_SCALE = 1024.0f;
@@ -53,6 +55,7 @@ struct StoreF16 {
   static constexpr bool exact = false;
   __device__ static inline C ld(const T* p) { return (float)*p * F16_INV; }
   __device__ static inline void st(T* p, C v) { *p = (_Float16)(v * F16_SCALE); }
+  __device__ static inline C round(C v) { return (float)(_Float16)(v * F16_SCALE) * F16_INV; }  // = ld(st(v))
 };
 
 template <class S, int M>
@@ -101,6 +104,8 @@ struct StepArgs {
   double* fc_cur;        // force terms written this step (per force slot: x, y)
   const double* fc_prev; // force terms of the previous step
   double* prev_out;      // where the previous step's group forces go (null: not wanted)
+  const double* pv_cur;  // outward velocity of every pressure-boundary node, this step
+  double* pv_next;       // the same for the next step (written by the boundary update)
 };
 
 // Outlet controller (src/sim.rs:443-455): the mean outward normal velocity over the
@@ -108,42 +113,52 @@ struct StepArgs {
 // left-to-right sum in node creation order (it feeds back into every later step, so the
 // order matters for bit equality).  Deviation storage: a fixed-order tree.  Every thread of
 // the workgroup must call this; all get the result.
+// Outward normal velocity n.u of one node from its (stored-precision) populations, with the
+// arithmetic of src/sim.rs:432-439, 447-449.
 template <class S>
-__device__ inline double normal_velocity(const typename S::T* __restrict__ A, const Geom& g, const Coll& c,
-                                         int x, int y, int nx_, int ny_) {
-  const int64_t idx = (int64_t)y * g.pitch + x;
+__device__ inline double normal_velocity_of(const typename S::C* f, const Coll& c, int nx_, int ny_) {
   if constexpr (S::exact) {
-    double f[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = A[i * g.plane + idx];
     double rho, jx, jy;
     Exact::moments(f, rho, jx, jy);
     const double ux = jx / rho - 0.5 * c.gx, uy = jy / rho - 0.5 * c.gy;
     return (double)nx_ * ux + (double)ny_ * uy;
   } else {
-    float d[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) d[i] = S::ld(A + i * g.plane + idx);
     float drho, jx, jy;
-    Dev::moments(d, drho, jx, jy);
+    Dev::moments(f, drho, jx, jy);
     const double rho = 1.0 + (double)drho;
     const double ux = (double)jx / rho - 0.5 * c.gx, uy = (double)jy / rho - 0.5 * c.gy;
     return (double)nx_ * ux + (double)ny_ * uy;
   }
 }
 
+// The same from the populations in A (used once at the start of a run).
 template <class S>
-__device__ double outlet_offset(const typename S::T* __restrict__ A, const Geom& g, const Coll& c,
-                                const DevTables& t, double target, double sqrt_cs2, double* red) {
+__global__ __launch_bounds__(256) void k_outlet_prime(const typename S::T* __restrict__ A, Geom g, Coll c,
+                                                      DevTables t, double* __restrict__ pv) {
+  const int k = blockIdx.x * 256 + (int)threadIdx.x;
+  if (k >= t.np) return;
+  const int64_t idx = (int64_t)t.py[k] * g.pitch + t.px[k];
+  typename S::C f[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) f[i] = S::ld(A + i * g.plane + idx);
+  pv[k] = normal_velocity_of<S>(f, c, t.pnx[k], t.pny[k]);
+}
+
+// Density offset of the outlet controller from the nodes' outward velocities `pv` (which the
+// previous step's boundary update left behind): serial left-to-right sum for FP64 storage,
+// fixed-order tree otherwise.  Every thread of the workgroup must call this.
+template <class S>
+__device__ double outlet_offset(const double* __restrict__ pv, int np, double target, double sqrt_cs2,
+                                double* red) {
   double sum = 0.0;  // serial running sum (exact) or this thread's strided partial (tree)
-  for (int base = 0; base < t.np; base += STEP_T) {
+  for (int base = 0; base < np; base += STEP_T) {
     const int k = base + (int)threadIdx.x;
-    const double v = k < t.np ? normal_velocity<S>(A, g, c, t.px[k], t.py[k], t.pnx[k], t.pny[k]) : 0.0;
+    const double v = k < np ? pv[k] : 0.0;
     if constexpr (S::exact) {
       red[threadIdx.x] = v;
       __syncthreads();
       if (threadIdx.x == 0) {
-        const int m = min(STEP_T, t.np - base);
+        const int m = min(STEP_T, np - base);
         for (int i = 0; i < m; ++i) sum += red[i];
       }
       __syncthreads();
@@ -161,7 +176,7 @@ __device__ double outlet_offset(const typename S::T* __restrict__ A, const Geom&
     sum = red[0];
     __syncthreads();
   }
-  if (threadIdx.x == 0) red[0] = (sum / (double)t.np - target) / sqrt_cs2;
+  if (threadIdx.x == 0) red[0] = (sum / (double)np - target) / sqrt_cs2;
   __syncthreads();
   const double drho = red[0];
   __syncthreads();
@@ -195,7 +210,7 @@ __device__ void reduce_forces(const double* __restrict__ fc, const int32_t* __re
 template <class S, int M>
 __device__ void boundary_node(const typename S::T* __restrict__ A, typename S::T* __restrict__ B, const Geom& g,
                               const Coll& c, const DevTables& t, const StepCtl& ctl, double drho, double sqrt_cs2,
-                              int b, double* __restrict__ fc) {
+                              int b, double* __restrict__ fc, double* __restrict__ pv_next) {
   using C = typename S::C;
   const int x = t.bx[b], y = t.by[b];
   const int64_t idx = (int64_t)y * g.pitch + x;
@@ -291,6 +306,12 @@ __device__ void boundary_node(const typename S::T* __restrict__ A, typename S::T
   collide<S, M>(f, c);
 #pragma unroll
   for (int i = 0; i < Q; ++i) S::st(B + i * g.plane + idx, f[i]);
+  if (b < t.npb) {  // pressure-boundary node b is outlet-controller node b: its velocity next step
+    C r[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) r[i] = S::round(f[i]);
+    pv_next[b] = normal_velocity_of<S>(r, c, t.pnx[b], t.pny[b]);
+  }
 }
 
 template <class S, int M>
@@ -303,11 +324,11 @@ __global__ __launch_bounds__(STEP_T) void k_step(const typename S::T* __restrict
     if (blk == 0 && sa.prev_out) reduce_forces(sa.fc_prev, t.gfirst, t.ngroups, sa.prev_out, red);
     double drho = 0.0;
     if (ctl.has_target && t.np > 0 && blk * STEP_T < t.npb) {
-      drho = outlet_offset<S>(A, g, c, t, ctl.target, sa.sqrt_cs2, red);
+      drho = outlet_offset<S>(sa.pv_cur, t.np, ctl.target, sa.sqrt_cs2, red);
       if (blk == 0 && threadIdx.x == 0) *t.drho = drho;
     }
     const int b = blk * STEP_T + (int)threadIdx.x;
-    if (b < t.nb) boundary_node<S, M>(A, B, g, c, t, ctl, drho, sa.sqrt_cs2, b, sa.fc_cur);
+    if (b < t.nb) boundary_node<S, M>(A, B, g, c, t, ctl, drho, sa.sqrt_cs2, b, sa.fc_cur, sa.pv_next);
     return;
   }
   int tile = blk - sa.nlb;
@@ -362,6 +383,9 @@ static vl_status launch_step(vl_sim* s, const StepCtl& ctl, double* prev_out) {
   sa.fc_cur = t.fcontrib + (s->t & 1) * ns2;
   sa.fc_prev = t.fcontrib + ((s->t + 1) & 1) * ns2;
   sa.prev_out = (t.ngroups > 0) ? prev_out : nullptr;
+  const int64_t np1 = std::max(t.np, 1);
+  sa.pv_cur = t.pvel + (s->t & 1) * np1;
+  sa.pv_next = t.pvel + ((s->t + 1) & 1) * np1;
   StepCtl c2 = ctl;
   if (t.np == 0) c2.has_target = 0;
   const int nbulk_launch = sa.per_xcd ? sa.per_xcd * 8 : sa.nbulk;
@@ -394,6 +418,28 @@ static vl_status one_step(vl_sim* s, const StepCtl& ctl, double* prev_out) {
     case VL_F16: return step_dispatch<StoreF16>(s, ctl, prev_out);
     default: return step_dispatch<StoreF64>(s, ctl, prev_out);
   }
+}
+
+// Outward velocities of the outlet-controller nodes from the current A (start of a run).
+static vl_status prime_outlet(vl_sim* s) {
+  DevTables& t = s->dt;
+  double* pv = t.pvel + (s->t & 1) * (int64_t)std::max(t.np, 1);
+  const unsigned grid = (unsigned)((t.np + 255) / 256);
+  switch (s->precision) {
+    case VL_F32:
+      hipLaunchKernelGGL(k_outlet_prime<StoreF32>, dim3(grid), dim3(256), 0, s->stream,
+                         (const float*)s->buf[s->cur], s->g, s->coll, t, pv);
+      break;
+    case VL_F16:
+      hipLaunchKernelGGL(k_outlet_prime<StoreF16>, dim3(grid), dim3(256), 0, s->stream,
+                         (const _Float16*)s->buf[s->cur], s->g, s->coll, t, pv);
+      break;
+    default:
+      hipLaunchKernelGGL(k_outlet_prime<StoreF64>, dim3(grid), dim3(256), 0, s->stream,
+                         (const double*)s->buf[s->cur], s->g, s->coll, t, pv);
+  }
+  VL_HIP(hipGetLastError());
+  return VL_OK;
 }
 
 // Group forces of the step that just ran (time s->t - 1) into `out`.
@@ -484,7 +530,7 @@ static void free_sim(vl_sim* s) {
   for (void* p : {(void*)t.bx, (void*)t.by, (void*)t.bfirst, (void*)t.dir, (void*)t.kind, (void*)t.vgroup,
                   (void*)t.lflags, (void*)t.lnx, (void*)t.lny, (void*)t.fslot, (void*)t.q, (void*)t.uwx,
                   (void*)t.uwy, (void*)t.cu6, (void*)t.rho, (void*)t.px, (void*)t.py, (void*)t.pnx,
-                  (void*)t.pny, (void*)t.gfirst, (void*)t.fcontrib, (void*)t.drho})
+                  (void*)t.pny, (void*)t.gfirst, (void*)t.fcontrib, (void*)t.drho, (void*)t.pvel})
     f(p);
   f(s->dhist);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -613,6 +659,11 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   s->nlb = (t.nb + STEP_T - 1) / STEP_T;
   if (s->nlb == 0 && t.ngroups > 0) s->nlb = 1;
   if (s->xcd_remap) s->nlb = (s->nlb + 7) / 8 * 8;
+  bool same = t.np == t.npb;
+  for (int32_t k = 0; same && k < t.np; ++k) same = tb.bnode_x[k] == tb.p_x[k] && tb.bnode_y[k] == tb.p_y[k];
+  if (!same) return set_error(VL_EINVAL, "internal: outlet-controller nodes do not match the pressure-boundary nodes");
+  VL_HIP(hipMalloc((void**)&t.pvel, (size_t)std::max(t.np, 1) * 2 * sizeof(double)));
+  VL_HIP(hipMemset(t.pvel, 0, (size_t)std::max(t.np, 1) * 2 * sizeof(double)));
   VL_HIP(hipMalloc((void**)&t.drho, sizeof(double)));
   VL_HIP(hipMemset(t.drho, 0, sizeof(double)));
   s->last_force.assign((size_t)t.ngroups * 2, 0.0);
@@ -664,6 +715,7 @@ static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double*
   const int64_t chunk = RUN_CHUNK;
   VL_TRY(ensure_hist(s, std::min<int64_t>(std::max<int64_t>(nsteps, 1), chunk)));
   if (s->timing) VL_TRY(ensure_events(s));
+  if (nsteps > 0 && s->dt.np > 0) VL_TRY(prime_outlet(s));
   for (int64_t k0 = 0; k0 < nsteps; k0 += chunk) {
     const int64_t m = std::min(chunk, nsteps - k0);
     if (s->timing) {

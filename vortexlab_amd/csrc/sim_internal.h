@@ -54,6 +54,7 @@ struct DevTables {
   int32_t* gfirst = nullptr;   // ngroups+1 slot offsets
   double* fcontrib = nullptr;  // 2 steps x nslots x (x, y)
   double* drho = nullptr;      // 1 value: outlet density offset of the current step
+  double* pvel = nullptr;      // 2 steps x np outward velocities of the outlet-controller nodes
 };
 
 }  // namespace vl
