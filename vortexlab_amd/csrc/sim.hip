@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "lbm_math.h"
@@ -36,6 +37,7 @@ struct StoreF64 {
   static constexpr bool exact = true;
   __device__ static inline C ld(const T* p) { return *p; }
   __device__ static inline void st(T* p, C v) { *p = v; }
+  __device__ static inline void st_stream(T* p, C v) { __builtin_nontemporal_store(v, p); }
   __device__ static inline C round(C v) { return v; }
 };
 struct StoreF32 {
@@ -44,6 +46,7 @@ struct StoreF32 {
   static constexpr bool exact = false;
   __device__ static inline C ld(const T* p) { return *p; }
   __device__ static inline void st(T* p, C v) { *p = v; }
+  __device__ static inline void st_stream(T* p, C v) { __builtin_nontemporal_store(v, p); }
   __device__ static inline C round(C v) { return v; }
 };
 // FP16 deviations scaled by 2^10: keeps |d| ~ 1e-6..1e-1 inside the normal half range.
@@ -55,6 +58,7 @@ struct StoreF16 {
   static constexpr bool exact = false;
   __device__ static inline C ld(const T* p) { return (float)*p * F16_INV; }
   __device__ static inline void st(T* p, C v) { *p = (_Float16)(v * F16_SCALE); }
+  __device__ static inline void st_stream(T* p, C v) { __builtin_nontemporal_store((_Float16)(v * F16_SCALE), p); }
   __device__ static inline C round(C v) { return (float)(_Float16)(v * F16_SCALE) * F16_INV; }  // = ld(st(v))
 };
 
@@ -96,10 +100,8 @@ __device__ inline void pull(const typename S::T* __restrict__ A, const Geom& g, 
 constexpr int STEP_T = 256;
 
 struct StepArgs {
-  int nlb;        // boundary workgroups (a multiple of 8 when the XCD remap is on)
-  int xblocks;    // bulk workgroups per row
-  int nbulk;      // bulk workgroups that carry work
-  int per_xcd;    // bulk workgroups per XCD (remap on), else 0
+  int nlb;        // boundary workgroups
+  int xblocks;    // bulk tiles per row (fused kernel)
   double sqrt_cs2;
   double* fc_cur;        // force terms written this step (per force slot: x, y)
   const double* fc_prev; // force terms of the previous step
@@ -314,30 +316,28 @@ __device__ void boundary_node(const typename S::T* __restrict__ A, typename S::T
   }
 }
 
+// Workgroup `blk` of the boundary part of a step.
 template <class S, int M>
-__global__ __launch_bounds__(STEP_T) void k_step(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
-                                                 const uint8_t* __restrict__ flag, Geom g, Coll c, DevTables t,
-                                                 StepCtl ctl, StepArgs sa) {
-  __shared__ double red[STEP_T];
-  const int blk = (int)blockIdx.x;
-  if (blk < sa.nlb) {
-    if (blk == 0 && sa.prev_out) reduce_forces(sa.fc_prev, t.gfirst, t.ngroups, sa.prev_out, red);
-    double drho = 0.0;
-    if (ctl.has_target && t.np > 0 && blk * STEP_T < t.npb) {
-      drho = outlet_offset<S>(sa.pv_cur, t.np, ctl.target, sa.sqrt_cs2, red);
-      if (blk == 0 && threadIdx.x == 0) *t.drho = drho;
-    }
-    const int b = blk * STEP_T + (int)threadIdx.x;
-    if (b < t.nb) boundary_node<S, M>(A, B, g, c, t, ctl, drho, sa.sqrt_cs2, b, sa.fc_cur, sa.pv_next);
-    return;
+__device__ inline void boundary_block(int blk, const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
+                                      const Geom& g, const Coll& c, const DevTables& t, const StepCtl& ctl,
+                                      const StepArgs& sa, double* red) {
+  if (blk == 0 && sa.prev_out) reduce_forces(sa.fc_prev, t.gfirst, t.ngroups, sa.prev_out, red);
+  double drho = 0.0;
+  if (ctl.has_target && t.np > 0 && blk * STEP_T < t.npb) {
+    drho = outlet_offset<S>(sa.pv_cur, t.np, ctl.target, sa.sqrt_cs2, red);
+    if (blk == 0 && threadIdx.x == 0) *t.drho = drho;
   }
-  int tile = blk - sa.nlb;
-  if (sa.per_xcd) {  // consecutive tiles on one XCD: its L2 sees whole row segments
-    tile = (tile & 7) * sa.per_xcd + (tile >> 3);
-    if (tile >= sa.nbulk) return;
-  }
-  const int y = tile / sa.xblocks;
-  const int x = (tile - y * sa.xblocks) * STEP_T + (int)threadIdx.x;
+  const int b = blk * STEP_T + (int)threadIdx.x;
+  if (b < t.nb) boundary_node<S, M>(A, B, g, c, t, ctl, drho, sa.sqrt_cs2, b, sa.fc_cur, sa.pv_next);
+}
+
+// One interior node of bulk tile `tile` (tiles are `width` consecutive nodes of one row).
+template <class S, int M>
+__device__ inline void bulk_node(int tile, int width, int xblocks, const typename S::T* __restrict__ A,
+                                 typename S::T* __restrict__ B, const uint8_t* __restrict__ flag, const Geom& g,
+                                 const Coll& c) {
+  const int y = tile / xblocks;
+  const int x = (tile - y * xblocks) * width + (int)threadIdx.x;
   if (x >= g.nx) return;
   const int64_t idx = (int64_t)y * g.pitch + x;
   if (flag[idx] != 0) return;
@@ -345,7 +345,39 @@ __global__ __launch_bounds__(STEP_T) void k_step(const typename S::T* __restrict
   pull<S>(A, g, x, y, f);
   collide<S, M>(f, c);
 #pragma unroll
-  for (int i = 0; i < Q; ++i) S::st(B + i * g.plane + idx, f[i]);
+  for (int i = 0; i < Q; ++i) S::st_stream(B + i * g.plane + idx, f[i]);
+}
+
+// Small domains: the whole step in one launch (boundary workgroups first).
+template <class S, int M>
+__global__ __launch_bounds__(STEP_T) void k_step(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
+                                                 const uint8_t* __restrict__ flag, Geom g, Coll c, DevTables t,
+                                                 StepCtl ctl, StepArgs sa) {
+  __shared__ double red[STEP_T];
+  const int blk = (int)blockIdx.x;
+  if (blk < sa.nlb)
+    boundary_block<S, M>(blk, A, B, g, c, t, ctl, sa, red);
+  else
+    bulk_node<S, M>(blk - sa.nlb, STEP_T, sa.xblocks, A, B, flag, g, c);
+}
+
+// Large domains: the boundary part and the bulk as two kernels on two streams, so each gets
+// its own register budget (the bulk stays lean enough for 8 waves per SIMD) and the short
+// boundary kernel runs concurrently with the bulk.
+template <class S, int M>
+__global__ __launch_bounds__(STEP_T) void k_boundary(const typename S::T* __restrict__ A,
+                                                     typename S::T* __restrict__ B, Geom g, Coll c, DevTables t,
+                                                     StepCtl ctl, StepArgs sa) {
+  __shared__ double red[STEP_T];
+  boundary_block<S, M>((int)blockIdx.x, A, B, g, c, t, ctl, sa, red);
+}
+
+constexpr int BULK_T = 512;
+
+template <class S, int M>
+__global__ __launch_bounds__(BULK_T) void k_bulk(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
+                                                 const uint8_t* __restrict__ flag, Geom g, Coll c, int xblocks) {
+  bulk_node<S, M>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, g, c);
 }
 
 // Group forces of the last step of a run (the in-step reduction covers the ones before).
@@ -366,6 +398,8 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
 static const double SQRT_CS2 = std::sqrt(1.0 / 3.0);
 
 // Launch one step.  `prev_out`: where the previous step's group forces go (or null).
+// Fused mode: one kernel.  Split mode: the boundary kernel on the side stream and the bulk on
+// the main stream, forked after the previous step and joined before the next one.
 template <class S, int M>
 static vl_status launch_step(vl_sim* s, const StepCtl& ctl, double* prev_out) {
   using T = typename S::T;
@@ -376,8 +410,6 @@ static vl_status launch_step(vl_sim* s, const StepCtl& ctl, double* prev_out) {
   StepArgs sa;
   sa.nlb = s->nlb;
   sa.xblocks = (g.nx + STEP_T - 1) / STEP_T;
-  sa.nbulk = sa.xblocks * g.ny;
-  sa.per_xcd = s->xcd_remap ? (sa.nbulk + 7) / 8 : 0;
   sa.sqrt_cs2 = SQRT_CS2;
   const int64_t ns2 = (int64_t)std::max(t.nslots, 1) * 2;
   sa.fc_cur = t.fcontrib + (s->t & 1) * ns2;
@@ -388,15 +420,29 @@ static vl_status launch_step(vl_sim* s, const StepCtl& ctl, double* prev_out) {
   sa.pv_next = t.pvel + ((s->t + 1) & 1) * np1;
   StepCtl c2 = ctl;
   if (t.np == 0) c2.has_target = 0;
-  const int nbulk_launch = sa.per_xcd ? sa.per_xcd * 8 : sa.nbulk;
   const bool timed = s->timing && s->ev_used + 2 <= (int64_t)s->ev.size();
-  if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
-  hipLaunchKernelGGL((k_step<S, M>), dim3((unsigned)(sa.nlb + nbulk_launch)), dim3(STEP_T), 0, s->stream, A, B,
-                     s->dflag, g, s->coll, t, c2, sa);
+  if (!s->split) {
+    if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
+    hipLaunchKernelGGL((k_step<S, M>), dim3((unsigned)(sa.nlb + sa.xblocks * g.ny)), dim3(STEP_T), 0, s->stream, A,
+                       B, s->dflag, g, s->coll, t, c2, sa);
+  } else {
+    if (sa.nlb > 0) {
+      VL_HIP(hipEventRecord(s->ev_fork, s->stream));
+      VL_HIP(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+      hipLaunchKernelGGL((k_boundary<S, M>), dim3((unsigned)sa.nlb), dim3(STEP_T), 0, s->side, A, B, g, s->coll, t,
+                         c2, sa);
+      VL_HIP(hipEventRecord(s->ev_join, s->side));
+    }
+    if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
+    const int xb = (g.nx + BULK_T - 1) / BULK_T;
+    hipLaunchKernelGGL((k_bulk<S, M>), dim3((unsigned)(xb * g.ny)), dim3(BULK_T), 0, s->stream, A, B, s->dflag, g,
+                       s->coll, xb);
+  }
   if (timed) {
     VL_HIP(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
     s->ev_used += 2;
   }
+  if (s->split && sa.nlb > 0) VL_HIP(hipStreamWaitEvent(s->stream, s->ev_join, 0));
   VL_HIP(hipGetLastError());
   s->cur ^= 1;
   s->t += 1;
@@ -536,6 +582,10 @@ static void free_sim(vl_sim* s) {
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : s->ev_chunk)
     if (e) (void)hipEventDestroy(e);
+  if (s->side) (void)hipStreamSynchronize(s->side);
+  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+  if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+  if (s->side) (void)hipStreamDestroy(s->side);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -654,11 +704,17 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
     VL_HIP(hipMalloc((void**)&t.fcontrib, fcb));
     VL_HIP(hipMemset(t.fcontrib, 0, fcb));
   }
-  s->xcd_remap = true;
-  if (const char* e = std::getenv("VL_XCD_REMAP")) s->xcd_remap = e[0] != '0';
   s->nlb = (t.nb + STEP_T - 1) / STEP_T;
   if (s->nlb == 0 && t.ngroups > 0) s->nlb = 1;
-  if (s->xcd_remap) s->nlb = (s->nlb + 7) / 8 * 8;
+  // Split the step into two concurrent kernels once the bulk is large enough for the
+  // per-step fork/join to be negligible (VL_STEP_MODE=fused|split overrides).
+  s->split = (int64_t)d.nx * d.ny >= ((int64_t)1 << 21);
+  if (const char* e = std::getenv("VL_STEP_MODE")) s->split = std::string(e) == "split";
+  if (s->split) {
+    VL_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    VL_HIP(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+    VL_HIP(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+  }
   bool same = t.np == t.npb;
   for (int32_t k = 0; same && k < t.np; ++k) same = tb.bnode_x[k] == tb.p_x[k] && tb.bnode_y[k] == tb.p_y[k];
   if (!same) return set_error(VL_EINVAL, "internal: outlet-controller nodes do not match the pressure-boundary nodes");

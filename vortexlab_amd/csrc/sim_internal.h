@@ -72,8 +72,10 @@ struct vl_sim {
   double outlet_density_offset = 0.0;
   int64_t t = 0;
   int device = 0;
-  int nlb = 0;            // boundary workgroups of the step kernel
-  bool xcd_remap = true;  // bulk tiles in contiguous runs per XCD
+  int nlb = 0;                       // boundary workgroups of a step
+  bool split = false;                // boundary and bulk as two concurrent kernels
+  hipStream_t side = nullptr;        // stream of the boundary kernel in split mode
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipStream_t stream = nullptr;
   void* buf[2] = {nullptr, nullptr};  // population arrays; buf[cur] is A
   int cur = 0;
