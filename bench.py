@@ -61,7 +61,7 @@ def run_gpu(vl, api, precision, steps, warmup, scale):
                step_ms=total_ms / max(tsteps, 1), coef=np.concatenate([coef_w, coef]), d=spec.body.args[4],
                dt=spec.u / spec.body.args[4])
     log(f"[bench] {PREC_NAME[precision]}: {steps} steps in {secs:.2f} s = {res['glups']:.2f} GLUPS, "
-        f"step kernel {res['bulk_ms'] * 1e3:.1f} us, step incl. gaps {res['step_ms'] * 1e3:.1f} us")
+        f"bulk kernel {res['bulk_ms'] * 1e3:.1f} us, whole step {res['step_ms'] * 1e3:.1f} us")
     ch.close()
     return res
 
@@ -129,10 +129,11 @@ def main():
             "dcd_rel": abs(s_main.cd_mean / s_ref.cd_mean - 1.0),
             "dcl_rms_rel": abs(s_main.cl_rms / cl_ref - 1.0),
             "st": s_main.st, "st_fp64": s_ref.st,
-            "fp64_step_frac_hbm": round(BYTES_PER_NODE[0] * ref["fluid"] / (ref["bulk_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                        4),
+            "fp64_bulk_frac_hbm": round(BYTES_PER_NODE[0] * (ref["fluid"] - ref["bnodes"]) / (ref["bulk_ms"] * 1e-3)
+                                        / 1e9 / HBM_PEAK_GBS, 4),
         }
-    traffic_alg = BYTES_PER_NODE[prec] * main_run["fluid"]  # one k_step launch updates every fluid node
+    # the dominant kernel (k_bulk) updates the interior fluid nodes; boundary nodes run concurrently
+    traffic_alg = BYTES_PER_NODE[prec] * (main_run["fluid"] - main_run["bnodes"])
     achieved = traffic_alg / (main_run["bulk_ms"] * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"bulk_traffic_{PREC_NAME[prec]}.json")
@@ -155,7 +156,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"k_step<{PREC_NAME[prec]}>", "kernel_us": round(main_run["bulk_ms"] * 1e3, 2),
+                     "kernel": f"k_bulk<{PREC_NAME[prec]}>", "kernel_us": round(main_run["bulk_ms"] * 1e3, 2),
                      "algorithmic_bytes_per_launch": traffic_alg,
                      "copy_gbs": round(copy.value, 1), "frac_of_copy": round(achieved / copy.value, 4)},
         "cpu_baseline": cpu,
