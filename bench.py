@@ -51,11 +51,16 @@ def run_gpu(vl, api, precision, steps, warmup, scale):
     log(f"[bench] {PREC_NAME[precision]}: built {info.nx}x{info.ny}, {info.fluid_nodes} fluid nodes, "
         f"{info.link_count} links in {time.time() - t0:.1f} s")
     coef_w = ch.run(warmup, spec.body.args[4]) if warmup else np.zeros((0, 2))
-    ch.sim.set_timing(True)
+    # timed region: the production path (replayed step-window graphs)
     t0 = time.perf_counter()
     coef = ch.run(steps, spec.body.args[4])
     secs = time.perf_counter() - t0
+    # kernel pass: the same step launched directly with HIP events around every bulk kernel on
+    # the stream it runs on (events cannot sit inside a replayed graph)
+    ch.sim.set_timing(True)
+    ch.run(min(300, steps), spec.body.args[4])
     bulk_ms, bulk_n, total_ms, tsteps = ch.sim.timing()
+    ch.sim.set_timing(False)
     res = dict(precision=precision, fluid=int(info.fluid_nodes), bnodes=int(info.bnode_count), secs=secs,
                glups=info.fluid_nodes * steps / secs / 1e9, bulk_ms=bulk_ms / max(bulk_n, 1),
                step_ms=total_ms / max(tsteps, 1), coef=np.concatenate([coef_w, coef]), d=spec.body.args[4],
@@ -149,7 +154,7 @@ def main():
         "metric": METRIC, "value": round(main_run["glups"], 3), "unit": "GLUPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(main_run["secs"] / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": PREC_NAME[prec],
-        "data": "synthetic (flow from rest, seeded by the geometry)",
+        "data": "synthetic (uniform stream at t=0 past the notched square)",
         "config": {"workload": f"D2Q9 TRT 3/16 {8192 // args.scale}x{4096 // args.scale}, notched square "
                                f"D={256 // args.scale}, Re=1000, u=0.04, {PREC_NAME[prec]}-deviation storage",
                    "nx": 8192 // args.scale, "ny": 4096 // args.scale, "fluid_nodes": main_run["fluid"],

@@ -105,7 +105,12 @@ struct StepArgs {
   double sqrt_cs2;
   double* fc_cur;        // force terms written this step (per force slot: x, y)
   const double* fc_prev; // force terms of the previous step
-  double* prev_out;      // where the previous step's group forces go (null: not wanted)
+  // Per-step controls and force history live in device tables indexed by the step's place in
+  // the current run chunk, k = *kbase + k_local, so a captured window of steps can be replayed.
+  const StepCtl* ctls;   // controls of the chunk's steps
+  const int64_t* kbase;  // device counter: first chunk step of this window
+  int k_local;           // step within the window
+  double* hist;          // group forces of the chunk's steps (ngroups x 2 each), or null
   const double* pv_cur;  // outward velocity of every pressure-boundary node, this step
   double* pv_next;       // the same for the next step (written by the boundary update)
 };
@@ -319,9 +324,12 @@ __device__ void boundary_node(const typename S::T* __restrict__ A, typename S::T
 // Workgroup `blk` of the boundary part of a step.
 template <class S, int M>
 __device__ inline void boundary_block(int blk, const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
-                                      const Geom& g, const Coll& c, const DevTables& t, const StepCtl& ctl,
-                                      const StepArgs& sa, double* red) {
-  if (blk == 0 && sa.prev_out) reduce_forces(sa.fc_prev, t.gfirst, t.ngroups, sa.prev_out, red);
+                                      const Geom& g, const Coll& c, const DevTables& t, const StepArgs& sa,
+                                      double* red) {
+  const int64_t k = *sa.kbase + sa.k_local;
+  const StepCtl ctl = sa.ctls[k];
+  // the force terms of the previous step of this chunk go to its history slot
+  if (blk == 0 && sa.hist && k > 0) reduce_forces(sa.fc_prev, t.gfirst, t.ngroups, sa.hist + (k - 1) * 2 * t.ngroups, red);
   double drho = 0.0;
   if (ctl.has_target && t.np > 0 && blk * STEP_T < t.npb) {
     drho = outlet_offset<S>(sa.pv_cur, t.np, ctl.target, sa.sqrt_cs2, red);
@@ -352,11 +360,11 @@ __device__ inline void bulk_node(int tile, int width, int xblocks, const typenam
 template <class S, int M>
 __global__ __launch_bounds__(STEP_T) void k_step(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
                                                  const uint8_t* __restrict__ flag, Geom g, Coll c, DevTables t,
-                                                 StepCtl ctl, StepArgs sa) {
+                                                 StepArgs sa) {
   __shared__ double red[STEP_T];
   const int blk = (int)blockIdx.x;
   if (blk < sa.nlb)
-    boundary_block<S, M>(blk, A, B, g, c, t, ctl, sa, red);
+    boundary_block<S, M>(blk, A, B, g, c, t, sa, red);
   else
     bulk_node<S, M>(blk - sa.nlb, STEP_T, sa.xblocks, A, B, flag, g, c);
 }
@@ -367,9 +375,9 @@ __global__ __launch_bounds__(STEP_T) void k_step(const typename S::T* __restrict
 template <class S, int M>
 __global__ __launch_bounds__(STEP_T) void k_boundary(const typename S::T* __restrict__ A,
                                                      typename S::T* __restrict__ B, Geom g, Coll c, DevTables t,
-                                                     StepCtl ctl, StepArgs sa) {
+                                                     StepArgs sa) {
   __shared__ double red[STEP_T];
-  boundary_block<S, M>((int)blockIdx.x, A, B, g, c, t, ctl, sa, red);
+  boundary_block<S, M>((int)blockIdx.x, A, B, g, c, t, sa, red);
 }
 
 constexpr int BULK_T = 512;
@@ -378,6 +386,11 @@ template <class S, int M>
 __global__ __launch_bounds__(BULK_T) void k_bulk(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
                                                  const uint8_t* __restrict__ flag, Geom g, Coll c, int xblocks) {
   bulk_node<S, M>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, g, c);
+}
+
+// Advances the window counter at the end of a replayed window of steps.
+__global__ void k_advance(int64_t* kbase, int64_t by) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *kbase += by;
 }
 
 // Group forces of the last step of a run (the in-step reduction covers the ones before).
@@ -398,12 +411,14 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
 // Host side
 // ------------------------------------------------------------------------------------------
 static const double SQRT_CS2 = std::sqrt(1.0 / 3.0);
+// Steps per run chunk: controls go up and forces come back once per chunk.
+static constexpr int64_t RUN_CHUNK = 4096;
 
-// Launch one step.  `prev_out`: where the previous step's group forces go (or null).
+// Launch one step, step `k_local` of a window whose first chunk index is *kbase.
 // Fused mode: one kernel.  Split mode: the boundary kernel on the side stream and the bulk on
 // the main stream, forked after the previous step and joined before the next one.
 template <class S, int M>
-static vl_status launch_step(vl_sim* s, const StepCtl& ctl, double* prev_out) {
+static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool timed_ok) {
   using T = typename S::T;
   const T* A = (const T*)s->buf[s->cur];
   T* B = (T*)s->buf[s->cur ^ 1];
@@ -416,23 +431,24 @@ static vl_status launch_step(vl_sim* s, const StepCtl& ctl, double* prev_out) {
   const int64_t ns2 = (int64_t)std::max(t.nslots, 1) * 2;
   sa.fc_cur = t.fcontrib + (s->t & 1) * ns2;
   sa.fc_prev = t.fcontrib + ((s->t + 1) & 1) * ns2;
-  sa.prev_out = (t.ngroups > 0) ? prev_out : nullptr;
+  sa.ctls = s->d_ctl;
+  sa.kbase = kbase;
+  sa.k_local = k_local;
+  sa.hist = t.ngroups > 0 ? s->dhist : nullptr;
   const int64_t np1 = std::max(t.np, 1);
   sa.pv_cur = t.pvel + (s->t & 1) * np1;
   sa.pv_next = t.pvel + ((s->t + 1) & 1) * np1;
-  StepCtl c2 = ctl;
-  if (t.np == 0) c2.has_target = 0;
-  const bool timed = s->timing && s->ev_used + 2 <= (int64_t)s->ev.size();
+  const bool timed = timed_ok && s->timing && s->ev_used + 2 <= (int64_t)s->ev.size();
   if (!s->split) {
     if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
     hipLaunchKernelGGL((k_step<S, M>), dim3((unsigned)(sa.nlb + sa.xblocks * g.ny)), dim3(STEP_T), 0, s->stream, A,
-                       B, s->dflag, g, s->coll, t, c2, sa);
+                       B, s->dflag, g, s->coll, t, sa);
   } else {
     if (sa.nlb > 0) {
       VL_HIP(hipEventRecord(s->ev_fork, s->stream));
       VL_HIP(hipStreamWaitEvent(s->side, s->ev_fork, 0));
       hipLaunchKernelGGL((k_boundary<S, M>), dim3((unsigned)sa.nlb), dim3(STEP_T), 0, s->side, A, B, g, s->coll, t,
-                         c2, sa);
+                         sa);
       VL_HIP(hipEventRecord(s->ev_join, s->side));
     }
     if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
@@ -452,19 +468,19 @@ static vl_status launch_step(vl_sim* s, const StepCtl& ctl, double* prev_out) {
 }
 
 template <class S>
-static vl_status step_dispatch(vl_sim* s, const StepCtl& ctl, double* prev_out) {
+static vl_status step_dispatch(vl_sim* s, const int64_t* kbase, int k_local, bool timed_ok) {
   switch (s->coll.model) {
-    case MODEL_FORCED: return launch_step<S, MODEL_FORCED>(s, ctl, prev_out);
-    case MODEL_BGK: return launch_step<S, MODEL_BGK>(s, ctl, prev_out);
-    default: return launch_step<S, MODEL_TRT>(s, ctl, prev_out);
+    case MODEL_FORCED: return launch_step<S, MODEL_FORCED>(s, kbase, k_local, timed_ok);
+    case MODEL_BGK: return launch_step<S, MODEL_BGK>(s, kbase, k_local, timed_ok);
+    default: return launch_step<S, MODEL_TRT>(s, kbase, k_local, timed_ok);
   }
 }
 
-static vl_status one_step(vl_sim* s, const StepCtl& ctl, double* prev_out) {
+static vl_status one_step(vl_sim* s, const int64_t* kbase, int k_local, bool timed_ok) {
   switch (s->precision) {
-    case VL_F32: return step_dispatch<StoreF32>(s, ctl, prev_out);
-    case VL_F16: return step_dispatch<StoreF16>(s, ctl, prev_out);
-    default: return step_dispatch<StoreF64>(s, ctl, prev_out);
+    case VL_F32: return step_dispatch<StoreF32>(s, kbase, k_local, timed_ok);
+    case VL_F16: return step_dispatch<StoreF16>(s, kbase, k_local, timed_ok);
+    default: return step_dispatch<StoreF64>(s, kbase, k_local, timed_ok);
   }
 }
 
@@ -581,6 +597,11 @@ static void free_sim(vl_sim* s) {
                   (void*)t.pny, (void*)t.gfirst, (void*)t.fcontrib, (void*)t.drho, (void*)t.pvel})
     f(p);
   f(s->dhist);
+  f(s->d_ctl);
+  f(s->d_kbase);
+  if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+  for (hipGraphExec_t& ge : s->graph)
+    if (ge) (void)hipGraphExecDestroy(ge);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : s->ev_chunk)
     if (e) (void)hipEventDestroy(e);
@@ -725,7 +746,14 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   VL_HIP(hipMalloc((void**)&t.drho, sizeof(double)));
   VL_HIP(hipMemset(t.drho, 0, sizeof(double)));
   s->last_force.assign((size_t)t.ngroups * 2, 0.0);
-  VL_TRY(ensure_hist(s, 1));
+  VL_TRY(ensure_hist(s, RUN_CHUNK));  // fixed for the sim's lifetime: graphs hold the pointer
+  VL_HIP(hipMalloc((void**)&s->d_ctl, RUN_CHUNK * sizeof(StepCtl)));
+  VL_HIP(hipHostMalloc((void**)&s->h_ctl, RUN_CHUNK * sizeof(StepCtl), hipHostMallocDefault));
+  VL_HIP(hipMalloc((void**)&s->d_kbase, 2 * sizeof(int64_t)));
+  VL_HIP(hipMemset(s->d_kbase, 0, 2 * sizeof(int64_t)));
+  s->d_kzero = s->d_kbase + 1;
+  s->use_graphs = true;
+  if (const char* e = std::getenv("VL_GRAPHS")) s->use_graphs = e[0] != '0';
   // rest state, as Sim::from_builder leaves it (src/sim.rs:352)
   std::vector<char> host(s->pop_bytes, 0);
   for (int64_t y = 0; y < d.ny; ++y)
@@ -736,7 +764,6 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   return VL_OK;
 }
 
-static constexpr int64_t RUN_CHUNK = 4096;
 
 static vl_status ensure_events(vl_sim* s) {
   if (!s->ev_chunk[0]) {
@@ -767,21 +794,55 @@ static vl_status collect_times(vl_sim* s, int64_t steps) {
   return VL_OK;
 }
 
+// A window of GRAPH_K steps captured once per (buffer parity, step parity) and replayed: the
+// kernel arguments of a window do not change between replays (controls and history are
+// indexed through the device counter kbase), so no per-step host work remains.
+static constexpr int GRAPH_K = 32;
+
+static vl_status get_window_graph(vl_sim* s, hipGraphExec_t* out) {
+  const int key = (s->cur & 1) * 2 + (int)(s->t & 1);
+  if (s->graph[key]) {
+    *out = s->graph[key];
+    return VL_OK;
+  }
+  const int cur0 = s->cur;
+  const int64_t t0 = s->t;
+  hipGraph_t gph = nullptr;
+  VL_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  vl_status st = VL_OK;
+  for (int k = 0; k < GRAPH_K && st == VL_OK; ++k) st = one_step(s, s->d_kbase, k, false);
+  if (st == VL_OK) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s->stream, s->d_kbase, (int64_t)GRAPH_K);
+  hipError_t e = hipStreamEndCapture(s->stream, &gph);
+  s->cur = cur0;  // capture recorded the steps without running them
+  s->t = t0;
+  if (st != VL_OK) {
+    if (gph) (void)hipGraphDestroy(gph);
+    return st;
+  }
+  if (e != hipSuccess) return set_error(VL_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&s->graph[key], gph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gph);
+  if (e != hipSuccess) {
+    s->graph[key] = nullptr;
+    return set_error(VL_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  *out = s->graph[key];
+  return VL_OK;
+}
+
 static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist) {
   if (nsteps < 0) return set_error(VL_EINVAL, "negative step count");
   const int64_t ng = s->dt.ngroups;
   const int64_t chunk = RUN_CHUNK;
-  VL_TRY(ensure_hist(s, std::min<int64_t>(std::max<int64_t>(nsteps, 1), chunk)));
   if (s->timing) VL_TRY(ensure_events(s));
   if (nsteps > 0 && s->dt.np > 0) VL_TRY(prime_outlet(s));
+  // Graph replay is used without kernel timing (events cannot sit inside the window).
+  const bool graphs = s->use_graphs && !s->timing;
   for (int64_t k0 = 0; k0 < nsteps; k0 += chunk) {
     const int64_t m = std::min(chunk, nsteps - k0);
-    if (s->timing) {
-      s->ev_used = 0;
-      VL_HIP(hipEventRecord(s->ev_chunk[0], s->stream));
-    }
+    // controls of this chunk -> pinned staging -> device table
     for (int64_t k = 0; k < m; ++k) {
-      StepCtl c;
+      StepCtl& c = s->h_ctl[k];
       if (ctl) {
         for (int g = 0; g < 4; ++g) c.vscale[g] = ctl[k0 + k].vscale[g];
         c.target = ctl[k0 + k].target;
@@ -791,14 +852,25 @@ static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double*
         c.target = s->target;
         c.has_target = s->has_target;
       }
-      // step k also reduces the force terms of step k-1 into its history slot
-      VL_TRY(one_step(s, c, (ng > 0 && k > 0) ? s->dhist + (k - 1) * ng * 2 : nullptr));
-      if (k0 + k == nsteps - 1) {
-        s->has_target = c.has_target;
-        s->target = c.target;
-        for (int g = 0; g < 4; ++g) s->vscale[g] = c.vscale[g];
+      if (s->dt.np == 0) c.has_target = 0;
+    }
+    VL_HIP(hipMemcpyAsync(s->d_ctl, s->h_ctl, (size_t)m * sizeof(StepCtl), hipMemcpyHostToDevice, s->stream));
+    if (s->timing) {
+      s->ev_used = 0;
+      VL_HIP(hipEventRecord(s->ev_chunk[0], s->stream));
+    }
+    int64_t k = 0;
+    if (graphs && m >= GRAPH_K) {
+      VL_HIP(hipMemsetAsync(s->d_kbase, 0, sizeof(int64_t), s->stream));
+      for (; k + GRAPH_K <= m; k += GRAPH_K) {
+        hipGraphExec_t ge;
+        VL_TRY(get_window_graph(s, &ge));
+        VL_HIP(hipGraphLaunch(ge, s->stream));
+        s->cur ^= (GRAPH_K & 1);
+        s->t += GRAPH_K;
       }
     }
+    for (; k < m; ++k) VL_TRY(one_step(s, s->d_kzero, (int)k, true));  // remainder, launched directly
     if (ng > 0) VL_TRY(finish_forces(s, s->dhist + (m - 1) * ng * 2));
     if (s->timing) VL_HIP(hipEventRecord(s->ev_chunk[1], s->stream));
     if (ng > 0) {
@@ -810,6 +882,10 @@ static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double*
     }
     VL_HIP(hipStreamSynchronize(s->stream));
     if (s->timing) VL_TRY(collect_times(s, m));
+    const StepCtl& last = s->h_ctl[m - 1];
+    s->has_target = last.has_target;
+    s->target = last.target;
+    for (int g = 0; g < 4; ++g) s->vscale[g] = last.vscale[g];
   }
   if (nsteps > 0 && s->has_target && s->dt.np > 0)
     VL_HIP(hipMemcpy(&s->outlet_density_offset, s->dt.drho, sizeof(double), hipMemcpyDeviceToHost));

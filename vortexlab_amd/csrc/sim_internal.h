@@ -83,7 +83,13 @@ struct vl_sim {
   size_t pop_bytes = 0;  // bytes of one population array
   uint8_t* dflag = nullptr;
   vl::DevTables dt;
-  double* dhist = nullptr;  // device force history (hist_cap steps)
+  double* dhist = nullptr;  // device force history of a run chunk (hist_cap steps)
+  vl::StepCtl* d_ctl = nullptr;  // controls of a run chunk (device)
+  vl::StepCtl* h_ctl = nullptr;  // pinned staging for them
+  int64_t* d_kbase = nullptr;    // window counter for replayed step graphs
+  int64_t* d_kzero = nullptr;    // a device zero (directly launched steps)
+  bool use_graphs = true;
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // by (buffer, step) parity
   int64_t hist_cap = 0;
   std::vector<double> last_force;  // Sim::force after the last step
   int64_t device_bytes = 0;
