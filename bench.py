@@ -98,7 +98,7 @@ def cpu_baseline(api, seconds_target, scale):
     log(f"[cpu] {steps} steps in {secs:.1f} s = {g * 1e3:.1f} MLUPS")
     return {"value": round(g, 5), "unit": "GLUPS", "cores": threads, "kind": "port",
             "sample": f"oracle (C++ restatement of the reference solver), {steps} steps of the "
-                      f"{8192 // scale}x{4096 // scale} domain from rest, {threads} threads, {secs:.1f} s"}
+                      f"{8192 // scale}x{4096 // scale} domain from t=0, {threads} threads, {secs:.1f} s"}
 
 
 def main():
