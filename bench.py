@@ -101,6 +101,65 @@ def cpu_baseline(api, seconds_target, scale):
                       f"{8192 // scale}x{4096 // scale} domain from t=0, {threads} threads, {secs:.1f} s"}
 
 
+def batch_bench(vl, api, ncases, seed, sample):
+    """The batched sweep: `ncases` seeded towing-tank cases (random star polygons, grid
+    200-1400 x 80-600, D 10-60, Re 40-250, BGK/TRT, u = 0.05, 20 D/U each) in FP32 storage, one
+    batch on one GPU; then `sample` of the same cases run one at a time (single-case path, FP32,
+    and FP64 for the drag check)."""
+    from vortexlab_amd import workloads
+    cases = workloads.sweep_cases(seed, ncases)
+    be = vl.backend()
+    t0 = time.perf_counter()
+    b = api.Batch(be, api.F32)
+    for c in cases:
+        b.add(c.spec, c.steps, c.d)
+    b.build()
+    t_build = time.perf_counter() - t0
+    info = b.info()
+    log(f"[batch] {ncases} cases, {info.fluid_nodes / 1e6:.1f} M fluid nodes, {info.lattice_updates / 1e12:.3f} T "
+        f"updates; set-up {t_build:.1f} s; tiles {info.tile_w}x{info.tile_h}, grid {info.grid_bulk}")
+    t0 = time.perf_counter()
+    b.run()
+    t_run = time.perf_counter() - t0
+    results = [b.result(k) for k in range(ncases)]
+    diverged = sum(1 for r in results if r[2] != 0)
+    log(f"[batch] run {t_run:.2f} s = {ncases / t_run:.2f} cases/s, {info.lattice_updates / t_run / 1e9:.1f} GLUPS; "
+        f"{diverged} diverged")
+    rng = np.random.default_rng(seed + 1)
+    pick = sorted(rng.choice(ncases, size=min(sample, ncases), replace=False).tolist())
+    seq_secs, dcd = 0.0, []
+    for k in pick:
+        c = cases[k]
+        ch = vl.make_channel(c.spec, api.F32)
+        t0 = time.perf_counter()
+        try:
+            ch.run(c.steps, c.d)
+        except api.Diverged:
+            pass
+        seq_secs += time.perf_counter() - t0
+        ch.close()
+        coef, done, st = results[k]
+        if st == 0:
+            ref = vl.make_channel(c.spec, api.F64)
+            try:
+                rc = ref.run(c.steps, c.d)
+                h = c.steps // 2
+                dcd.append(abs(coef[h:, 0].mean() / rc[h:, 0].mean() - 1.0))
+            except api.Diverged:
+                pass
+            ref.close()
+    seq_rate = len(pick) / seq_secs
+    log(f"[batch] sequential single-case runs: {len(pick)} cases in {seq_secs:.2f} s = {seq_rate:.2f} cases/s; "
+        f"max |dCd| vs FP64 {max(dcd) if dcd else float('nan'):.2e} over {len(dcd)} cases")
+    b.close()
+    return {"batched_cases_per_s": round(ncases / t_run, 3), "batched_glups": round(info.lattice_updates / t_run / 1e9, 2),
+            "batched_cases": ncases, "batched_diverged": diverged, "batched_seconds": round(t_run, 2),
+            "batched_setup_seconds": round(t_build, 2), "batched_tile": f"{info.tile_w}x{info.tile_h}",
+            "sequential_cases_per_s": round(seq_rate, 3), "sequential_sample": len(pick),
+            "batched_vs_sequential": round((ncases / t_run) / seq_rate, 2),
+            "batched_max_dcd_vs_fp64": (max(dcd) if dcd else None), "batched_dcd_checked": len(dcd)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +169,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline sample length (0 = skip)")
     ap.add_argument("--scale", type=int, default=1, help="divide the domain by this (testing only)")
+    ap.add_argument("--batch-cases", type=int, default=512, help="cases of the batched sweep (0 = skip)")
+    ap.add_argument("--batch-sample", type=int, default=8, help="cases re-run one at a time for the comparison")
+    ap.add_argument("--batch-seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of "
@@ -149,6 +211,8 @@ def main():
     copy = ctypes.c_double()
     vl.backend().check(vl.backend().device_copy_bandwidth(1 << 30, 20, ctypes.byref(copy)))
     log(f"[bench] plain device copy (1 GiB): {copy.value:.0f} GB/s")
+    if args.batch_cases > 0:
+        extra.update(batch_bench(vl, api, args.batch_cases, args.batch_seed, args.batch_sample))
     cpu = cpu_baseline(api, args.cpu_seconds, args.scale) if args.cpu_seconds > 0 else None
     out = {
         "metric": METRIC, "value": round(main_run["glups"], 3), "unit": "GLUPS", "n_gpus": 1,

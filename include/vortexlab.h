@@ -221,6 +221,37 @@ vl_status vl_channel_run(vl_channel* c, int64_t nsteps, double d, double* coef);
 vl_status vl_channel_coefficients(vl_channel* c, double d, double* cd, double* cl);
 
 /* ------------------------------------------------------------------------------------
+ * Batched cases (no reference counterpart: the reference runs one case at a time,
+ * src/cases/tunnel.rs:152-192).  Each case is a Channel (spec + body outline) run for its own
+ * number of steps; all cases step together on one GPU through two persistent kernels over a
+ * packed tile table, in replayed step-window graphs.  A case's results are bit-identical to
+ * the same case run alone with vl_channel_run in the same precision.
+ * ---------------------------------------------------------------------------------- */
+typedef struct vl_batch vl_batch;
+typedef struct {
+  int64_t ncases, total_nodes, fluid_nodes, lattice_updates, max_steps, t;
+  int32_t boundary_tiles, bulk_tiles, tile_w, tile_h, grid_bulk, grid_boundary;
+  double last_run_seconds;
+  int64_t device_bytes;
+  double autotune_ms[3][3]; /* bulk step time per (tile shape 64x4 / 128x2 / 256x1, grid 4/8/16 per CU) */
+} vl_batch_info;
+vl_status vl_batch_create(int precision, vl_batch** out);
+/* Adds a case (the channel of vl_channel_create; `body` consumed) that runs `steps` steps;
+ * coefficients use the reference length `d`.  Returns the case id, or -1. */
+int vl_batch_add(vl_batch* b, const vl_channel_spec* spec, vl_shape* body, int64_t steps, double d);
+/* Host set-up of every case (in parallel), device packing, and the tile-shape autotuner. */
+vl_status vl_batch_build(vl_batch* b);
+/* Advances every live case by up to max_steps steps (< 0: until all cases are done).  A case
+ * whose drag turns non-finite is retired (status 1) while the others go on. */
+vl_status vl_batch_run(vl_batch* b, int64_t max_steps);
+vl_status vl_batch_info_get(vl_batch* b, vl_batch_info* out);
+/* (cd, cl) per step of case `id` (2 * steps doubles), the steps completed, and its status
+ * (0 running or finished, 1 diverged). */
+vl_status vl_batch_case_result(vl_batch* b, int id, double* coef, int64_t* steps_done, int* status);
+vl_status vl_batch_read_case(vl_batch* b, int id, double* f);
+void vl_batch_destroy(vl_batch* b);
+
+/* ------------------------------------------------------------------------------------
  * Force-history statistics: src/cases/channel.rs:157-207 and src/signal.rs (host, FP64).
  * ---------------------------------------------------------------------------------- */
 typedef struct {

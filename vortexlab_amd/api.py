@@ -68,6 +68,15 @@ class StatsDesc(C.Structure):
                 ("periodic", C.c_int32), ("pad_", C.c_int32)]
 
 
+class BatchInfoDesc(C.Structure):
+    _fields_ = [("ncases", C.c_int64), ("total_nodes", C.c_int64), ("fluid_nodes", C.c_int64),
+                ("lattice_updates", C.c_int64), ("max_steps", C.c_int64), ("t", C.c_int64),
+                ("boundary_tiles", C.c_int32), ("bulk_tiles", C.c_int32), ("tile_w", C.c_int32),
+                ("tile_h", C.c_int32), ("grid_bulk", C.c_int32), ("grid_boundary", C.c_int32),
+                ("last_run_seconds", C.c_double), ("device_bytes", C.c_int64),
+                ("autotune_ms", (C.c_double * 3) * 3)]
+
+
 class VortexError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"[status {status}] {message}")
@@ -132,6 +141,14 @@ _SIGS = [
     ("dominant_frequency", C.c_int, [_D, C.c_size_t, C.c_double, _D]),
     ("crossing_frequency", C.c_int, [_D, C.c_size_t, C.c_double, _D, C.POINTER(C.c_int)]),
     ("fft", C.c_int, [_D, _D, C.c_size_t]),
+    ("batch_create", C.c_int, [C.c_int, C.POINTER(_VP)]),
+    ("batch_add", C.c_int, [_VP, C.POINTER(ChannelSpecDesc), _VP, C.c_int64, C.c_double]),
+    ("batch_build", C.c_int, [_VP]),
+    ("batch_run", C.c_int, [_VP, C.c_int64]),
+    ("batch_info_get", C.c_int, [_VP, C.POINTER(BatchInfoDesc)]),
+    ("batch_case_result", C.c_int, [_VP, C.c_int, _D, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    ("batch_read_case", C.c_int, [_VP, C.c_int, _D]),
+    ("batch_destroy", None, [_VP]),
 ]
 
 
@@ -522,19 +539,24 @@ class ChannelSpec:
     kick: Optional[Kick] = None
 
 
+def _spec_desc(spec: ChannelSpec) -> ChannelSpecDesc:
+    d = ChannelSpecDesc()
+    d.nx, d.ny, d.inflow, d.collision = spec.nx, spec.ny, spec.inflow, spec.collision
+    d.u, d.nu, d.magic, d.ramp_steps = spec.u, spec.nu, spec.magic, spec.ramp_steps
+    if spec.kick is not None:
+        d.has_kick = 1
+        k = spec.kick
+        d.kick.cx, d.kick.cy, d.kick.omega, d.kick.start, d.kick.duration = (k.cx, k.cy, k.omega, k.start,
+                                                                             k.duration)
+    return d
+
+
 class Channel:
     """A body in a channel with its per-step driver (src/cases/channel.rs:78-155)."""
 
     def __init__(self, be: Backend, spec: ChannelSpec, arg: int = 0):
         self.be = be
-        d = ChannelSpecDesc()
-        d.nx, d.ny, d.inflow, d.collision = spec.nx, spec.ny, spec.inflow, spec.collision
-        d.u, d.nu, d.magic, d.ramp_steps = spec.u, spec.nu, spec.magic, spec.ramp_steps
-        if spec.kick is not None:
-            d.has_kick = 1
-            k = spec.kick
-            d.kick.cx, d.kick.cy, d.kick.omega, d.kick.start, d.kick.duration = (k.cx, k.cy, k.omega, k.start,
-                                                                                 k.duration)
+        d = _spec_desc(spec)
         out = C.c_void_p()
         be.check(be.channel_create(C.byref(d), spec.body.make(be), arg, C.byref(out)))
         self.h = out.value
@@ -559,6 +581,58 @@ class Channel:
         if self.h:
             self.sim.h = None
             self.be.channel_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many channel cases stepped together on one GPU (GPU library only)."""
+
+    def __init__(self, be: Backend, precision: int = F32):
+        self.be = be
+        out = C.c_void_p()
+        be.check(be.batch_create(precision, C.byref(out)))
+        self.h = out.value
+        self.steps = []
+
+    def add(self, spec: ChannelSpec, steps: int, d: float) -> int:
+        k = self.be.batch_add(self.h, C.byref(_spec_desc(spec)), spec.body.make(self.be), int(steps), float(d))
+        if k < 0:
+            self.be.check(EINVAL)
+        self.steps.append(int(steps))
+        return k
+
+    def build(self):
+        self.be.check(self.be.batch_build(self.h))
+
+    def run(self, max_steps: int = -1):
+        self.be.check(self.be.batch_run(self.h, int(max_steps)))
+
+    def info(self) -> BatchInfoDesc:
+        i = BatchInfoDesc()
+        self.be.check(self.be.batch_info_get(self.h, C.byref(i)))
+        return i
+
+    def result(self, k: int):
+        """(coef [steps, 2], steps_done, status) of case k."""
+        coef = np.zeros((self.steps[k], 2))
+        done, st = C.c_int64(), C.c_int()
+        self.be.check(self.be.batch_case_result(self.h, k, _dptr(coef), C.byref(done), C.byref(st)))
+        return coef, int(done.value), int(st.value)
+
+    def read_case(self, k: int, nx: int, ny: int) -> np.ndarray:
+        out = np.zeros((Q, ny, nx))
+        self.be.check(self.be.batch_read_case(self.h, k, _dptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            self.be.batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

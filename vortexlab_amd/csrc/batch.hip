@@ -1,0 +1,877 @@
+// Batched solver: hundreds of independent body-in-channel cases of uneven size stepped
+// together by two persistent kernels over a packed tile table (no reference counterpart: the
+// reference runs one case at a time, src/cases/tunnel.rs:152-192).
+//
+// Layout: every case keeps its own pitched planes (as a single-domain vl_sim would) inside two
+// shared population pools; its boundary tables sit in packed pools and a BatchCase record holds
+// pointers into them.  The tile table lists boundary tiles (256 boundary nodes of one case) and
+// bulk tiles (TW x TH interior nodes of one case), each sorted by the case's step count so the
+// tiles still running at step t are a prefix; two device arrays give that prefix per step.
+//
+// Per node the arithmetic is the single-domain solver's (step_device.h), the per-step controls
+// are the Channel driver's (ramp computed on device with the same operations, kick taken from a
+// host-computed table), so a case in a batch gives the same bits as the same case run alone.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "step_device.h"
+
+namespace vl {
+
+struct BatchCase {
+  Geom g;
+  Coll c;
+  DevTables t;          // pointers into the packed table pools
+  int64_t pop_off;      // element offset of the case's array in each population pool
+  int64_t flag_off;     // byte offset of the case's flags
+  double u_ref, coef_k;  // coefficients = coef_k * force  (2 / (u_ref^2 d))
+  uint64_t ramp_steps;
+  int64_t kick_start, kick_dur, kick_off;  // kick values for [start, start+dur) at kick_off
+  int64_t steps;        // steps this case runs
+  int64_t hist_off;     // (cd, cl) history offset, in doubles
+  int32_t bnd_tile0;    // its first boundary tile
+  int32_t absorbing;
+  int32_t inflow;       // initial state when ramp_steps == 0
+  int32_t pad_;
+  double inflow_u, inflow_h;
+};
+
+constexpr int BT = 256;  // threads of both batch kernels
+
+__device__ inline double ramp_dev(uint64_t t, uint64_t steps) {  // src/cases/mod.rs:11-18
+  if (steps == 0 || t >= steps) return 1.0;
+  const double s = (double)t / (double)steps;
+  return s * s * (3.0 - 2.0 * s);
+}
+
+template <class S>
+__global__ __launch_bounds__(BT) void k_batch_boundary(const typename S::T* __restrict__ A,
+                                                       typename S::T* __restrict__ B,
+                                                       const BatchCase* __restrict__ cases,
+                                                       const int32_t* __restrict__ tcase,
+                                                       const int32_t* __restrict__ tb0,
+                                                       const int32_t* __restrict__ act, const int64_t* tbase,
+                                                       int t_local, const double* __restrict__ kick_tab,
+                                                       double* __restrict__ hist, int32_t* __restrict__ status,
+                                                       int64_t* __restrict__ done, double sqrt_cs2) {
+  __shared__ double red[BT];
+  __shared__ double fsum[2];
+  __shared__ int skip;
+  const int64_t t = *tbase + t_local;
+  const int nact = act[t];
+  for (int i = blockIdx.x; i < nact; i += gridDim.x) {
+    const int ci = tcase[i];
+    if (threadIdx.x == 0) skip = status[ci];
+    __syncthreads();
+    const int sk = skip;
+    __syncthreads();
+    if (sk) continue;
+    const BatchCase& cs = cases[ci];
+    const int64_t ns2 = (int64_t)max(cs.t.nslots, 1) * 2;
+    const int64_t np1 = max(cs.t.np, 1);
+    // force terms of the previous step -> coefficient history (and the divergence check)
+    if (i == cs.bnd_tile0 && t >= 1 && t - 1 < cs.steps) {
+      double cd = 0.0, cl = 0.0;
+      if (cs.t.ngroups > 0) {
+        reduce_forces(cs.t.fcontrib + ((t + 1) & 1) * ns2, cs.t.gfirst, 1, fsum, red);
+        cd = cs.coef_k * fsum[0];
+        cl = cs.coef_k * fsum[1];
+      }
+      if (threadIdx.x == 0) {
+        hist[cs.hist_off + 2 * (t - 1)] = cd;
+        hist[cs.hist_off + 2 * (t - 1) + 1] = cl;
+        done[ci] = t;
+        if (!isfinite(cd)) status[ci] = 1;
+      }
+    }
+    if (t < cs.steps) {
+      StepCtl ctl;
+      const double r = ramp_dev((uint64_t)t, cs.ramp_steps);
+      ctl.vscale[0] = 1.0;
+      ctl.vscale[1] = r;
+      ctl.vscale[2] = (t >= cs.kick_start && t < cs.kick_start + cs.kick_dur) ? kick_tab[cs.kick_off + t - cs.kick_start]
+                                                                             : 0.0;
+      ctl.vscale[3] = 1.0;
+      ctl.target = cs.u_ref * r;
+      ctl.has_target = cs.absorbing && cs.t.np > 0;
+      const int b0 = tb0[i];
+      double drho = 0.0;
+      if (ctl.has_target && b0 < cs.t.npb)
+        drho = outlet_offset<S>(cs.t.pvel + (t & 1) * np1, cs.t.np, ctl.target, sqrt_cs2, red);
+      if (b0 == 0 && threadIdx.x == 0) *cs.t.drho = drho;
+      const int b = b0 + (int)threadIdx.x;
+      if (b < cs.t.nb) {
+        const typename S::T* a = A + cs.pop_off;
+        typename S::T* bb = B + cs.pop_off;
+        double* fc = cs.t.fcontrib + (t & 1) * ns2;
+        double* pvn = cs.t.pvel + ((t + 1) & 1) * np1;
+        if (cs.c.model == MODEL_BGK)
+          boundary_node<S, MODEL_BGK>(a, bb, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
+        else
+          boundary_node<S, MODEL_TRT>(a, bb, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
+      }
+    }
+    __syncthreads();  // `red` is reused by the next tile
+  }
+}
+
+// Bulk tiles: TW x TH interior nodes of one case per pass of a workgroup (TW * TH = BT).
+template <class S>
+__global__ __launch_bounds__(BT) void k_batch_bulk(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
+                                                   const uint8_t* __restrict__ flags,
+                                                   const BatchCase* __restrict__ cases,
+                                                   const int32_t* __restrict__ tcase, const int32_t* __restrict__ ty0,
+                                                   const int32_t* __restrict__ tx0, const int32_t* __restrict__ act,
+                                                   const int64_t* tbase, int t_local, int tw_log2,
+                                                   const int32_t* __restrict__ status) {
+  const int64_t t = *tbase + t_local;
+  const int nact = act[t];
+  const int tx = (int)threadIdx.x & ((1 << tw_log2) - 1);
+  const int ty = (int)threadIdx.x >> tw_log2;
+  for (int i = blockIdx.x; i < nact; i += gridDim.x) {
+    const int ci = tcase[i];
+    if (status[ci]) continue;
+    const BatchCase& cs = cases[ci];
+    const int x = tx0[i] + tx, y = ty0[i] + ty;
+    if (x >= cs.g.nx || y >= cs.g.ny) continue;
+    const int64_t idx = (int64_t)y * cs.g.pitch + x;
+    if (flags[cs.flag_off + idx] != 0) continue;
+    const typename S::T* a = A + cs.pop_off;
+    typename S::T* b = B + cs.pop_off;
+    typename S::C f[Q];
+    pull<S>(a, cs.g, x, y, f);
+    if (cs.c.model == MODEL_BGK)
+      collide<S, MODEL_BGK>(f, cs.c);
+    else
+      collide<S, MODEL_TRT>(f, cs.c);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) S::st_stream(b + q * cs.g.plane + idx, f[q]);
+  }
+}
+
+// Initial state of every case: rest, or the inflow profile when the case has no ramp
+// (Channel::new, src/cases/channel.rs:126-128); non-fluid cells at rest (src/sim.rs:362).
+template <class S>
+__global__ __launch_bounds__(BT) void k_batch_init(typename S::T* __restrict__ A, typename S::T* __restrict__ B,
+                                                   const uint8_t* __restrict__ flags,
+                                                   const BatchCase* __restrict__ cases, int ci) {
+  const BatchCase& cs = cases[ci];
+  const int64_t n = (int64_t)cs.g.pitch * cs.g.ny;
+  for (int64_t e = (int64_t)blockIdx.x * BT + threadIdx.x; e < n; e += (int64_t)gridDim.x * BT) {
+    const int y = (int)(e / cs.g.pitch), x = (int)(e - (int64_t)y * cs.g.pitch);
+    double ux = 0.0;
+    if (x < cs.g.nx && flags[cs.flag_off + e] != 2 && cs.ramp_steps == 0) {
+      if (cs.inflow == VL_INFLOW_PARABOLIC) {
+        const double h = cs.inflow_h, u_mean = cs.inflow_u;
+        const double s = fmin(fmax((double)y - 0.5, 0.0), h);
+        ux = 6.0 * u_mean * s * (h - s) / (h * h);
+      } else {
+        ux = cs.inflow_u;
+      }
+    }
+    double feq[Q];
+    Exact::equilibrium(1.0, ux, 0.0, feq);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      typename S::C v;
+      if constexpr (S::exact)
+        v = feq[q];
+      else
+        v = (float)(feq[q] - wgt(q));
+      S::st(A + cs.pop_off + q * cs.g.plane + e, v);
+      S::st(B + cs.pop_off + q * cs.g.plane + e, v);
+    }
+  }
+}
+
+__global__ void k_batch_advance(int64_t* tbase, int64_t by) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *tbase += by;
+}
+
+}  // namespace vl
+
+using namespace vl;
+
+// ==========================================================================================
+// Host side
+// ==========================================================================================
+struct vl_batch {
+  int precision = VL_F32;
+  size_t elem = 4;
+  struct Pending {
+    vl_channel_spec spec;
+    vl_shape* body;
+    int64_t steps;
+    double d;
+  };
+  std::vector<Pending> pending;
+  bool built = false;
+  // per case (host)
+  std::vector<BatchCase> hcases;
+  std::vector<Tables> tabs;
+  int64_t total_nodes = 0, total_fluid = 0, total_lu = 0, max_steps = 0;
+  // device
+  hipStream_t stream = nullptr, side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  void* pool[2] = {nullptr, nullptr};
+  uint8_t* dflags = nullptr;
+  BatchCase* dcases = nullptr;
+  std::vector<void*> allocs;  // table pools etc.
+  int32_t *d_bcase = nullptr, *d_bb0 = nullptr, *d_bact = nullptr;
+  int32_t *d_kcase = nullptr, *d_ky0 = nullptr, *d_kx0 = nullptr, *d_kact = nullptr;
+  int32_t* d_status = nullptr;
+  int64_t* d_done = nullptr;
+  int64_t* d_tbase = nullptr;  // [0] window counter, [1] a zero
+  double* d_hist = nullptr;
+  double* d_kick = nullptr;
+  int nbnd_tiles = 0, nbulk_tiles = 0;
+  int tw_log2 = 6, th = 4, grid_bulk = 0, grid_bnd = 0;
+  int cur = 0;
+  int64_t t = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  bool use_graphs = true;
+  double autotune_ms[3][3] = {};
+  double last_run_seconds = 0.0;
+  int64_t device_bytes = 0;
+};
+
+namespace {
+
+constexpr int GRAPH_W = 32;
+const double SQRT_CS2_B = std::sqrt(1.0 / 3.0);
+
+int64_t rup(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+template <class V>
+vl_status dupload(vl_batch* b, V*& dst, const std::vector<V>& src) {
+  const size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(V);
+  VL_HIP(hipMalloc((void**)&dst, bytes));
+  b->allocs.push_back((void*)dst);
+  b->device_bytes += (int64_t)bytes;
+  if (!src.empty()) VL_HIP(hipMemcpy(dst, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice));
+  return VL_OK;
+}
+
+template <class V>
+vl_status dalloc(vl_batch* b, V*& dst, size_t n) {
+  VL_HIP(hipMalloc((void**)&dst, std::max<size_t>(n, 1) * sizeof(V)));
+  b->allocs.push_back((void*)dst);
+  b->device_bytes += (int64_t)(std::max<size_t>(n, 1) * sizeof(V));
+  return VL_OK;
+}
+
+#define VL_TRYB(x)                \
+  do {                            \
+    vl_status st_ = (x);          \
+    if (st_ != VL_OK) return st_; \
+  } while (0)
+
+// Host set-up of one case: the channel domain, its boundary tables, outlet nodes first.
+vl_status prepare_case(const vl_batch::Pending& p, Tables& tab, vl_velocity& inlet, int32_t& npb) {
+  vl_builder* b = channel_builder(&p.spec, p.body, &inlet);
+  if (!b) return VL_EINVAL;
+  vl_status st = build_tables(b->d, tab);
+  vl_builder_free(b);
+  if (st != VL_OK) return st;
+  npb = outlet_nodes_first(tab);
+  if (!outlet_nodes_match(tab, npb)) return set_error(VL_EINVAL, "internal: outlet-node order");
+  return VL_OK;
+}
+
+// Bulk tile table for a tile shape: every (TW x TH) block of every case that holds at least one
+// interior node, cases in descending step order.
+void bulk_tiles(const vl_batch* b, const std::vector<int>& order, int tw, int th, std::vector<int32_t>& tc,
+                std::vector<int32_t>& ty, std::vector<int32_t>& tx, std::vector<int32_t>& case_end) {
+  tc.clear();
+  ty.clear();
+  tx.clear();
+  case_end.assign(b->hcases.size(), 0);
+  for (int ci : order) {
+    const Tables& tab = b->tabs[ci];
+    const int64_t nx = tab.nx, ny = tab.ny;
+    for (int64_t y0 = 0; y0 < ny; y0 += th)
+      for (int64_t x0 = 0; x0 < nx; x0 += tw) {
+        bool any = false;
+        for (int64_t y = y0; y < std::min(ny, y0 + th) && !any; ++y)
+          for (int64_t x = x0; x < std::min(nx, x0 + tw) && !any; ++x) any = tab.flag[y * nx + x] == 0;
+        if (!any) continue;
+        tc.push_back(ci);
+        ty.push_back((int32_t)y0);
+        tx.push_back((int32_t)x0);
+      }
+    case_end[ci] = (int32_t)tc.size();
+  }
+}
+
+template <class S>
+vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local) {
+  using T = typename S::T;
+  const T* A = (const T*)b->pool[b->cur];
+  T* B = (T*)b->pool[b->cur ^ 1];
+  VL_HIP(hipEventRecord(b->ev_fork, b->stream));
+  VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+  hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, b->side, A, B, b->dcases,
+                     b->d_bcase, b->d_bb0, b->d_bact, tbase, t_local, b->d_kick, b->d_hist, b->d_status, b->d_done,
+                     SQRT_CS2_B);
+  VL_HIP(hipEventRecord(b->ev_join, b->side));
+  hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)b->grid_bulk), dim3(BT), 0, b->stream, A, B, b->dflags,
+                     b->dcases, b->d_kcase, b->d_ky0, b->d_kx0, b->d_kact, tbase, t_local, b->tw_log2, b->d_status);
+  VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+  VL_HIP(hipGetLastError());
+  b->cur ^= 1;
+  b->t += 1;
+  return VL_OK;
+}
+
+vl_status batch_step(vl_batch* b, const int64_t* tbase, int t_local) {
+  switch (b->precision) {
+    case VL_F64: return launch_batch_step<StoreF64>(b, tbase, t_local);
+    case VL_F16: return launch_batch_step<StoreF16>(b, tbase, t_local);
+    default: return launch_batch_step<StoreF32>(b, tbase, t_local);
+  }
+}
+
+template <class S>
+vl_status launch_bulk_only(vl_batch* b, int grid, int tw_log2, const int32_t* kc, const int32_t* ky,
+                           const int32_t* kx, const int32_t* kact) {
+  using T = typename S::T;
+  hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)grid), dim3(BT), 0, b->stream, (const T*)b->pool[b->cur],
+                     (T*)b->pool[b->cur ^ 1], b->dflags, b->dcases, kc, ky, kx, kact, b->d_tbase + 1, 0, tw_log2,
+                     b->d_status);
+  VL_HIP(hipGetLastError());
+  return VL_OK;
+}
+
+vl_status bulk_only(vl_batch* b, int grid, int tw_log2, const int32_t* kc, const int32_t* ky, const int32_t* kx,
+                    const int32_t* kact) {
+  switch (b->precision) {
+    case VL_F64: return launch_bulk_only<StoreF64>(b, grid, tw_log2, kc, ky, kx, kact);
+    case VL_F16: return launch_bulk_only<StoreF16>(b, grid, tw_log2, kc, ky, kx, kact);
+    default: return launch_bulk_only<StoreF32>(b, grid, tw_log2, kc, ky, kx, kact);
+  }
+}
+
+void free_batch(vl_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->side) (void)hipStreamSynchronize(b->side);
+  for (auto& p : b->pending)
+    if (p.body) vl_shape_free(p.body);
+  for (void* p : b->allocs) (void)hipFree(p);
+  if (b->pool[0]) (void)hipFree(b->pool[0]);
+  if (b->pool[1]) (void)hipFree(b->pool[1]);
+  for (auto& g : b->graph)
+    if (g) (void)hipGraphExecDestroy(g);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+  if (b->side) (void)hipStreamDestroy(b->side);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+vl_status build(vl_batch* b) {
+  const size_t nc = b->pending.size();
+  if (nc == 0) return set_error(VL_EINVAL, "batch has no cases");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(VL_EDEVICE, "no HIP device");
+  // ---- host set-up of every case, in parallel ----
+  b->tabs.assign(nc, Tables{});
+  std::vector<vl_velocity> inlet(nc);
+  std::vector<int32_t> npb(nc, 0);
+  std::vector<vl_status> st(nc, VL_OK);
+  std::vector<std::string> err(nc);
+  {
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nth; ++w)
+      pool.emplace_back([&, w] {
+        for (size_t i = w; i < nc; i += nth) {
+          st[i] = prepare_case(b->pending[i], b->tabs[i], inlet[i], npb[i]);
+          b->pending[i].body = nullptr;  // consumed by the builder
+          if (st[i] != VL_OK) err[i] = vl_last_error();
+        }
+      });
+    for (auto& th : pool) th.join();
+  }
+  for (size_t i = 0; i < nc; ++i)
+    if (st[i] != VL_OK) return set_error(st[i], "case " + std::to_string(i) + ": " + err[i]);
+  // ---- packing ----
+  b->hcases.assign(nc, BatchCase{});
+  int64_t pop_elems = 0, flag_bytes = 0;
+  std::vector<double> kick_tab;
+  int64_t hist_len = 0;
+  for (size_t i = 0; i < nc; ++i) {
+    const auto& p = b->pending[i];
+    const Tables& tab = b->tabs[i];
+    BatchCase& c = b->hcases[i];
+    c.g.nx = (int32_t)tab.nx;
+    c.g.ny = (int32_t)tab.ny;
+    c.g.pitch = (int32_t)rup(tab.nx, 64);
+    c.g.periodic_x = c.g.periodic_y = 0;
+    c.g.plane = rup((int64_t)c.g.pitch * tab.ny, 64);
+    double op, om;
+    const double tau = p.spec.nu / (1.0 / 3.0) + 0.5;
+    omegas(p.spec.collision, p.spec.magic, tau, op, om);
+    c.c = Coll{op, om, 0.0, 0.0, (float)op, (float)om, 0.0f, 0.0f, p.spec.collision == VL_BGK ? MODEL_BGK : MODEL_TRT};
+    c.pop_off = pop_elems;
+    pop_elems += Q * c.g.plane;
+    c.flag_off = flag_bytes;
+    flag_bytes += (int64_t)c.g.pitch * tab.ny;
+    c.u_ref = p.spec.u;
+    c.coef_k = 2.0 / (p.spec.u * p.spec.u * p.d);  // Channel::coefficients
+    c.ramp_steps = p.spec.ramp_steps;
+    c.kick_start = p.spec.has_kick ? (int64_t)p.spec.kick.start : 0;
+    c.kick_dur = p.spec.has_kick ? (int64_t)p.spec.kick.duration : 0;
+    c.kick_off = (int64_t)kick_tab.size();
+    for (int64_t k = 0; k < c.kick_dur; ++k)
+      kick_tab.push_back(channel_kick(p.spec.kick, true, (uint64_t)(c.kick_start + k)));
+    c.steps = p.steps;
+    c.hist_off = hist_len;
+    hist_len += 2 * p.steps;
+    c.absorbing = 1;
+    c.inflow = p.spec.inflow;
+    c.inflow_u = p.spec.u;
+    c.inflow_h = (double)(tab.ny - 2);
+    b->total_nodes += (int64_t)tab.nx * tab.ny;
+    b->total_fluid += tab.fluid_nodes;
+    b->total_lu += tab.fluid_nodes * p.steps;
+    b->max_steps = std::max(b->max_steps, p.steps);
+    (void)inlet;
+  }
+  VL_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  VL_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  VL_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+  VL_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+  VL_HIP(hipMalloc(&b->pool[0], (size_t)pop_elems * b->elem));
+  VL_HIP(hipMalloc(&b->pool[1], (size_t)pop_elems * b->elem));
+  b->device_bytes += 2 * pop_elems * (int64_t)b->elem;
+  // flags (pitched; padding columns are "not fluid")
+  {
+    std::vector<uint8_t> fl((size_t)flag_bytes, 2);
+    for (size_t i = 0; i < nc; ++i) {
+      const Tables& tab = b->tabs[i];
+      const BatchCase& c = b->hcases[i];
+      for (int64_t y = 0; y < tab.ny; ++y)
+        std::memcpy(&fl[(size_t)(c.flag_off + y * c.g.pitch)], &tab.flag[(size_t)(y * tab.nx)], (size_t)tab.nx);
+    }
+    VL_TRYB(dupload(b, b->dflags, fl));
+  }
+  // packed boundary tables
+  {
+    std::vector<int32_t> bx, by, bfirst, fslot, px, py, gfirst;
+    std::vector<uint8_t> dir, kind, vgroup, lflags;
+    std::vector<int8_t> lnx, lny, pnx, pny;
+    std::vector<double> q, uwx, uwy, cu6, rho;
+    std::vector<int64_t> o_b(nc), o_bf(nc), o_l(nc), o_p(nc), o_g(nc), o_fc(nc), o_pv(nc);
+    int64_t fc_len = 0, pv_len = 0;
+    for (size_t i = 0; i < nc; ++i) {
+      const Tables& tab = b->tabs[i];
+      o_b[i] = (int64_t)bx.size();
+      o_bf[i] = (int64_t)bfirst.size();
+      o_l[i] = (int64_t)dir.size();
+      o_p[i] = (int64_t)px.size();
+      o_g[i] = (int64_t)gfirst.size();
+      bx.insert(bx.end(), tab.bnode_x.begin(), tab.bnode_x.end());
+      by.insert(by.end(), tab.bnode_y.begin(), tab.bnode_y.end());
+      bfirst.insert(bfirst.end(), tab.bnode_first.begin(), tab.bnode_first.end());  // case-local link ids
+      dir.insert(dir.end(), tab.l_dir.begin(), tab.l_dir.end());
+      kind.insert(kind.end(), tab.l_kind.begin(), tab.l_kind.end());
+      vgroup.insert(vgroup.end(), tab.l_vgroup.begin(), tab.l_vgroup.end());
+      lflags.insert(lflags.end(), tab.l_flags.begin(), tab.l_flags.end());
+      lnx.insert(lnx.end(), tab.l_nx.begin(), tab.l_nx.end());
+      lny.insert(lny.end(), tab.l_ny.begin(), tab.l_ny.end());
+      fslot.insert(fslot.end(), tab.l_fslot.begin(), tab.l_fslot.end());
+      q.insert(q.end(), tab.l_q.begin(), tab.l_q.end());
+      uwx.insert(uwx.end(), tab.l_uwx.begin(), tab.l_uwx.end());
+      uwy.insert(uwy.end(), tab.l_uwy.begin(), tab.l_uwy.end());
+      cu6.insert(cu6.end(), tab.l_cu6.begin(), tab.l_cu6.end());
+      rho.insert(rho.end(), tab.l_rho.begin(), tab.l_rho.end());
+      px.insert(px.end(), tab.p_x.begin(), tab.p_x.end());
+      py.insert(py.end(), tab.p_y.begin(), tab.p_y.end());
+      pnx.insert(pnx.end(), tab.p_nx.begin(), tab.p_nx.end());
+      pny.insert(pny.end(), tab.p_ny.begin(), tab.p_ny.end());
+      gfirst.insert(gfirst.end(), tab.f_group_first.begin(), tab.f_group_first.end());
+      o_fc[i] = fc_len;
+      fc_len += 4 * std::max<int64_t>((int64_t)tab.f_link.size(), 1);
+      o_pv[i] = pv_len;
+      pv_len += 2 * std::max<int64_t>((int64_t)tab.p_x.size(), 1);
+    }
+    int32_t *d_bx, *d_by, *d_bf, *d_fs, *d_px, *d_py, *d_gf;
+    uint8_t *d_dir, *d_kind, *d_vg, *d_lf;
+    int8_t *d_lnx, *d_lny, *d_pnx, *d_pny;
+    double *d_q, *d_uwx, *d_uwy, *d_cu6, *d_rho, *d_fc, *d_pv, *d_drho;
+    VL_TRYB(dupload(b, d_bx, bx));
+    VL_TRYB(dupload(b, d_by, by));
+    VL_TRYB(dupload(b, d_bf, bfirst));
+    VL_TRYB(dupload(b, d_fs, fslot));
+    VL_TRYB(dupload(b, d_px, px));
+    VL_TRYB(dupload(b, d_py, py));
+    VL_TRYB(dupload(b, d_gf, gfirst));
+    VL_TRYB(dupload(b, d_dir, dir));
+    VL_TRYB(dupload(b, d_kind, kind));
+    VL_TRYB(dupload(b, d_vg, vgroup));
+    VL_TRYB(dupload(b, d_lf, lflags));
+    VL_TRYB(dupload(b, d_lnx, lnx));
+    VL_TRYB(dupload(b, d_lny, lny));
+    VL_TRYB(dupload(b, d_pnx, pnx));
+    VL_TRYB(dupload(b, d_pny, pny));
+    VL_TRYB(dupload(b, d_q, q));
+    VL_TRYB(dupload(b, d_uwx, uwx));
+    VL_TRYB(dupload(b, d_uwy, uwy));
+    VL_TRYB(dupload(b, d_cu6, cu6));
+    VL_TRYB(dupload(b, d_rho, rho));
+    std::vector<double> zeros_fc((size_t)fc_len, 0.0), zeros_pv((size_t)pv_len, 0.0), zeros_d(nc, 0.0);
+    VL_TRYB(dupload(b, d_fc, zeros_fc));
+    VL_TRYB(dupload(b, d_pv, zeros_pv));
+    VL_TRYB(dupload(b, d_drho, zeros_d));
+    for (size_t i = 0; i < nc; ++i) {
+      const Tables& tab = b->tabs[i];
+      DevTables& t = b->hcases[i].t;
+      t.nb = (int32_t)tab.bnode_x.size();
+      t.nl = (int32_t)tab.l_dir.size();
+      t.np = (int32_t)tab.p_x.size();
+      t.npb = npb[i];
+      t.nslots = (int32_t)tab.f_link.size();
+      t.ngroups = tab.force_groups;
+      if (t.ngroups > 1) return set_error(VL_EINVAL, "batched cases have one force group (the body)");
+      t.bx = d_bx + o_b[i];
+      t.by = d_by + o_b[i];
+      t.bfirst = d_bf + o_bf[i];
+      t.dir = d_dir + o_l[i];
+      t.kind = d_kind + o_l[i];
+      t.vgroup = d_vg + o_l[i];
+      t.lflags = d_lf + o_l[i];
+      t.lnx = d_lnx + o_l[i];
+      t.lny = d_lny + o_l[i];
+      t.fslot = d_fs + o_l[i];
+      t.q = d_q + o_l[i];
+      t.uwx = d_uwx + o_l[i];
+      t.uwy = d_uwy + o_l[i];
+      t.cu6 = d_cu6 + o_l[i];
+      t.rho = d_rho + o_l[i];
+      t.px = d_px + o_p[i];
+      t.py = d_py + o_p[i];
+      t.pnx = d_pnx + o_p[i];
+      t.pny = d_pny + o_p[i];
+      t.gfirst = d_gf + o_g[i];
+      t.fcontrib = d_fc + o_fc[i];
+      t.pvel = d_pv + o_pv[i];
+      t.drho = d_drho + i;
+    }
+  }
+  // ---- tile tables: cases in descending step order ----
+  std::vector<int> order(nc);
+  for (size_t i = 0; i < nc; ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return b->hcases[a].steps > b->hcases[c].steps; });
+  std::vector<int32_t> bc, bb0, bact((size_t)b->max_steps + 2, 0);
+  std::vector<int32_t> bnd_end(nc, 0);
+  for (int ci : order) {
+    BatchCase& c = b->hcases[ci];
+    c.bnd_tile0 = (int32_t)bc.size();
+    const int32_t nb = c.t.nb;
+    for (int32_t b0 = 0; b0 < std::max(nb, 1); b0 += BT) {  // at least one tile: it owns the force history
+      bc.push_back(ci);
+      bb0.push_back(b0);
+    }
+    bnd_end[ci] = (int32_t)bc.size();
+  }
+  b->nbnd_tiles = (int)bc.size();
+  // boundary tiles run for t <= steps (step `steps` only reduces the last step's forces)
+  for (int64_t tt = 0; tt <= b->max_steps + 1; ++tt) {
+    int32_t n = 0;
+    for (int ci : order)
+      if (b->hcases[ci].steps >= tt) n = bnd_end[ci];
+    bact[(size_t)tt] = n;
+  }
+  VL_TRYB(dupload(b, b->d_bcase, bc));
+  VL_TRYB(dupload(b, b->d_bb0, bb0));
+  VL_TRYB(dupload(b, b->d_bact, bact));
+  VL_TRYB(dupload(b, b->dcases, b->hcases));
+  VL_TRYB(dalloc(b, b->d_status, nc));
+  VL_TRYB(dalloc(b, b->d_done, nc));
+  VL_HIP(hipMemset(b->d_status, 0, nc * sizeof(int32_t)));
+  VL_HIP(hipMemset(b->d_done, 0, nc * sizeof(int64_t)));
+  VL_TRYB(dalloc(b, b->d_tbase, 2));
+  VL_HIP(hipMemset(b->d_tbase, 0, 2 * sizeof(int64_t)));
+  VL_TRYB(dalloc(b, b->d_hist, (size_t)std::max<int64_t>(hist_len, 2)));
+  VL_HIP(hipMemset(b->d_hist, 0, (size_t)std::max<int64_t>(hist_len, 2) * sizeof(double)));
+  if (kick_tab.empty()) kick_tab.push_back(0.0);
+  VL_TRYB(dupload(b, b->d_kick, kick_tab));
+  // ---- initial state ----
+  for (size_t i = 0; i < nc; ++i) {
+    switch (b->precision) {
+      case VL_F64:
+        hipLaunchKernelGGL(k_batch_init<StoreF64>, dim3(1024), dim3(BT), 0, b->stream, (double*)b->pool[0],
+                           (double*)b->pool[1], b->dflags, b->dcases, (int)i);
+        break;
+      case VL_F16:
+        hipLaunchKernelGGL(k_batch_init<StoreF16>, dim3(1024), dim3(BT), 0, b->stream, (_Float16*)b->pool[0],
+                           (_Float16*)b->pool[1], b->dflags, b->dcases, (int)i);
+        break;
+      default:
+        hipLaunchKernelGGL(k_batch_init<StoreF32>, dim3(1024), dim3(BT), 0, b->stream, (float*)b->pool[0],
+                           (float*)b->pool[1], b->dflags, b->dcases, (int)i);
+    }
+    VL_HIP(hipGetLastError());
+  }
+  // pressure-node velocities of the initial state: the boundary update writes them from then on
+  for (size_t i = 0; i < nc; ++i) {
+    const BatchCase& c = b->hcases[i];
+    if (c.t.np == 0) continue;
+    const unsigned grid = (unsigned)((c.t.np + 255) / 256);
+    switch (b->precision) {
+      case VL_F64:
+        hipLaunchKernelGGL(k_outlet_prime<StoreF64>, dim3(grid), dim3(256), 0, b->stream,
+                           (const double*)b->pool[0] + c.pop_off, c.g, c.c, c.t, c.t.pvel);
+        break;
+      case VL_F16:
+        hipLaunchKernelGGL(k_outlet_prime<StoreF16>, dim3(grid), dim3(256), 0, b->stream,
+                           (const _Float16*)b->pool[0] + c.pop_off, c.g, c.c, c.t, c.t.pvel);
+        break;
+      default:
+        hipLaunchKernelGGL(k_outlet_prime<StoreF32>, dim3(grid), dim3(256), 0, b->stream,
+                           (const float*)b->pool[0] + c.pop_off, c.g, c.c, c.t, c.t.pvel);
+    }
+    VL_HIP(hipGetLastError());
+  }
+  VL_HIP(hipStreamSynchronize(b->stream));
+  // ---- autotune the bulk tile shape and the persistent grid on the real batch ----
+  hipDeviceProp_t prop;
+  VL_HIP(hipGetDeviceProperties(&prop, 0));
+  const int cus = std::max(1, prop.multiProcessorCount);
+  b->grid_bnd = std::max(1, std::min(b->nbnd_tiles, cus * 2));
+  const int shapes[3][2] = {{6, 4}, {7, 2}, {8, 1}};  // (log2 TW, TH)
+  const int mults[3] = {4, 8, 16};
+  double best = 1e30;
+  std::vector<int32_t> kc, ky, kx, kend;
+  hipEvent_t e0, e1;
+  VL_HIP(hipEventCreate(&e0));
+  VL_HIP(hipEventCreate(&e1));
+  int best_s = 0, best_m = 1;
+  for (int si = 0; si < 3; ++si) {
+    bulk_tiles(b, order, 1 << shapes[si][0], shapes[si][1], kc, ky, kx, kend);
+    int32_t *tkc, *tky, *tkx, *tact;
+    std::vector<int32_t> act1(1, (int32_t)kc.size());
+    VL_TRYB(dupload(b, tkc, kc));
+    VL_TRYB(dupload(b, tky, ky));
+    VL_TRYB(dupload(b, tkx, kx));
+    VL_TRYB(dupload(b, tact, act1));
+    for (int mi = 0; mi < 3; ++mi) {
+      const int grid = std::max(1, std::min((int)kc.size(), cus * mults[mi]));
+      // A -> B with the same A every time: the state does not change
+      VL_TRYB(bulk_only(b, grid, shapes[si][0], tkc, tky, tkx, tact));
+      VL_HIP(hipEventRecord(e0, b->stream));
+      for (int r = 0; r < 3; ++r) VL_TRYB(bulk_only(b, grid, shapes[si][0], tkc, tky, tkx, tact));
+      VL_HIP(hipEventRecord(e1, b->stream));
+      VL_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      VL_HIP(hipEventElapsedTime(&ms, e0, e1));
+      b->autotune_ms[si][mi] = ms / 3.0;
+      if (ms < best) {
+        best = ms;
+        best_s = si;
+        best_m = mi;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->tw_log2 = shapes[best_s][0];
+  b->th = shapes[best_s][1];
+  bulk_tiles(b, order, 1 << b->tw_log2, b->th, kc, ky, kx, kend);
+  b->nbulk_tiles = (int)kc.size();
+  b->grid_bulk = std::max(1, std::min(b->nbulk_tiles, cus * mults[best_m]));
+  std::vector<int32_t> kact((size_t)b->max_steps + 2, 0);
+  for (int64_t tt = 0; tt <= b->max_steps + 1; ++tt) {
+    int32_t n = 0;
+    for (int ci : order)
+      if (b->hcases[ci].steps > tt) n = kend[ci];
+    kact[(size_t)tt] = n;
+  }
+  VL_TRYB(dupload(b, b->d_kcase, kc));
+  VL_TRYB(dupload(b, b->d_ky0, ky));
+  VL_TRYB(dupload(b, b->d_kx0, kx));
+  VL_TRYB(dupload(b, b->d_kact, kact));
+  b->use_graphs = true;
+  if (const char* e = std::getenv("VL_GRAPHS")) b->use_graphs = e[0] != '0';
+  b->built = true;
+  return VL_OK;
+}
+
+vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
+  const int key = b->cur & 1;
+  if (b->graph[key]) {
+    *out = b->graph[key];
+    return VL_OK;
+  }
+  const int cur0 = b->cur;
+  const int64_t t0 = b->t;
+  hipGraph_t g = nullptr;
+  VL_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  vl_status st = VL_OK;
+  for (int k = 0; k < GRAPH_W && st == VL_OK; ++k) st = batch_step(b, b->d_tbase, k);
+  if (st == VL_OK) hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, b->stream, b->d_tbase, (int64_t)GRAPH_W);
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  b->cur = cur0;
+  b->t = t0;
+  if (st != VL_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return set_error(VL_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&b->graph[key], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    b->graph[key] = nullptr;
+    return set_error(VL_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  *out = b->graph[key];
+  return VL_OK;
+}
+
+}  // namespace
+
+// ==========================================================================================
+// C ABI
+// ==========================================================================================
+extern "C" {
+
+vl_status vl_batch_create(int precision, vl_batch** out) {
+  if (!out) return set_error(VL_EINVAL, "null argument");
+  if (precision != VL_F64 && precision != VL_F32 && precision != VL_F16)
+    return set_error(VL_EINVAL, "precision must be VL_F64, VL_F32 or VL_F16");
+  auto* b = new vl_batch;
+  b->precision = precision;
+  b->elem = precision == VL_F64 ? 8 : (precision == VL_F32 ? 4 : 2);
+  *out = b;
+  return VL_OK;
+}
+
+int vl_batch_add(vl_batch* b, const vl_channel_spec* spec, vl_shape* body, int64_t steps, double d) {
+  if (!b || !spec || !body || b->built || steps < 1 || !(d > 0.0)) {
+    if (body) vl_shape_free(body);
+    set_error(VL_EINVAL, b && b->built ? "batch already built" : "bad case (spec, body, steps >= 1, d > 0)");
+    return -1;
+  }
+  b->pending.push_back({*spec, body, steps, d});
+  return (int)b->pending.size() - 1;
+}
+
+vl_status vl_batch_build(vl_batch* b) {
+  if (!b) return set_error(VL_EINVAL, "null batch");
+  if (b->built) return VL_OK;
+  vl_status st = build(b);
+  for (auto& p : b->pending)
+    if (p.body) {
+      vl_shape_free(p.body);
+      p.body = nullptr;
+    }
+  return st;
+}
+
+vl_status vl_batch_run(vl_batch* b, int64_t max_steps) {
+  if (!b) return set_error(VL_EINVAL, "null batch");
+  if (!b->built) VL_TRYB(vl_batch_build(b));
+  const int64_t end = b->max_steps + 1;  // the extra step reduces the last step's forces
+  int64_t todo = end - b->t;
+  if (max_steps >= 0) todo = std::min(todo, max_steps);
+  if (todo <= 0) return VL_OK;
+  const auto t0 = std::chrono::steady_clock::now();
+  int64_t k = 0;
+  if (b->use_graphs && todo >= GRAPH_W) {
+    // the window counter holds the global step of the window's first step
+    VL_HIP(hipMemcpyAsync(b->d_tbase, &b->t, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
+    VL_HIP(hipStreamSynchronize(b->stream));
+    for (; k + GRAPH_W <= todo; k += GRAPH_W) {
+      hipGraphExec_t ge;
+      VL_TRYB(window_graph(b, &ge));
+      VL_HIP(hipGraphLaunch(ge, b->stream));
+      b->t += GRAPH_W;
+    }
+  }
+  if (k < todo) {
+    // remaining steps launched directly; their base is the global step in a device scalar
+    VL_HIP(hipStreamSynchronize(b->stream));
+    VL_HIP(hipMemcpyAsync(b->d_tbase, &b->t, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
+    VL_HIP(hipStreamSynchronize(b->stream));
+    for (int kk = 0; k < todo; ++k, ++kk) VL_TRYB(batch_step(b, b->d_tbase, kk));
+  }
+  VL_HIP(hipStreamSynchronize(b->stream));
+  b->last_run_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return VL_OK;
+}
+
+vl_status vl_batch_info_get(vl_batch* b, vl_batch_info* o) {
+  if (!b || !o) return set_error(VL_EINVAL, "null argument");
+  std::memset(o, 0, sizeof(*o));
+  o->ncases = (int64_t)(b->built ? b->hcases.size() : b->pending.size());
+  o->total_nodes = b->total_nodes;
+  o->fluid_nodes = b->total_fluid;
+  o->lattice_updates = b->total_lu;
+  o->max_steps = b->max_steps;
+  o->t = b->t;
+  o->boundary_tiles = b->nbnd_tiles;
+  o->bulk_tiles = b->nbulk_tiles;
+  o->tile_w = 1 << b->tw_log2;
+  o->tile_h = b->th;
+  o->grid_bulk = b->grid_bulk;
+  o->grid_boundary = b->grid_bnd;
+  o->last_run_seconds = b->last_run_seconds;
+  o->device_bytes = b->device_bytes;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) o->autotune_ms[i][j] = b->autotune_ms[i][j];
+  return VL_OK;
+}
+
+vl_status vl_batch_case_result(vl_batch* b, int id, double* coef, int64_t* steps_done, int* status) {
+  if (!b || !b->built || id < 0 || (size_t)id >= b->hcases.size()) return set_error(VL_EINVAL, "bad case id");
+  VL_HIP(hipStreamSynchronize(b->stream));
+  const BatchCase& c = b->hcases[id];
+  int32_t stv = 0;
+  int64_t done = 0;
+  VL_HIP(hipMemcpy(&stv, b->d_status + id, sizeof(int32_t), hipMemcpyDeviceToHost));
+  VL_HIP(hipMemcpy(&done, b->d_done + id, sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (steps_done) *steps_done = done;
+  if (status) *status = stv;
+  if (coef && c.steps > 0)
+    VL_HIP(hipMemcpy(coef, b->d_hist + c.hist_off, (size_t)(2 * c.steps) * sizeof(double), hipMemcpyDeviceToHost));
+  return VL_OK;
+}
+
+// Populations of one case in the reference layout (FP64 f_i).
+vl_status vl_batch_read_case(vl_batch* b, int id, double* f) {
+  if (!b || !b->built || id < 0 || (size_t)id >= b->hcases.size() || !f) return set_error(VL_EINVAL, "bad case id");
+  VL_HIP(hipStreamSynchronize(b->stream));
+  const BatchCase& c = b->hcases[id];
+  std::vector<char> host((size_t)(Q * c.g.plane) * b->elem);
+  VL_HIP(hipMemcpy(host.data(), (char*)b->pool[b->cur] + (size_t)c.pop_off * b->elem, host.size(),
+                   hipMemcpyDeviceToHost));
+  const int64_t nx = c.g.nx, ny = c.g.ny, n = nx * ny;
+  for (int i = 0; i < Q; ++i)
+    for (int64_t y = 0; y < ny; ++y)
+      for (int64_t x = 0; x < nx; ++x) {
+        const int64_t e = i * c.g.plane + y * c.g.pitch + x;
+        double v;
+        if (b->precision == VL_F64)
+          v = ((const double*)host.data())[e];
+        else if (b->precision == VL_F32)
+          v = wgt(i) + (double)((const float*)host.data())[e];
+        else
+          v = wgt(i) + (double)((float)((const _Float16*)host.data())[e] * F16_INV);
+        f[i * n + y * nx + x] = v;
+      }
+  return VL_OK;
+}
+
+void vl_batch_destroy(vl_batch* b) { free_batch(b); }
+
+}  // extern "C"

@@ -23,6 +23,10 @@ struct vl_channel {
   int64_t force_groups = 0;
 };
 
+namespace vl {
+double channel_kick(const vl_kick& k, bool has_kick, uint64_t t);
+}
+
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -31,15 +35,7 @@ constexpr double kPi = 3.14159265358979323846;
 vl_ctl controls(vl_channel* c, uint64_t t) {
   vl_ctl k{};
   c->vscale[1] = vl::ramp(t, c->ramp_steps);
-  double kick = 0.0;
-  if (c->has_kick && t >= c->kick.start && t < c->kick.start + c->kick.duration) {
-    const double s = (double)(t - c->kick.start) / (double)c->kick.duration;
-    const double a = std::sin(kPi * s);
-    const double b = std::sin(2.0 * kPi * s);
-    const double sgn = std::isnan(b) ? b : std::copysign(1.0, b);  // f64::signum
-    kick = c->kick.omega * (a * a) * sgn;
-  }
-  c->vscale[2] = kick;
+  c->vscale[2] = vl::channel_kick(c->kick, c->has_kick, t);
   for (int g = 0; g < 4; ++g) k.vscale[g] = c->vscale[g];
   k.has_target = c->absorbing ? 1 : 0;
   k.target = c->u_ref * c->vscale[1];
@@ -50,15 +46,24 @@ vl_ctl controls(vl_channel* c, uint64_t t) {
 
 extern "C" {
 
-vl_status vl_channel_create(const vl_channel_spec* sp, vl_shape* body, int precision, vl_channel** out) {
-  if (!sp || !body || !out) {
+}  // extern "C"
+
+namespace vl {
+
+// The domain of Channel::new (src/cases/channel.rs:89-125): inlet wall cells at x = 0 moving
+// with the inflow profile (velocity group 1), pressure outlet at x = nx-1, side walls (at rest,
+// moving with the stream, or radiating), and the body (velocity group 2, force group 0).
+// `body` is consumed.  Returns the builder, or null with the error set.
+vl_builder* channel_builder(const vl_channel_spec* sp, vl_shape* body, vl_velocity* inlet_out) {
+  if (!sp || !body) {
     if (body) vl_shape_free(body);
-    return vl::set_error(VL_EINVAL, "null argument");
+    set_error(VL_EINVAL, "null argument");
+    return nullptr;
   }
-  *out = nullptr;
   if (sp->nx < 3 || sp->ny < 3) {
     vl_shape_free(body);
-    return vl::set_error(VL_EINVAL, "grid must be at least 3 x 3");
+    set_error(VL_EINVAL, "grid must be at least 3 x 3");
+    return nullptr;
   }
   const size_t nx = (size_t)sp->nx, ny = (size_t)sp->ny;
   const double h = (double)(ny - 2);
@@ -73,6 +78,7 @@ vl_status vl_channel_create(const vl_channel_spec* sp, vl_shape* body, int preci
     inlet_v.kind = VL_VEL_CONST;
     inlet_v.p[0] = sp->u;
   }
+  if (inlet_out) *inlet_out = inlet_v;
   const int inlet = vl_builder_add_wall(b, &inlet_v, 1, -1, nullptr);
   const int outlet = vl_builder_add_pressure(b, 1.0);
   int walls;
@@ -99,11 +105,39 @@ vl_status vl_channel_create(const vl_channel_spec* sp, vl_shape* body, int preci
   spin.p[0] = sp->kick.cx;
   spin.p[1] = sp->kick.cy;
   const int bodyk = vl_builder_add_wall(b, sp->has_kick ? &spin : nullptr, 2, 0, body);
-  vl_status st = vl_builder_fill_shape(b, bodyk);
-  if (st != VL_OK) {
+  if (vl_builder_fill_shape(b, bodyk) != VL_OK) {
     vl_builder_free(b);
-    return st;
+    return nullptr;
   }
+  return b;
+}
+
+// Controls of step t of a channel (src/cases/channel.rs:136-147): the inflow ramp, the body
+// kick and the outflow target.
+double channel_kick(const vl_kick& k, bool has_kick, uint64_t t) {
+  if (!has_kick || t < k.start || t >= k.start + k.duration) return 0.0;
+  const double s = (double)(t - k.start) / (double)k.duration;
+  const double a = std::sin(kPi * s);
+  const double b = std::sin(2.0 * kPi * s);
+  const double sgn = std::isnan(b) ? b : std::copysign(1.0, b);  // f64::signum
+  return k.omega * (a * a) * sgn;
+}
+
+}  // namespace vl
+
+extern "C" {
+
+vl_status vl_channel_create(const vl_channel_spec* sp, vl_shape* body, int precision, vl_channel** out) {
+  if (!out) {
+    if (body) vl_shape_free(body);
+    return vl::set_error(VL_EINVAL, "null argument");
+  }
+  *out = nullptr;
+  vl_velocity inlet_v{};
+  vl_builder* b = vl::channel_builder(sp, body, &inlet_v);
+  if (!b) return VL_EINVAL;
+  const size_t nx = (size_t)sp->nx, ny = (size_t)sp->ny;
+  vl_status st;
   vl_sim* sim = nullptr;
   st = vl_sim_create(b, precision, &sim);
   if (st != VL_OK) return st;

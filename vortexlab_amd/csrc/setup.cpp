@@ -276,6 +276,56 @@ vl_status build_tables(const Domain& d, Tables& t) {
   return VL_OK;
 }
 
+int32_t outlet_nodes_first(Tables& tb) {
+  const size_t nb = tb.bnode_x.size();
+  std::vector<char> needs(nb, 0);
+  for (size_t b = 0; b < nb; ++b)
+    for (int32_t l = tb.bnode_first[b]; l < tb.bnode_first[b + 1]; ++l)
+      if (tb.l_kind[l] == LINK_PRESSURE && !(tb.l_flags[l] & 1)) needs[b] = 1;
+  std::vector<int32_t> order;
+  order.reserve(nb);
+  for (size_t b = 0; b < nb; ++b)
+    if (needs[b]) order.push_back((int32_t)b);
+  const int32_t npb = (int32_t)order.size();
+  for (size_t b = 0; b < nb; ++b)
+    if (!needs[b]) order.push_back((int32_t)b);
+  Tables r = tb;
+  r.bnode_x.clear();
+  r.bnode_y.clear();
+  r.bnode_first.assign(1, 0);
+  auto clear_all = [](auto&... v) { (v.clear(), ...); };
+  clear_all(r.l_dir, r.l_kind, r.l_vgroup, r.l_flags, r.l_nx, r.l_ny, r.l_fslot, r.l_q, r.l_uwx, r.l_uwy, r.l_cu6,
+            r.l_rho);
+  for (int32_t b : order) {
+    r.bnode_x.push_back(tb.bnode_x[b]);
+    r.bnode_y.push_back(tb.bnode_y[b]);
+    for (int32_t l = tb.bnode_first[b]; l < tb.bnode_first[b + 1]; ++l) {
+      r.l_dir.push_back(tb.l_dir[l]);
+      r.l_kind.push_back(tb.l_kind[l]);
+      r.l_vgroup.push_back(tb.l_vgroup[l]);
+      r.l_flags.push_back(tb.l_flags[l]);
+      r.l_nx.push_back(tb.l_nx[l]);
+      r.l_ny.push_back(tb.l_ny[l]);
+      r.l_fslot.push_back(tb.l_fslot[l]);
+      r.l_q.push_back(tb.l_q[l]);
+      r.l_uwx.push_back(tb.l_uwx[l]);
+      r.l_uwy.push_back(tb.l_uwy[l]);
+      r.l_cu6.push_back(tb.l_cu6[l]);
+      r.l_rho.push_back(tb.l_rho[l]);
+    }
+    r.bnode_first.push_back((int32_t)r.l_dir.size());
+  }
+  tb = std::move(r);
+  return npb;
+}
+
+bool outlet_nodes_match(const Tables& tb, int32_t npb) {
+  if ((size_t)npb != tb.p_x.size()) return false;
+  for (int32_t k = 0; k < npb; ++k)
+    if (tb.bnode_x[k] != tb.p_x[k] || tb.bnode_y[k] != tb.p_y[k]) return false;
+  return true;
+}
+
 }  // namespace vl
 
 // ==========================================================================================

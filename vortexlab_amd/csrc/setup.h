@@ -84,6 +84,17 @@ void omegas(int collision, double magic, double tau, double& op, double& om);
 // Smooth inflow ramp (src/cases/mod.rs:11-18).
 double ramp(uint64_t t, uint64_t steps);
 
+// Boundary nodes whose links use the outlet controller first (in creation order, so boundary
+// node k < npb is outlet-controller node k); the rest after.  Link order inside a node and
+// force slots are unchanged.  Returns npb.
+int32_t outlet_nodes_first(Tables& t);
+bool outlet_nodes_match(const Tables& t, int32_t npb);
+
+// Channel::new's domain (src/cases/channel.rs:89-125); `body` is consumed.
+vl_builder* channel_builder(const vl_channel_spec* sp, vl_shape* body, vl_velocity* inlet_out);
+// The body kick of step t (src/cases/channel.rs:140-146).
+double channel_kick(const vl_kick& k, bool has_kick, uint64_t t);
+
 }  // namespace vl
 
 struct vl_shape {

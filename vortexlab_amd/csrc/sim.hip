@@ -25,336 +25,9 @@
 
 #include "lbm_math.h"
 #include "sim_internal.h"
+#include "step_device.h"
 
 namespace vl {
-
-// ------------------------------------------------------------------------------------------
-// Storage classes
-// ------------------------------------------------------------------------------------------
-struct StoreF64 {
-  using T = double;
-  using C = double;
-  static constexpr bool exact = true;
-  __device__ static inline C ld(const T* p) { return *p; }
-  __device__ static inline void st(T* p, C v) { *p = v; }
-  __device__ static inline void st_stream(T* p, C v) { __builtin_nontemporal_store(v, p); }
-  __device__ static inline C round(C v) { return v; }
-};
-struct StoreF32 {
-  using T = float;
-  using C = float;
-  static constexpr bool exact = false;
-  __device__ static inline C ld(const T* p) { return *p; }
-  __device__ static inline void st(T* p, C v) { *p = v; }
-  __device__ static inline void st_stream(T* p, C v) { __builtin_nontemporal_store(v, p); }
-  __device__ static inline C round(C v) { return v; }
-};
-// FP16 deviations scaled by 2^10: keeps |d| ~ 1e-6..1e-1 inside the normal half range.
-constexpr float F16_SCALE = 1024.0f;
-constexpr float F16_INV = 1.0f / 1024.0f;
-struct StoreF16 {
-  using T = _Float16;
-  using C = float;
-  static constexpr bool exact = false;
-  __device__ static inline C ld(const T* p) { return (float)*p * F16_INV; }
-  __device__ static inline void st(T* p, C v) { *p = (_Float16)(v * F16_SCALE); }
-  __device__ static inline void st_stream(T* p, C v) { __builtin_nontemporal_store((_Float16)(v * F16_SCALE), p); }
-  __device__ static inline C round(C v) { return (float)(_Float16)(v * F16_SCALE) * F16_INV; }  // = ld(st(v))
-};
-
-template <class S, int M>
-__device__ inline void collide(typename S::C* f, const Coll& c) {
-  if constexpr (S::exact)
-    Exact::collide<M>(f, c.op, c.om, c.gx, c.gy);
-  else
-    Dev::collide<M>(f, c.opf, c.omf, c.gxf, c.gyf);
-}
-
-__device__ inline int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
-
-// Pull the nine populations of node (x, y) from their upstream neighbours in A.
-template <class S>
-__device__ inline void pull(const typename S::T* __restrict__ A, const Geom& g, int x, int y,
-                            typename S::C* f) {
-  const int xm = wrapi(x - 1, g.nx), xp = wrapi(x + 1, g.nx);
-  const int ym = wrapi(y - 1, g.ny), yp = wrapi(y + 1, g.ny);
-  const int64_t r0 = (int64_t)y * g.pitch, rm = (int64_t)ym * g.pitch, rp = (int64_t)yp * g.pitch;
-  const int64_t P = g.plane;
-  f[0] = S::ld(A + r0 + x);
-  f[1] = S::ld(A + P + r0 + xm);
-  f[2] = S::ld(A + 2 * P + rm + x);
-  f[3] = S::ld(A + 3 * P + r0 + xp);
-  f[4] = S::ld(A + 4 * P + rp + x);
-  f[5] = S::ld(A + 5 * P + rm + xm);
-  f[6] = S::ld(A + 6 * P + rm + xp);
-  f[7] = S::ld(A + 7 * P + rp + xp);
-  f[8] = S::ld(A + 8 * P + rp + xm);
-}
-
-// ------------------------------------------------------------------------------------------
-// The fused step kernel.  One launch per time step; workgroups [0, nlb) handle the boundary
-// nodes (and the outlet controller and force reduction), the rest stream-collide the
-// interior.  Boundary workgroups have the lowest ids so they are dispatched first and finish
-// under the shadow of the bulk.
-// ------------------------------------------------------------------------------------------
-constexpr int STEP_T = 256;
-
-struct StepArgs {
-  int nlb;        // boundary workgroups
-  int xblocks;    // bulk tiles per row (fused kernel)
-  double sqrt_cs2;
-  double* fc_cur;        // force terms written this step (per force slot: x, y)
-  const double* fc_prev; // force terms of the previous step
-  // Per-step controls and force history live in device tables indexed by the step's place in
-  // the current run chunk, k = *kbase + k_local, so a captured window of steps can be replayed.
-  const StepCtl* ctls;   // controls of the chunk's steps
-  const int64_t* kbase;  // device counter: first chunk step of this window
-  int k_local;           // step within the window
-  double* hist;          // group forces of the chunk's steps (ngroups x 2 each), or null
-  const double* pv_cur;  // outward velocity of every pressure-boundary node, this step
-  double* pv_next;       // the same for the next step (written by the boundary update)
-};
-
-// Outlet controller (src/sim.rs:443-455): the mean outward normal velocity over the
-// pressure-boundary nodes, as a density offset.  FP64 storage: the reference's serial
-// left-to-right sum in node creation order (it feeds back into every later step, so the
-// order matters for bit equality).  Deviation storage: a fixed-order tree.  Every thread of
-// the workgroup must call this; all get the result.
-// Outward normal velocity n.u of one node from its (stored-precision) populations, with the
-// arithmetic of src/sim.rs:432-439, 447-449.
-template <class S>
-__device__ inline double normal_velocity_of(const typename S::C* f, const Coll& c, int nx_, int ny_) {
-  if constexpr (S::exact) {
-    double rho, jx, jy;
-    Exact::moments(f, rho, jx, jy);
-    const double ux = jx / rho - 0.5 * c.gx, uy = jy / rho - 0.5 * c.gy;
-    return (double)nx_ * ux + (double)ny_ * uy;
-  } else {
-    float drho, jx, jy;
-    Dev::moments(f, drho, jx, jy);
-    const double rho = 1.0 + (double)drho;
-    const double ux = (double)jx / rho - 0.5 * c.gx, uy = (double)jy / rho - 0.5 * c.gy;
-    return (double)nx_ * ux + (double)ny_ * uy;
-  }
-}
-
-// The same from the populations in A (used once at the start of a run).
-template <class S>
-__global__ __launch_bounds__(256) void k_outlet_prime(const typename S::T* __restrict__ A, Geom g, Coll c,
-                                                      DevTables t, double* __restrict__ pv) {
-  const int k = blockIdx.x * 256 + (int)threadIdx.x;
-  if (k >= t.np) return;
-  const int64_t idx = (int64_t)t.py[k] * g.pitch + t.px[k];
-  typename S::C f[Q];
-#pragma unroll
-  for (int i = 0; i < Q; ++i) f[i] = S::ld(A + i * g.plane + idx);
-  pv[k] = normal_velocity_of<S>(f, c, t.pnx[k], t.pny[k]);
-}
-
-// Density offset of the outlet controller from the nodes' outward velocities `pv` (which the
-// previous step's boundary update left behind): serial left-to-right sum for FP64 storage,
-// fixed-order tree otherwise.  Every thread of the workgroup must call this.
-template <class S>
-__device__ double outlet_offset(const double* __restrict__ pv, int np, double target, double sqrt_cs2,
-                                double* red) {
-  double sum = 0.0;  // serial running sum (exact) or this thread's strided partial (tree)
-  for (int base = 0; base < np; base += STEP_T) {
-    const int k = base + (int)threadIdx.x;
-    const double v = k < np ? pv[k] : 0.0;
-    if constexpr (S::exact) {
-      red[threadIdx.x] = v;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        const int m = min(STEP_T, np - base);
-        for (int i = 0; i < m; ++i) sum += red[i];
-      }
-      __syncthreads();
-    } else {
-      sum += v;
-    }
-  }
-  if constexpr (!S::exact) {
-    red[threadIdx.x] = sum;
-    __syncthreads();
-    for (int w = STEP_T / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    sum = red[0];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) red[0] = (sum / (double)np - target) / sqrt_cs2;
-  __syncthreads();
-  const double drho = red[0];
-  __syncthreads();
-  return drho;
-}
-
-// Force groups: fixed-order FP64 tree.  Thread k sums slots k, k+T, k+2T, ... in order, then
-// a fixed pairwise tree over the threads: the same association every step and every run.
-__device__ void reduce_forces(const double* __restrict__ fc, const int32_t* __restrict__ gfirst, int ngroups,
-                              double* __restrict__ out, double* red) {
-  for (int grp = 0; grp < ngroups; ++grp) {
-    const int s0 = gfirst[grp], s1 = gfirst[grp + 1];
-    for (int comp = 0; comp < 2; ++comp) {
-      double a = 0.0;
-      for (int s = s0 + (int)threadIdx.x; s < s1; s += STEP_T) a += fc[2 * s + comp];
-      red[threadIdx.x] = a;
-      __syncthreads();
-      for (int w = STEP_T / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) out[2 * grp + comp] = red[0];
-      __syncthreads();
-    }
-  }
-}
-
-// One boundary node (src/sim.rs:456-507): pull, replace the populations that would come out
-// of non-fluid cells by the link rules in link order, collide, write B; each force link
-// stores its momentum-exchange term.
-template <class S, int M>
-__device__ void boundary_node(const typename S::T* __restrict__ A, typename S::T* __restrict__ B, const Geom& g,
-                              const Coll& c, const DevTables& t, const StepCtl& ctl, double drho, double sqrt_cs2,
-                              int b, double* __restrict__ fc, double* __restrict__ pv_next) {
-  using C = typename S::C;
-  const int x = t.bx[b], y = t.by[b];
-  const int64_t idx = (int64_t)y * g.pitch + x;
-  C f[Q];
-  pull<S>(A, g, x, y, f);
-  // Local velocity for anti-bounce-back links, computed once (the reference recomputes the
-  // same value per link).
-  bool have_u = false;
-  double ux = 0.0, uy = 0.0;
-  const int l1 = t.bfirst[b + 1];
-  for (int l = t.bfirst[b]; l < l1; ++l) {
-    const int j = t.dir[l];
-    const int i = opp(j);
-    const C out = S::ld(A + j * g.plane + idx);
-    const double scale = ctl.vscale[t.vgroup[l]];
-    C back;
-    if (t.kind[l] == LINK_WALL) {
-      const double q = t.q[l];
-      if constexpr (S::exact) {
-        const double corr = t.cu6[l] * scale;
-        if (q >= 0.5) {
-          const double k = 0.5 / q;
-          back = k * (out - corr) + (1.0 - k) * A[i * g.plane + idx];
-        } else if (t.lflags[l] & 2) {
-          const int bxn = wrapi(x - cx(j), g.nx), byn = wrapi(y - cy(j), g.ny);
-          back = 2.0 * q * out + (1.0 - 2.0 * q) * A[j * g.plane + (int64_t)byn * g.pitch + bxn] - corr;
-        } else {
-          back = out - corr;
-        }
-      } else {
-        // The wall rule is affine with weights summing to one, so it holds unchanged for
-        // deviations (the w_i terms cancel).
-        const float corr = (float)(t.cu6[l] * scale);
-        const float qf = (float)q;
-        if (q >= 0.5) {
-          const float k = (float)(0.5 / q);
-          back = k * (out - corr) + (1.0f - k) * S::ld(A + i * g.plane + idx);
-        } else if (t.lflags[l] & 2) {
-          const int bxn = wrapi(x - cx(j), g.nx), byn = wrapi(y - cy(j), g.ny);
-          back = 2.0f * qf * out + (1.0f - 2.0f * qf) * S::ld(A + j * g.plane + (int64_t)byn * g.pitch + bxn) -
-                 corr;
-        } else {
-          back = out - corr;
-        }
-      }
-    } else {
-      if (!have_u) {
-        have_u = true;
-        if constexpr (S::exact) {
-          double fa[Q];
-#pragma unroll
-          for (int k = 0; k < Q; ++k) fa[k] = A[k * g.plane + idx];
-          double rho, jx, jy;
-          Exact::moments(fa, rho, jx, jy);
-          ux = jx / rho - 0.5 * c.gx;
-          uy = jy / rho - 0.5 * c.gy;
-        } else {
-          float da[Q];
-#pragma unroll
-          for (int k = 0; k < Q; ++k) da[k] = S::ld(A + k * g.plane + idx);
-          float dr, jx, jy;
-          Dev::moments(da, dr, jx, jy);
-          const double rho = 1.0 + (double)dr;
-          ux = (double)jx / rho - 0.5 * c.gx;
-          uy = (double)jy / rho - 0.5 * c.gy;
-        }
-      }
-      const double cu = 3.0 * ((double)cx(j) * ux + (double)cy(j) * uy);
-      const double off = (t.lflags[l] & 1) ? ((double)t.lnx[l] * ux + (double)t.lny[l] * uy) / sqrt_cs2 : drho;
-      if constexpr (S::exact) {
-        const double rho_w = t.rho[l] + off;
-        back = -out + 2.0 * wgt(j) * rho_w * (1.0 + 0.5 * cu * cu - 1.5 * (ux * ux + uy * uy));
-      } else {
-        // back - w = -d_out + 2 w (drho_w + rho_w (cu^2/2 - 1.5 u^2))
-        const double drw = (t.rho[l] - 1.0) + off;
-        const double rho_w = 1.0 + drw;
-        back = (C)(-(double)out + 2.0 * wgt(j) * (drw + rho_w * (0.5 * cu * cu - 1.5 * (ux * ux + uy * uy))));
-      }
-    }
-    f[i] = back;
-    const int slot = t.fslot[l];
-    if (slot >= 0) {
-      const double wx = t.uwx[l] * scale, wy = t.uwy[l] * scale;
-      double o = (double)out, bk = (double)back;
-      if constexpr (!S::exact) {
-        o += wgt(j);
-        bk += wgt(j);
-      }
-      fc[2 * slot] = (double)cx(j) * (o + bk) - wx * (o - bk);
-      fc[2 * slot + 1] = (double)cy(j) * (o + bk) - wy * (o - bk);
-    }
-  }
-  collide<S, M>(f, c);
-#pragma unroll
-  for (int i = 0; i < Q; ++i) S::st(B + i * g.plane + idx, f[i]);
-  if (b < t.npb) {  // pressure-boundary node b is outlet-controller node b: its velocity next step
-    C r[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) r[i] = S::round(f[i]);
-    pv_next[b] = normal_velocity_of<S>(r, c, t.pnx[b], t.pny[b]);
-  }
-}
-
-// Workgroup `blk` of the boundary part of a step.
-template <class S, int M>
-__device__ inline void boundary_block(int blk, const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
-                                      const Geom& g, const Coll& c, const DevTables& t, const StepArgs& sa,
-                                      double* red) {
-  const int64_t k = *sa.kbase + sa.k_local;
-  const StepCtl ctl = sa.ctls[k];
-  // the force terms of the previous step of this chunk go to its history slot
-  if (blk == 0 && sa.hist && k > 0) reduce_forces(sa.fc_prev, t.gfirst, t.ngroups, sa.hist + (k - 1) * 2 * t.ngroups, red);
-  double drho = 0.0;
-  if (ctl.has_target && t.np > 0 && blk * STEP_T < t.npb) {
-    drho = outlet_offset<S>(sa.pv_cur, t.np, ctl.target, sa.sqrt_cs2, red);
-    if (blk == 0 && threadIdx.x == 0) *t.drho = drho;
-  }
-  const int b = blk * STEP_T + (int)threadIdx.x;
-  if (b < t.nb) boundary_node<S, M>(A, B, g, c, t, ctl, drho, sa.sqrt_cs2, b, sa.fc_cur, sa.pv_next);
-}
-
-// One interior node of bulk tile `tile` (tiles are `width` consecutive nodes of one row).
-template <class S, int M>
-__device__ inline void bulk_node(int tile, int width, int xblocks, const typename S::T* __restrict__ A,
-                                 typename S::T* __restrict__ B, const uint8_t* __restrict__ flag, const Geom& g,
-                                 const Coll& c) {
-  const int y = tile / xblocks;
-  const int x = (tile - y * xblocks) * width + (int)threadIdx.x;
-  if (x >= g.nx) return;
-  const int64_t idx = (int64_t)y * g.pitch + x;
-  if (flag[idx] != 0) return;
-  typename S::C f[Q];
-  pull<S>(A, g, x, y, f);
-  collide<S, M>(f, c);
-#pragma unroll
-  for (int i = 0; i < Q; ++i) S::st_stream(B + i * g.plane + idx, f[i]);
-}
 
 // Small domains: the whole step in one launch (boundary workgroups first).
 template <class S, int M>
@@ -651,51 +324,9 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   for (int64_t y = 0; y < d.ny; ++y)
     std::memcpy(&pflag[y * s->g.pitch], &s->tab.flag[y * d.nx], (size_t)d.nx);
   VL_TRY(upload(s, s->dflag, pflag));
-  // Boundary nodes whose links need the outlet controller go first, so only the leading
-  // boundary workgroups compute it.  Link order inside each node, and the force-slot order,
-  // are unchanged, so results do not depend on this permutation.
+  // Outlet-controller nodes first: only the leading boundary workgroups need the offset.
   Tables tb = s->tab;
-  {
-    const size_t nb = tb.bnode_x.size();
-    std::vector<int32_t> order;
-    order.reserve(nb);
-    std::vector<char> needs(nb, 0);
-    for (size_t b = 0; b < nb; ++b)
-      for (int32_t l = tb.bnode_first[b]; l < tb.bnode_first[b + 1]; ++l)
-        if (tb.l_kind[l] == LINK_PRESSURE && !(tb.l_flags[l] & 1)) needs[b] = 1;
-    for (size_t b = 0; b < nb; ++b)
-      if (needs[b]) order.push_back((int32_t)b);
-    s->dt.npb = (int32_t)order.size();
-    for (size_t b = 0; b < nb; ++b)
-      if (!needs[b]) order.push_back((int32_t)b);
-    Tables r = tb;
-    r.bnode_x.clear();
-    r.bnode_y.clear();
-    r.bnode_first.assign(1, 0);
-    auto clear_links = [&](auto&... v) { (v.clear(), ...); };
-    clear_links(r.l_dir, r.l_kind, r.l_vgroup, r.l_flags, r.l_nx, r.l_ny, r.l_fslot, r.l_q, r.l_uwx, r.l_uwy,
-                r.l_cu6, r.l_rho);
-    for (int32_t b : order) {
-      r.bnode_x.push_back(tb.bnode_x[b]);
-      r.bnode_y.push_back(tb.bnode_y[b]);
-      for (int32_t l = tb.bnode_first[b]; l < tb.bnode_first[b + 1]; ++l) {
-        r.l_dir.push_back(tb.l_dir[l]);
-        r.l_kind.push_back(tb.l_kind[l]);
-        r.l_vgroup.push_back(tb.l_vgroup[l]);
-        r.l_flags.push_back(tb.l_flags[l]);
-        r.l_nx.push_back(tb.l_nx[l]);
-        r.l_ny.push_back(tb.l_ny[l]);
-        r.l_fslot.push_back(tb.l_fslot[l]);
-        r.l_q.push_back(tb.l_q[l]);
-        r.l_uwx.push_back(tb.l_uwx[l]);
-        r.l_uwy.push_back(tb.l_uwy[l]);
-        r.l_cu6.push_back(tb.l_cu6[l]);
-        r.l_rho.push_back(tb.l_rho[l]);
-      }
-      r.bnode_first.push_back((int32_t)r.l_dir.size());
-    }
-    tb = std::move(r);
-  }
+  s->dt.npb = outlet_nodes_first(tb);
   DevTables& t = s->dt;
   t.nb = (int32_t)tb.bnode_x.size();
   t.nl = (int32_t)tb.l_dir.size();
@@ -738,9 +369,8 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
     VL_HIP(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
     VL_HIP(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
   }
-  bool same = t.np == t.npb;
-  for (int32_t k = 0; same && k < t.np; ++k) same = tb.bnode_x[k] == tb.p_x[k] && tb.bnode_y[k] == tb.p_y[k];
-  if (!same) return set_error(VL_EINVAL, "internal: outlet-controller nodes do not match the pressure-boundary nodes");
+  if (!outlet_nodes_match(tb, t.npb))
+    return set_error(VL_EINVAL, "internal: outlet-controller nodes do not match the pressure-boundary nodes");
   VL_HIP(hipMalloc((void**)&t.pvel, (size_t)std::max(t.np, 1) * 2 * sizeof(double)));
   VL_HIP(hipMemset(t.pvel, 0, (size_t)std::max(t.np, 1) * 2 * sizeof(double)));
   VL_HIP(hipMalloc((void**)&t.drho, sizeof(double)));
