@@ -852,8 +852,11 @@ vl_status vl_batch_read_case(vl_batch* b, int id, double* f) {
   if (!b || !b->built || id < 0 || (size_t)id >= b->hcases.size() || !f) return set_error(VL_EINVAL, "bad case id");
   VL_HIP(hipStreamSynchronize(b->stream));
   const BatchCase& c = b->hcases[id];
+  // A case stops updating after its own step count while the batch goes on, so its latest
+  // populations sit in the pool of its own update parity (every case starts in pool 0).
+  const int64_t updates = std::min<int64_t>(b->t, c.steps);
   std::vector<char> host((size_t)(Q * c.g.plane) * b->elem);
-  VL_HIP(hipMemcpy(host.data(), (char*)b->pool[b->cur] + (size_t)c.pop_off * b->elem, host.size(),
+  VL_HIP(hipMemcpy(host.data(), (char*)b->pool[updates & 1] + (size_t)c.pop_off * b->elem, host.size(),
                    hipMemcpyDeviceToHost));
   const int64_t nx = c.g.nx, ny = c.g.ny, n = nx * ny;
   for (int i = 0; i < Q; ++i)
