@@ -252,6 +252,32 @@ vl_status vl_batch_read_case(vl_batch* b, int id, double* f);
 void vl_batch_destroy(vl_batch* b);
 
 /* ------------------------------------------------------------------------------------
+ * Surrogate network (no reference counterpart): a 64x64 body mask plus the scaled log Re ->
+ * (mean Cd, Cl', St), trained and evaluated by hand-written HIP kernels in FP32.  Layout:
+ * 4 x [conv 3x3 + ReLU + 2x2 max-pool] (1-32-32-64-64 channels), FC 1025-128 + ReLU, FC 128-3;
+ * parameters are one flat FP32 vector (per conv: weights [co][ci][3][3] then biases; then FC1
+ * weights [128][1025], biases; FC2 weights [3][128], biases).  Reductions run in a fixed order,
+ * so training is bitwise reproducible from the seed.
+ * ---------------------------------------------------------------------------------- */
+typedef struct vl_nn vl_nn;
+vl_status vl_nn_create(int max_batch, uint64_t seed, vl_nn** out);
+void vl_nn_destroy(vl_nn* q);
+int64_t vl_nn_param_count(void);
+vl_status vl_nn_get_params(vl_nn* q, float* out);
+vl_status vl_nn_set_params(vl_nn* q, const float* in);
+/* masks: n x 64 x 64 (row-major), feat: n scaled log Re, out: n x 3. */
+vl_status vl_nn_forward(vl_nn* q, const float* masks, const float* feat, int n, float* out);
+/* Mean-squared-error loss of one batch and the gradient of every parameter (no update). */
+vl_status vl_nn_gradients(vl_nn* q, const float* masks, const float* feat, const float* targets, int n,
+                          float* loss, float* grad);
+/* Training data kept on the device; epochs gather their batches there. */
+vl_status vl_nn_set_dataset(vl_nn* q, const float* masks, const float* feat, const float* targets,
+                            int64_t count);
+vl_status vl_nn_train_epoch(vl_nn* q, const int32_t* order, int64_t count, int batch, float lr,
+                            float* mean_loss);
+vl_status vl_nn_predict_dataset(vl_nn* q, const int32_t* order, int64_t count, float* out);
+
+/* ------------------------------------------------------------------------------------
  * Force-history statistics: src/cases/channel.rs:157-207 and src/signal.rs (host, FP64).
  * ---------------------------------------------------------------------------------- */
 typedef struct {

@@ -1,0 +1,75 @@
+"""Surrogate network on the GPU: the HIP forward and backward kernels against the float64
+oracle (oracle/nn_ref.py), bitwise-reproducible training, and an end-to-end run that shows
+what the trained ensemble gains over the method without it (the solver alone)."""
+import numpy as np
+import pytest
+
+from oracle import nn_ref
+import vortexlab_amd as vl
+from vortexlab_amd import surrogate
+
+pytestmark = pytest.mark.gpu
+
+
+def test_forward_and_gradients_match_oracle(gpu):
+    rng = np.random.default_rng(1)
+    net = surrogate.Net(gpu, seed=3, max_batch=8)
+    p = net.params().astype(np.float64)
+    masks = (rng.uniform(size=(4, 64, 64)) > 0.5).astype(np.float32)
+    s = rng.normal(size=4).astype(np.float32)
+    t = rng.normal(size=(4, 3)).astype(np.float32)
+    y_ref, _ = nn_ref.forward(p, masks.astype(np.float64), s.astype(np.float64))
+    y = net.forward(masks, s)
+    assert np.max(np.abs(y - y_ref)) <= 1e-4 * max(1.0, np.max(np.abs(y_ref)))
+    loss_ref, g_ref = nn_ref.loss_and_grad(p, masks.astype(np.float64), s.astype(np.float64), t.astype(np.float64))
+    loss, g = net.gradients(masks, s, t)
+    assert abs(loss - loss_ref) <= 1e-4 * abs(loss_ref)
+    off, _ = nn_ref.layout()
+    for name, (o, shape) in off.items():
+        n = int(np.prod(shape))
+        ref, got = g_ref[o:o + n], g[o:o + n]
+        scale = max(np.max(np.abs(ref)), 1e-12)
+        assert np.max(np.abs(got - ref)) <= 2e-4 * scale, f"{name}: max err {np.max(np.abs(got - ref)):.3e} / {scale:.3e}"
+    net.close()
+
+
+def test_training_is_reproducible(gpu):
+    rng = np.random.default_rng(2)
+    masks = (rng.uniform(size=(96, 64, 64)) > 0.5).astype(np.float32)
+    feat = rng.normal(size=96).astype(np.float32)
+    y = rng.normal(size=(96, 3)).astype(np.float32)
+    params = []
+    for _ in range(2):
+        net = surrogate.Net(gpu, seed=11, max_batch=32)
+        net.set_dataset(masks, feat, y)
+        for ep in range(3):
+            net.train_epoch(np.random.default_rng(ep).permutation(96), 32, 1e-3)
+        params.append(net.params())
+        net.close()
+    assert np.array_equal(params[0], params[1])
+
+
+def test_surrogate_gains_over_solver_alone(gpu):
+    """Small end-to-end run: solver-labelled training set from the batched path, a 4-member
+    ensemble, held-out shapes.  The ensemble must predict the drag better than the training
+    mean does, answer part of the queries itself, and cost far less per answer than a solve."""
+    kw = dict(d=8.0, units=20.0)
+    tr = surrogate.make_dataset(gpu, 21, 384, **kw)
+    te = surrogate.make_dataset(gpu, 22, 96, **kw)
+    assert tr.ok.mean() > 0.8 and te.ok.mean() > 0.8
+    ens = surrogate.Ensemble(gpu, 4, seed=3, max_batch=64)
+    ens.fit(tr.masks[tr.ok], tr.re[tr.ok], tr.y[tr.ok], epochs=30, batch=32, lr=2e-3)
+    import time
+    t0 = time.perf_counter()
+    mean, std = ens.predict(te.masks[te.ok], te.re[te.ok])
+    t_pred = time.perf_counter() - t0
+    truth = te.y[te.ok]
+    err_net = np.sqrt(np.mean((mean[:, 0] - truth[:, 0]) ** 2))
+    err_mean = np.sqrt(np.mean((ens.y_mu[0] - truth[:, 0]) ** 2))
+    print(f"[surrogate] Cd rmse {err_net:.4f} vs mean predictor {err_mean:.4f}; "
+          f"accepted {surrogate.accept(mean, std, ens.y_sd).mean():.2f}", flush=True)
+    assert err_net < 0.6 * err_mean
+    acc = surrogate.accept(mean, std, ens.y_sd, 0.05)
+    assert acc.any()
+    assert t_pred / len(truth) < 0.1 * te.solve_seconds / len(te.ok)
+    ens.close()

@@ -1,0 +1,324 @@
+"""Learned surrogate for the towing-tank sweep: body mask + Re -> (mean Cd, Cl', St).
+
+The labels come from the batched solver (vl_batch); the network is trained and evaluated by
+the HIP kernels behind vl_nn_* (vortexlab_amd/csrc/nn.hip).  A 4-member ensemble answers a
+query when its members agree (relative spread <= 2 % on every output), otherwise the query
+falls back to the solver, so the ensemble is also the judge of its own error.
+
+Usage (one GPU):  python -m vortexlab_amd.surrogate --train 8192 --test 1024 --out report.json
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import api, workloads
+
+IMG = 64
+N_OUT = 3
+OUT_NAMES = ("cd_mean", "cl_rms", "st")
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int32)
+
+
+def _bind(lib):
+    sig = {
+        "vl_nn_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
+        "vl_nn_destroy": (None, [C.c_void_p]),
+        "vl_nn_param_count": (C.c_int64, []),
+        "vl_nn_get_params": (C.c_int, [C.c_void_p, _F]),
+        "vl_nn_set_params": (C.c_int, [C.c_void_p, _F]),
+        "vl_nn_forward": (C.c_int, [C.c_void_p, _F, _F, C.c_int, _F]),
+        "vl_nn_gradients": (C.c_int, [C.c_void_p, _F, _F, _F, C.c_int, _F, _F]),
+        "vl_nn_set_dataset": (C.c_int, [C.c_void_p, _F, _F, _F, C.c_int64]),
+        "vl_nn_train_epoch": (C.c_int, [C.c_void_p, _I, C.c_int64, C.c_int, C.c_float, _F]),
+        "vl_nn_predict_dataset": (C.c_int, [C.c_void_p, _I, C.c_int64, _F]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+
+
+def _fp(a):
+    return a.ctypes.data_as(_F)
+
+
+class Net:
+    """One network instance on the GPU (parameters, Adam state, activations, dataset)."""
+
+    def __init__(self, be: api.Backend, seed: int, max_batch: int = 128):
+        self.be = be
+        _bind(be.lib)
+        h = C.c_void_p()
+        be.check(be.lib.vl_nn_create(max_batch, seed, C.byref(h)))
+        self.h = h.value
+        self.max_batch = max_batch
+        self.n_params = int(be.lib.vl_nn_param_count())
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, dtype=np.float32)
+        self.be.check(self.be.lib.vl_nn_get_params(self.h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self.be.check(self.be.lib.vl_nn_set_params(self.h, _fp(p)))
+
+    def forward(self, masks, feat) -> np.ndarray:
+        masks = np.ascontiguousarray(masks, dtype=np.float32).reshape(-1, IMG * IMG)
+        feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1)
+        out = np.zeros((len(feat), N_OUT), dtype=np.float32)
+        for b0 in range(0, len(feat), self.max_batch):
+            n = min(self.max_batch, len(feat) - b0)
+            o = np.zeros((n, N_OUT), dtype=np.float32)
+            self.be.check(self.be.lib.vl_nn_forward(self.h, _fp(masks[b0:b0 + n]), _fp(feat[b0:b0 + n]), n, _fp(o)))
+            out[b0:b0 + n] = o
+        return out
+
+    def gradients(self, masks, feat, targets):
+        masks = np.ascontiguousarray(masks, dtype=np.float32).reshape(-1, IMG * IMG)
+        feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1)
+        targets = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, N_OUT)
+        loss = C.c_float()
+        g = np.zeros(self.n_params, dtype=np.float32)
+        self.be.check(self.be.lib.vl_nn_gradients(self.h, _fp(masks), _fp(feat), _fp(targets), len(feat),
+                                                  C.byref(loss), _fp(g)))
+        return float(loss.value), g
+
+    def set_dataset(self, masks, feat, targets):
+        self._ds = [np.ascontiguousarray(masks, dtype=np.float32).reshape(-1, IMG * IMG),
+                    np.ascontiguousarray(feat, dtype=np.float32).reshape(-1),
+                    np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, N_OUT)]
+        self.be.check(self.be.lib.vl_nn_set_dataset(self.h, *[_fp(a) for a in self._ds], len(self._ds[1])))
+
+    def train_epoch(self, order, batch: int, lr: float) -> float:
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        loss = C.c_float()
+        self.be.check(self.be.lib.vl_nn_train_epoch(self.h, order.ctypes.data_as(_I), len(order), batch, lr,
+                                                    C.byref(loss)))
+        return float(loss.value)
+
+    def predict_dataset(self, idx) -> np.ndarray:
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        out = np.zeros((len(idx), N_OUT), dtype=np.float32)
+        self.be.check(self.be.lib.vl_nn_predict_dataset(self.h, idx.ctypes.data_as(_I), len(idx), _fp(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            self.be.lib.vl_nn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------------------
+# masks
+# ---------------------------------------------------------------------------------------
+def rasterize(outline: np.ndarray, cx: float, cy: float, d: float, half: float = 0.6) -> np.ndarray:
+    """64 x 64 mask of a polygon over the square of side 2 * half * d centred on (cx, cy) (the
+    star bodies reach at most d/2 from the centre): a pixel is 1 when its centre is inside by
+    the even-odd rule (the solver's polygon predicate)."""
+    side = 2.0 * half * d
+    c = (np.arange(IMG) + 0.5) / IMG * side - 0.5 * side
+    px, py = np.meshgrid(cx + c, cy + c)
+    inside = np.zeros(px.shape, dtype=bool)
+    x, y = outline[:, 0], outline[:, 1]
+    j = len(x) - 1
+    for i in range(len(x)):
+        xi, yi, xj, yj = x[i], y[i], x[j], y[j]
+        cross = (yi > py) != (yj > py)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xint = (xj - xi) * (py - yi) / (yj - yi) + xi
+        inside ^= cross & (px < xint)
+        j = i
+    return inside.astype(np.float32)
+
+
+# ---------------------------------------------------------------------------------------
+# dataset from the batched solver
+# ---------------------------------------------------------------------------------------
+@dataclass
+class Dataset:
+    masks: np.ndarray   # [n, 64, 64]
+    re: np.ndarray      # [n]
+    y: np.ndarray       # [n, 3] (cd_mean, cl_rms, st)
+    ok: np.ndarray      # [n] solver finished (did not diverge)
+    solve_seconds: float
+    lattice_updates: int
+
+
+def sample_cases(seed: int, n: int, d: float = 16.0, re=(40.0, 250.0), units: float = 30.0, u: float = 0.05,
+                 upstream: float = 5.0, downstream: float = 15.0, height: float = 8.0):
+    """n towing-tank cases with star-polygon bodies (workloads.star_polygon), Re uniform in
+    `re`, TRT, in a fixed channel of (upstream + downstream) x height diameters."""
+    rng = np.random.default_rng(seed)
+    nx = int(round((upstream + downstream) * d)) + 2
+    ny = int(round(height * d)) + 2
+    cases, masks, res = [], [], []
+    for _ in range(n):
+        r = float(rng.uniform(*re))
+        sub = np.random.default_rng(rng.integers(1 << 62))
+        holder = {}
+
+        def outline_fn(cx, cy, sub=sub, holder=holder):
+            holder["o"] = workloads.star_polygon(sub, d, cx, cy)
+            holder["c"] = (cx, cy)
+            return holder["o"]
+
+        c = workloads.tank_case(nx, ny, d, r, u, api.TRT, outline_fn, units=units, ramp_units=3.0, kick_units=1.0)
+        cx, cy = holder["c"]
+        cases.append(c)
+        masks.append(rasterize(holder["o"], cx, cy, d))
+        res.append(r)
+    return cases, np.stack(masks), np.asarray(res)
+
+
+def solve(be: api.Backend, cases, discard_units: float = 12.0, precision: int = api.F32):
+    """Runs the cases in one batch and returns their (cd_mean, cl_rms, st) and status."""
+    b = api.Batch(be, precision)
+    for c in cases:
+        b.add(c.spec, c.steps, c.d)
+    b.build()
+    t0 = time.perf_counter()
+    b.run()
+    secs = time.perf_counter() - t0
+    lu = int(b.info().lattice_updates)
+    y = np.zeros((len(cases), N_OUT))
+    ok = np.zeros(len(cases), dtype=bool)
+    for k, c in enumerate(cases):
+        coef, done, st = b.result(k)
+        if st != 0 or not np.all(np.isfinite(coef)):
+            continue
+        s = api.series_stats(be, coef[:, 0], coef[:, 1], c.spec.u / c.d, discard_units)
+        y[k] = (s.cd_mean, s.cl_rms, s.st)
+        ok[k] = True
+    b.close()
+    return y, ok, secs, lu
+
+
+def make_dataset(be, seed, n, **kw) -> Dataset:
+    cases, masks, re = sample_cases(seed, n, **kw)
+    y, ok, secs, lu = solve(be, cases)
+    return Dataset(masks, re, y, ok, secs, lu)
+
+
+# ---------------------------------------------------------------------------------------
+# ensemble
+# ---------------------------------------------------------------------------------------
+class Ensemble:
+    """Members trained on the same data from different seeds and sample orders."""
+
+    def __init__(self, be, members: int = 4, seed: int = 1, max_batch: int = 128):
+        self.nets = [Net(be, seed + 7919 * k, max_batch) for k in range(members)]
+        self.seed = seed
+
+    def fit(self, masks, re, y, epochs: int = 40, batch: int = 64, lr: float = 1e-3, log=None):
+        self.feat_mu, self.feat_sd = float(np.mean(np.log(re))), float(np.std(np.log(re)) + 1e-12)
+        self.y_mu, self.y_sd = y.mean(0), y.std(0) + 1e-9
+        feat = (np.log(re) - self.feat_mu) / self.feat_sd
+        yt = (y - self.y_mu) / self.y_sd
+        losses = []
+        for k, net in enumerate(self.nets):
+            net.set_dataset(masks, feat, yt)
+            rng = np.random.default_rng(self.seed * 1000 + k)
+            for ep in range(epochs):
+                lr_ep = lr * 0.5 * (1.0 + math.cos(math.pi * ep / epochs))  # cosine decay
+                loss = net.train_epoch(rng.permutation(len(feat)), batch, lr_ep)
+            losses.append(loss)
+            if log:
+                log(f"[surrogate] member {k}: final epoch loss {loss:.4f}")
+        return losses
+
+    def predict(self, masks, re):
+        feat = (np.log(re) - self.feat_mu) / self.feat_sd
+        outs = np.stack([net.forward(masks, feat) for net in self.nets]) * self.y_sd + self.y_mu
+        return outs.mean(0), outs.std(0)
+
+    def close(self):
+        for n in self.nets:
+            n.close()
+
+
+def accept(mean, std, y_scale, threshold: float = 0.02):
+    """A prediction is accepted when, for every output, the ensemble spread is <= threshold
+    of the predicted value (or of the output's dataset scale when the value is near zero)."""
+    ref = np.maximum(np.abs(mean), 0.1 * y_scale)
+    return np.all(std <= threshold * ref, axis=1)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--train", type=int, default=8192)
+    ap.add_argument("--test", type=int, default=1024)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--threshold", type=float, default=0.02)
+    ap.add_argument("--out", default="")
+    args = ap.parse_args()
+    import sys
+
+    import vortexlab_amd as vl
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    be = vl.backend()
+    t0 = time.perf_counter()
+    tr = make_dataset(be, args.seed, args.train)
+    log(f"[surrogate] training set: {tr.ok.sum()} of {args.train} cases solved in {tr.solve_seconds:.1f} s "
+        f"({tr.lattice_updates / tr.solve_seconds / 1e9:.1f} GLUPS)")
+    te = make_dataset(be, args.seed + 1, args.test)
+    log(f"[surrogate] held-out set: {te.ok.sum()} of {args.test} cases solved in {te.solve_seconds:.1f} s")
+    ens = Ensemble(be, 4, seed=args.seed)
+    t1 = time.perf_counter()
+    ens.fit(tr.masks[tr.ok], tr.re[tr.ok], tr.y[tr.ok], epochs=args.epochs, log=log)
+    t_train = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    mean, std = ens.predict(te.masks[te.ok], te.re[te.ok])
+    t_pred = time.perf_counter() - t2
+    truth = te.y[te.ok]
+    acc = accept(mean, std, ens.y_sd, args.threshold)
+    err = np.abs(mean - truth)
+    rel = err / np.maximum(np.abs(truth), 0.1 * ens.y_sd)
+    baseline = np.abs(ens.y_mu - truth) / np.maximum(np.abs(truth), 0.1 * ens.y_sd)
+    solve_per_case = te.solve_seconds / args.test
+    pred_per_case = t_pred / max(1, len(truth))
+    frac = float(acc.mean())
+    per_query = frac * pred_per_case + (1.0 - frac) * (solve_per_case + pred_per_case)
+    report = {
+        "train_cases": int(tr.ok.sum()), "test_cases": int(te.ok.sum()), "epochs": args.epochs,
+        "train_seconds": round(t_train, 2), "dataset_solve_seconds": round(tr.solve_seconds + te.solve_seconds, 2),
+        "accepted_fraction": round(frac, 4),
+        "median_rel_error_accepted": {k: float(np.median(rel[acc, i])) if acc.any() else None
+                                      for i, k in enumerate(OUT_NAMES)},
+        "p90_rel_error_accepted": {k: float(np.quantile(rel[acc, i], 0.9)) if acc.any() else None
+                                   for i, k in enumerate(OUT_NAMES)},
+        "median_rel_error_all": {k: float(np.median(rel[:, i])) for i, k in enumerate(OUT_NAMES)},
+        "median_rel_error_mean_predictor": {k: float(np.median(baseline[:, i])) for i, k in enumerate(OUT_NAMES)},
+        "solver_seconds_per_case_batched": solve_per_case,
+        "surrogate_seconds_per_case": pred_per_case,
+        "queries_per_hour_solver_only": 3600.0 / solve_per_case,
+        "queries_per_hour_with_surrogate": 3600.0 / per_query,
+        "solves_avoided_per_hour": frac * 3600.0 / per_query,
+        "total_seconds": round(time.perf_counter() - t0, 1),
+    }
+    ens.close()
+    print(json.dumps(report), flush=True)
+    if args.out:
+        with open(args.out, "w") as fh:
+            json.dump(report, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()
