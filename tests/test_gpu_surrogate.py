@@ -53,7 +53,7 @@ def test_surrogate_gains_over_solver_alone(gpu):
     """Small end-to-end run: solver-labelled training set from the batched path, a 4-member
     ensemble, held-out shapes.  The ensemble must predict the drag better than the training
     mean does, answer part of the queries itself, and cost far less per answer than a solve."""
-    kw = dict(d=8.0, units=20.0)
+    kw = dict(d=10.0, units=24.0, re=(40.0, 120.0))  # tau >= 0.525
     tr = surrogate.make_dataset(gpu, 21, 384, **kw)
     te = surrogate.make_dataset(gpu, 22, 96, **kw)
     assert tr.ok.mean() > 0.8 and te.ok.mean() > 0.8

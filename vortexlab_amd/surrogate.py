@@ -158,10 +158,11 @@ class Dataset:
     lattice_updates: int
 
 
-def sample_cases(seed: int, n: int, d: float = 16.0, re=(40.0, 250.0), units: float = 30.0, u: float = 0.05,
+def sample_cases(seed: int, n: int, d: float = 16.0, re=(40.0, 250.0), units: float = 30.0, u: float = 0.1,
                  upstream: float = 5.0, downstream: float = 15.0, height: float = 8.0):
     """n towing-tank cases with star-polygon bodies (workloads.star_polygon), Re uniform in
-    `re`, TRT, in a fixed channel of (upstream + downstream) x height diameters."""
+    `re`, TRT, in a fixed channel of (upstream + downstream) x height diameters.  At 16 cells
+    per diameter the stream speed is 0.1 (Mach 0.17), so tau >= 0.519 at Re 250."""
     rng = np.random.default_rng(seed)
     nx = int(round((upstream + downstream) * d)) + 2
     ny = int(round(height * d)) + 2
