@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -649,6 +650,7 @@ vl_status build(vl_batch* b) {
   VL_HIP(hipGetDeviceProperties(&prop, 0));
   const int cus = std::max(1, prop.multiProcessorCount);
   b->grid_bnd = std::max(1, std::min(b->nbnd_tiles, cus * 2));
+  if (const char* e = std::getenv("VL_BATCH_GRID_BND")) b->grid_bnd = std::max(1, std::min(b->nbnd_tiles, std::atoi(e)));
   const int shapes[3][2] = {{6, 4}, {7, 2}, {8, 1}};  // (log2 TW, TH)
   const int mults[3] = {4, 8, 16};
   double best = 1e30;
