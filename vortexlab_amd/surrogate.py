@@ -253,11 +253,13 @@ class Ensemble:
             n.close()
 
 
-def accept(mean, std, y_scale, threshold: float = 0.02):
-    """A prediction is accepted when, for every output, the ensemble spread is <= threshold
-    of the predicted value (or of the output's dataset scale when the value is near zero)."""
+def accept(mean, std, y_scale, threshold: float = 0.02, outputs=(0,)):
+    """A prediction is accepted when the ensemble's relative spread (standard deviation over
+    the members / |mean|, or / the output's dataset scale when the mean is near zero) is at
+    most `threshold` on each of `outputs` (default: the drag, the primary quantity)."""
     ref = np.maximum(np.abs(mean), 0.1 * y_scale)
-    return np.all(std <= threshold * ref, axis=1)
+    ok = std <= threshold * ref
+    return np.all(ok[:, list(outputs)], axis=1)
 
 
 def main():
@@ -294,6 +296,14 @@ def main():
     acc = accept(mean, std, ens.y_sd, args.threshold)
     err = np.abs(mean - truth)
     rel = err / np.maximum(np.abs(truth), 0.1 * ens.y_sd)
+    sweep = {}
+    for rule, outs in (("cd", (0,)), ("all", (0, 1, 2))):
+        for th in (0.01, 0.02, 0.05, 0.1):
+            a = accept(mean, std, ens.y_sd, th, outs)
+            sweep[f"{rule}@{th}"] = {"accepted": round(float(a.mean()), 4),
+                                     "median_rel_err_cd": float(np.median(rel[a, 0])) if a.any() else None,
+                                     "p90_rel_err_cd": float(np.quantile(rel[a, 0], 0.9)) if a.any() else None,
+                                     "median_rel_err_rejected_cd": float(np.median(rel[~a, 0])) if (~a).any() else None}
     baseline = np.abs(ens.y_mu - truth) / np.maximum(np.abs(truth), 0.1 * ens.y_sd)
     solve_per_case = te.solve_seconds / args.test
     pred_per_case = t_pred / max(1, len(truth))
@@ -314,6 +324,8 @@ def main():
         "queries_per_hour_solver_only": 3600.0 / solve_per_case,
         "queries_per_hour_with_surrogate": 3600.0 / per_query,
         "solves_avoided_per_hour": frac * 3600.0 / per_query,
+        "acceptance_sweep": sweep,
+        "rule": f"accept when the ensemble spread of Cd is <= {args.threshold:.0%} of the predicted Cd",
         "total_seconds": round(time.perf_counter() - t0, 1),
     }
     ens.close()
