@@ -32,42 +32,90 @@ PREC_NAME = {0: "f64", 1: "f32", 2: "f16"}
 
 
 def workload(api, nx=8192, ny=4096, u=0.04, d=256.0, re=1000.0, ramp_units=0.0):
-    """ramp_units = 0: the run starts from the uniform stream (Channel::new with no ramp), so a
-    20000-step (3.1 D/U) run has a drag transient to compare between precisions."""
+    """8192 x 4096, TRT 3/16, notched square D=256 at (2048, 2048), Re 1000.  ramp_units = 0: the
+    run starts from the uniform stream (Channel::new with no ramp), so a 20000-step (3.1 D/U) run
+    has a drag transient to compare between precisions."""
     body = api.section("recessed", 32.0 / 256.0, 2048.0, 2048.0, d, 0.0)
     return api.ChannelSpec(nx=nx, ny=ny, inflow=api.INFLOW_UNIFORM_OPEN, u=u, nu=u * d / re, body=body,
                            collision=api.TRT, magic=3.0 / 16.0, ramp_steps=int(ramp_units * d / u))
+
+
+def workload_c1(api):
+    """The wake case on 662 x 302: BGK, uniform inflow u = 0.05 between radiating ("free-slip")
+    sides, absorbing pressure outlet ("zero-gradient" outflow), circle D = 20 at (160, 151),
+    Re = 150; a 5-unit ramp and a 2-unit body kick as the reference's towing tank does."""
+    u, d, re = 0.05, 20.0, 150.0
+    unit = d / u
+    return api.ChannelSpec(nx=662, ny=302, inflow=api.INFLOW_UNIFORM_OPEN, u=u, nu=u * d / re,
+                           body=api.circle(160.0, 151.0, 10.0), collision=api.BGK, magic=0.0,
+                           ramp_steps=int(5 * unit), kick=api.Kick(160.0, 151.0, 0.4 * u / d, int(5 * unit), int(2 * unit)))
+
+
+def workload_c3(api):
+    """16384 x 8192, TRT 3/16, circle D = 512 at (4096, 4096), u = 0.03, Re = 200, from the
+    uniform stream."""
+    u, d, re = 0.03, 512.0, 200.0
+    return api.ChannelSpec(nx=16384, ny=8192, inflow=api.INFLOW_UNIFORM_OPEN, u=u, nu=u * d / re,
+                           body=api.circle(4096.0, 4096.0, 256.0), collision=api.TRT, magic=3.0 / 16.0, ramp_steps=0)
+
+
+WORKLOADS = {
+    # name: (spec builder, reference length, storage precision, default steps, description)
+    "c2": (lambda api, scale: workload(api, nx=8192 // scale, ny=4096 // scale, d=256.0 / scale), 256.0, "f32", 20000,
+           "D2Q9 TRT 3/16 {nx}x{ny}, notched square D={d:g}, Re=1000, u=0.04, {prec}-deviation storage"),
+    "c1": (lambda api, scale: workload_c1(api), 20.0, "f64", 35660,
+           "D2Q9 BGK 662x302 wake, circle D=20 at (160,151), Re=150, u=0.05, {prec} storage"),
+    "c3": (lambda api, scale: workload_c3(api), 512.0, "f16", 50000,
+           "D2Q9 TRT 3/16 16384x8192, circle D=512, Re=200, u=0.03, {prec}-deviation storage (x2^10)"),
+}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(vl, api, precision, steps, warmup, scale):
-    spec = workload(api, nx=8192 // scale, ny=4096 // scale, d=256.0 / scale)
+def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
+    builder, d, _, _, _ = WORKLOADS[wl]
+    spec = builder(api, scale)
+    d = d / scale if wl == "c2" else d
     t0 = time.time()
     ch = vl.make_channel(spec, precision)
     info = ch.sim.info()
     log(f"[bench] {PREC_NAME[precision]}: built {info.nx}x{info.ny}, {info.fluid_nodes} fluid nodes, "
         f"{info.link_count} links in {time.time() - t0:.1f} s")
-    coef_w = ch.run(warmup, spec.body.args[4]) if warmup else np.zeros((0, 2))
+    coef_w = ch.run(warmup, d) if warmup else np.zeros((0, 2))
     # timed region: the production path (replayed step-window graphs)
+    # long runs go in pieces of 10000 steps, each followed by a progress line
+    pieces, done = [], 0
     t0 = time.perf_counter()
-    coef = ch.run(steps, spec.body.args[4])
+    while done < steps:
+        n = min(10000, steps - done)
+        pieces.append(ch.run(n, d))
+        done += n
+        if steps > 10000:
+            log(f"[bench] {PREC_NAME[precision]}: {done}/{steps} steps, {time.perf_counter() - t0:.1f} s")
     secs = time.perf_counter() - t0
+    coef = np.concatenate(pieces)
+    state_hash = ch.sim.state_hash() if wl == "c1" else None
     # kernel pass: the same step launched directly with HIP events around every bulk kernel on
     # the stream it runs on (events cannot sit inside a replayed graph)
     ch.sim.set_timing(True)
-    ch.run(min(300, steps), spec.body.args[4])
+    ch.run(min(300, steps), d)
     bulk_ms, bulk_n, total_ms, tsteps = ch.sim.timing()
     ch.sim.set_timing(False)
+    info = ch.sim.info()
     res = dict(precision=precision, fluid=int(info.fluid_nodes), bnodes=int(info.bnode_count), secs=secs,
                glups=info.fluid_nodes * steps / secs / 1e9, bulk_ms=bulk_ms / max(bulk_n, 1),
-               step_ms=total_ms / max(tsteps, 1), coef=np.concatenate([coef_w, coef]), d=spec.body.args[4],
-               dt=spec.u / spec.body.args[4])
+               step_ms=total_ms / max(tsteps, 1), coef=np.concatenate([coef_w, coef]), d=d, dt=spec.u / d,
+               split=bool(info.split), steps_total=int(info.t), nx=int(info.nx), ny=int(info.ny), spec=spec,
+               state_hash=state_hash)
+    kname = "bulk kernel" if res["split"] else "step kernel"
     log(f"[bench] {PREC_NAME[precision]}: {steps} steps in {secs:.2f} s = {res['glups']:.2f} GLUPS, "
-        f"bulk kernel {res['bulk_ms'] * 1e3:.1f} us, whole step {res['step_ms'] * 1e3:.1f} us")
-    ch.close()
+        f"{kname} {res['bulk_ms'] * 1e3:.1f} us, whole step {res['step_ms'] * 1e3:.1f} us")
+    if keep:
+        res["channel"] = ch
+    else:
+        ch.close()
     return res
 
 
@@ -76,29 +124,47 @@ def coef_stats(vl, r):
     return vl.series_stats(cd, cl, r["dt"], 0.5 * len(cd) * r["dt"])
 
 
-def cpu_baseline(api, seconds_target, scale):
-    """The oracle on the same domain, all host cores (capped at 16), bounded step count."""
+def velocity_error(a, b):
+    """Per-node velocity L2 error of run a against run b (fluid nodes; solid nodes are 0 in both)."""
+    _, ax, ay = a.sim.macros_all()
+    _, bx, by = b.sim.macros_all()
+    num = np.sqrt(np.sum((ax - bx) ** 2 + (ay - by) ** 2))
+    return float(num / np.sqrt(np.sum(bx ** 2 + by ** 2)))
+
+
+def cpu_baseline(api, wl, seconds_target, scale, steps_full, gpu_hash=None):
+    """The oracle on the same domain, all host cores (capped at 16), a bounded number of steps
+    (the whole run when it fits in `seconds_target`)."""
     import oracle
     be = oracle.backend()
     threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
-    spec = workload(api, nx=8192 // scale, ny=4096 // scale, d=256.0 / scale)
+    builder, d, _, _, _ = WORKLOADS[wl]
+    spec = builder(api, scale)
+    d = d / scale if wl == "c2" else d
     t0 = time.time()
     ch = api.Channel(be, spec, threads)
     fluid = ch.sim.fluid_nodes
     log(f"[cpu] oracle built in {time.time() - t0:.1f} s, {threads} threads")
     t0 = time.perf_counter()
-    ch.run(2, spec.body.args[4])
+    ch.run(2, d)
     per = (time.perf_counter() - t0) / 2
-    steps = max(2, int(seconds_target / max(per, 1e-6)))
+    steps = max(2, min(steps_full - 2, int(seconds_target / max(per, 1e-6))))
     t0 = time.perf_counter()
-    ch.run(steps, spec.body.args[4])
+    ch.run(steps, d)
     secs = time.perf_counter() - t0
+    match = None
+    if gpu_hash is not None and steps + 2 == steps_full:
+        match = ch.sim.state_hash() == gpu_hash
     ch.close()
     g = fluid * steps / secs / 1e9
-    log(f"[cpu] {steps} steps in {secs:.1f} s = {g * 1e3:.1f} MLUPS")
-    return {"value": round(g, 5), "unit": "GLUPS", "cores": threads, "kind": "port",
-            "sample": f"oracle (C++ restatement of the reference solver), {steps} steps of the "
-                      f"{8192 // scale}x{4096 // scale} domain from t=0, {threads} threads, {secs:.1f} s"}
+    log(f"[cpu] {steps} steps in {secs:.1f} s = {g * 1e3:.1f} MLUPS" + ("" if match is None else f"; state hash "
+                                                                         f"matches GPU: {match}"))
+    out = {"value": round(g, 5), "unit": "GLUPS", "cores": threads, "kind": "port",
+           "sample": f"oracle (C++ restatement of the reference solver), {steps + 2} steps of the "
+                     f"{spec.nx}x{spec.ny} domain from t=0, {threads} threads, {secs:.1f} s timed"}
+    if match is not None:
+        out["full_run_state_hash_matches_gpu"] = match
+    return out
 
 
 def batch_bench(vl, api, ncases, seed, sample):
@@ -163,13 +229,15 @@ def batch_bench(vl, api, ncases, seed, sample):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--precision", choices=["f32", "f16", "f64"], default="f32")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's full run)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 100; 0 for c1)")
+    ap.add_argument("--precision", choices=["f32", "f16", "f64"], default=None)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline sample length (0 = skip)")
-    ap.add_argument("--scale", type=int, default=1, help="divide the domain by this (testing only)")
-    ap.add_argument("--batch-cases", type=int, default=512, help="cases of the batched sweep (0 = skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=None,
+                    help="CPU-baseline sample length (default 15 s; 90 s for c1, which covers its whole run; 0 = skip)")
+    ap.add_argument("--scale", type=int, default=1, help="divide the c2 domain by this (testing only)")
+    ap.add_argument("--batch-cases", type=int, default=None, help="cases of the batched sweep (0 = skip)")
     ap.add_argument("--batch-sample", type=int, default=8, help="cases re-run one at a time for the comparison")
     ap.add_argument("--batch-seed", type=int, default=2024)
     args = ap.parse_args()
@@ -182,11 +250,18 @@ def main():
     from vortexlab_amd import api
     if vl.device_count() < 1:
         raise SystemExit("no HIP device visible")
-    prec = {"f64": api.F64, "f32": api.F32, "f16": api.F16}[args.precision]
-    main_run = run_gpu(vl, api, prec, args.steps, args.warmup, args.scale)
+    wl = args.workload
+    _, _, prec_default, steps_default, desc = WORKLOADS[wl]
+    steps = args.steps if args.steps is not None else steps_default
+    warmup = args.warmup if args.warmup is not None else (0 if wl == "c1" else 100)
+    batch_cases = args.batch_cases if args.batch_cases is not None else (512 if wl == "c2" else 0)
+    pname = args.precision or prec_default
+    prec = {"f64": api.F64, "f32": api.F32, "f16": api.F16}[pname]
+    compare = not args.no_fp64 and prec != api.F64
+    main_run = run_gpu(vl, api, wl, prec, steps, warmup, args.scale, keep=compare and wl == "c3")
     extra = {}
-    if not args.no_fp64 and prec != api.F64:
-        ref = run_gpu(vl, api, api.F64, args.steps, args.warmup, args.scale)
+    if compare:
+        ref = run_gpu(vl, api, wl, api.F64, steps, warmup, args.scale, keep=wl == "c3")
         s_main, s_ref = coef_stats(vl, main_run), coef_stats(vl, ref)
         cl_ref = s_ref.cl_rms if s_ref.cl_rms != 0 else 1.0
         extra = {
@@ -199,11 +274,16 @@ def main():
             "fp64_bulk_frac_hbm": round(BYTES_PER_NODE[0] * (ref["fluid"] - ref["bnodes"]) / (ref["bulk_ms"] * 1e-3)
                                         / 1e9 / HBM_PEAK_GBS, 4),
         }
-    # the dominant kernel (k_bulk) updates the interior fluid nodes; boundary nodes run concurrently
-    traffic_alg = BYTES_PER_NODE[prec] * (main_run["fluid"] - main_run["bnodes"])
+        if wl == "c3":
+            extra["velocity_l2_error_vs_fp64"] = velocity_error(main_run["channel"], ref["channel"])
+            main_run["channel"].close()
+            ref["channel"].close()
+    # the dominant kernel: k_bulk (interior nodes) in split mode, k_step (every fluid node) fused
+    units = main_run["fluid"] - main_run["bnodes"] if main_run["split"] else main_run["fluid"]
+    traffic_alg = BYTES_PER_NODE[prec] * units
     achieved = traffic_alg / (main_run["bulk_ms"] * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"bulk_traffic_{PREC_NAME[prec]}.json")
+    tf = os.path.join(ROOT, "profiles", f"bulk_traffic_{wl}_{PREC_NAME[prec]}.json")
     if os.path.exists(tf) and args.scale == 1:
         with open(tf) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
@@ -211,21 +291,27 @@ def main():
     copy = ctypes.c_double()
     vl.backend().check(vl.backend().device_copy_bandwidth(1 << 30, 20, ctypes.byref(copy)))
     log(f"[bench] plain device copy (1 GiB): {copy.value:.0f} GB/s")
-    if args.batch_cases > 0:
-        extra.update(batch_bench(vl, api, args.batch_cases, args.batch_seed, args.batch_sample))
-    cpu = cpu_baseline(api, args.cpu_seconds, args.scale) if args.cpu_seconds > 0 else None
+    if batch_cases > 0:
+        extra.update(batch_bench(vl, api, batch_cases, args.batch_seed, args.batch_sample))
+    # the CPU run covers the whole c1 job when it fits its time budget: then its final state
+    # hash is compared with the GPU run's
+    gpu_hash = main_run["state_hash"] if prec == api.F64 and warmup == 0 else None
+    cpu_seconds = args.cpu_seconds if args.cpu_seconds is not None else (90.0 if wl == "c1" else 15.0)
+    cpu = (cpu_baseline(api, wl, cpu_seconds, args.scale, steps + warmup, gpu_hash)
+           if cpu_seconds > 0 else None)
+    kname = f"k_bulk<{PREC_NAME[prec]}>" if main_run["split"] else f"k_step<{PREC_NAME[prec]}>"
     out = {
         "metric": METRIC, "value": round(main_run["glups"], 3), "unit": "GLUPS", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(main_run["secs"] / args.steps * 1e3, 4),
+        "steps": steps, "warmup": warmup, "ms_per_step": round(main_run["secs"] / steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": PREC_NAME[prec],
-        "data": "synthetic (uniform stream at t=0 past the notched square)",
-        "config": {"workload": f"D2Q9 TRT 3/16 {8192 // args.scale}x{4096 // args.scale}, notched square "
-                               f"D={256 // args.scale}, Re=1000, u=0.04, {PREC_NAME[prec]}-deviation storage",
-                   "nx": 8192 // args.scale, "ny": 4096 // args.scale, "fluid_nodes": main_run["fluid"],
+        "data": "synthetic (seeded geometry; flow from t=0)",
+        "config": {"workload": desc.format(nx=main_run["nx"], ny=main_run["ny"], d=256.0 / args.scale,
+                                           prec=PREC_NAME[prec]),
+                   "nx": main_run["nx"], "ny": main_run["ny"], "fluid_nodes": main_run["fluid"],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"k_bulk<{PREC_NAME[prec]}>", "kernel_us": round(main_run["bulk_ms"] * 1e3, 2),
+                     "kernel": kname, "kernel_us": round(main_run["bulk_ms"] * 1e3, 2),
                      "algorithmic_bytes_per_launch": traffic_alg,
                      "copy_gbs": round(copy.value, 1), "frac_of_copy": round(achieved / copy.value, 4)},
         "cpu_baseline": cpu,

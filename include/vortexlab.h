@@ -133,6 +133,8 @@ typedef struct {
   double omega_plus, omega_minus;
   double outlet_density_offset; /* Sim::outlet_density_offset (last step) */
   int64_t device_bytes;
+  int32_t split;  /* 1: boundary and bulk run as two concurrent kernels; 0: one fused kernel */
+  int32_t pad_;
 } vl_sim_info;
 
 /* Per-step host controls (what Channel::step sets before Sim::step, cases/channel.rs:136-147).

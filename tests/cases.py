@@ -162,3 +162,16 @@ def config1_spec(steps_ramp=2000, kick_steps=800):
                            body=api.circle(160.0, 151.0, 10.0), collision=api.BGK, magic=0.0,
                            ramp_steps=steps_ramp,
                            kick=api.Kick(160.0, 151.0, 0.4 * u / d, steps_ramp, kick_steps))
+
+
+def wake_662x302_spec():
+    """The 662x302 wake case at its full definition (ramp 2000 steps = 5 D/U, kick 800 steps)."""
+    return config1_spec(2000, 800)
+
+
+def fnv1a(data: bytes) -> int:
+    """64-bit FNV-1a of a byte string (offset basis 0xcbf29ce484222325, prime 0x100000001b3)."""
+    h = 0xcbf29ce484222325
+    for b in data:
+        h = ((h ^ b) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h

@@ -6,13 +6,18 @@ reference's published state hashes).  Forces use a fixed-order tree instead of t
 reference's serial sum: they must agree to 1e-9 relative and be
 bitwise reproducible run to run.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
 from vortexlab_amd import api
 
 from cases import (COMOVING_SHAPES, closed_box, comoving_body, couette, cylinder_rig_spec, open_channel,
-                   periodic_box, small_cylinder_spec, tunnel_spec)
+                   periodic_box, small_cylinder_spec, tunnel_spec, wake_662x302_spec)
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +52,21 @@ def test_published_state_hash(gpu, n, steps, golden, fluid):
     assert ch.sim.fluid_nodes == fluid
     ch.run(steps, float(n))
     assert ch.sim.state_hash() == golden, f"hash {ch.sim.state_hash():016x} != {golden:016x}"
+
+
+def test_wake_662x302_full_run_matches_oracle(gpu):
+    """The 662x302 wake case (BGK, circle D=20, Re 150) over its full 35660 steps: final
+    populations bit-identical to the oracle's (state hash) and the Cd/Cl history within 1e-9 of
+    its scale at every 10th step."""
+    with open(os.path.join(GOLDEN, "wake_662x302_35660.json")) as fh:
+        gold = json.load(fh)
+    print(f"[wake] 662x302 {gold['steps']} steps", flush=True)
+    ch = api.Channel(gpu, wake_662x302_spec(), api.F64)
+    assert ch.sim.fluid_nodes == gold["fluid_nodes"]
+    coef = ch.run(gold["steps"], gold["d"])
+    assert f"{ch.sim.state_hash():016x}" == gold["state_hash"]
+    assert_force(coef[9::10], np.array(gold["coef_every_10"]), "wake Cd/Cl")
+    ch.close()
 
 
 @pytest.mark.parametrize("model", ["bgk", "trt", "trt14"])

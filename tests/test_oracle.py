@@ -178,3 +178,29 @@ def test_series_stats(orc):
     assert not api.series_stats(orc, np.ones(n), drift, dt, 10.0).periodic
     s = api.series_stats(orc, np.ones(n), np.zeros(n), dt, 10.0)
     assert not s.periodic and s.st == 0.0 and s.cl_rms == 0.0
+
+
+def test_fnv1a_known_answers():
+    """64-bit FNV-1a published test vectors (empty string, "a", "foobar")."""
+    from cases import fnv1a
+    assert fnv1a(b"") == 0xcbf29ce484222325
+    assert fnv1a(b"a") == 0xaf63dc4c8601ec8c
+    assert fnv1a(b"foobar") == 0x85944171f73967e8
+
+
+def test_wake_golden_fixture_is_consistent():
+    """The stored wake result: its history hash covers a float64 (35660 x 2) history whose every
+    10th row is stored, and the first 1000 steps of the oracle reproduce those rows exactly."""
+    import json
+    import os
+    from cases import wake_662x302_spec
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "wake_662x302_35660.json")) as fh:
+        gold = json.load(fh)
+    rows = np.array(gold["coef_every_10"])
+    assert rows.shape == (gold["steps"] // 10, 2) and np.all(np.isfinite(rows))
+    be = oracle.backend()
+    ch = api.Channel(be, wake_662x302_spec(), os.cpu_count() or 1)
+    coef = ch.run(1000, gold["d"])
+    assert ch.sim.fluid_nodes == gold["fluid_nodes"]
+    assert np.array_equal(coef[9::10], rows[:100])
+    ch.close()

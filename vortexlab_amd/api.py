@@ -43,7 +43,8 @@ class SimInfo(C.Structure):
                 ("bnode_count", C.c_int64), ("pnode_count", C.c_int64), ("force_groups", C.c_int64),
                 ("t", C.c_int64), ("precision", C.c_int32), ("collision", C.c_int32),
                 ("omega_plus", C.c_double), ("omega_minus", C.c_double),
-                ("outlet_density_offset", C.c_double), ("device_bytes", C.c_int64)]
+                ("outlet_density_offset", C.c_double), ("device_bytes", C.c_int64), ("split", C.c_int32),
+                ("pad_", C.c_int32)]
 
 
 class Ctl(C.Structure):

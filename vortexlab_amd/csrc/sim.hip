@@ -609,6 +609,7 @@ vl_status vl_sim_info_get(vl_sim* s, vl_sim_info* o) {
   o->omega_minus = s->coll.om;
   o->outlet_density_offset = s->outlet_density_offset;
   o->device_bytes = s->device_bytes;
+  o->split = s->split ? 1 : 0;
   return VL_OK;
 }
 
