@@ -31,7 +31,7 @@ BYTES_PER_NODE = {0: 144, 1: 72, 2: 36}  # 9 reads + 9 writes per fluid-node upd
 PREC_NAME = {0: "f64", 1: "f32", 2: "f16"}
 
 
-def workload(api, nx=8192, ny=4096, u=0.04, d=256.0, re=1000.0, ramp_units=0.0):
+def workload_notched(api, nx=8192, ny=4096, u=0.04, d=256.0, re=1000.0, ramp_units=0.0):
     """8192 x 4096, TRT 3/16, notched square D=256 at (2048, 2048), Re 1000.  ramp_units = 0: the
     run starts from the uniform stream (Channel::new with no ramp), so a 20000-step (3.1 D/U) run
     has a drag transient to compare between precisions."""
@@ -40,7 +40,7 @@ def workload(api, nx=8192, ny=4096, u=0.04, d=256.0, re=1000.0, ramp_units=0.0):
                            collision=api.TRT, magic=3.0 / 16.0, ramp_steps=int(ramp_units * d / u))
 
 
-def workload_c1(api):
+def workload_wake(api):
     """The wake case on 662 x 302: BGK, uniform inflow u = 0.05 between radiating ("free-slip")
     sides, absorbing pressure outlet ("zero-gradient" outflow), circle D = 20 at (160, 151),
     Re = 150; a 5-unit ramp and a 2-unit body kick as the reference's towing tank does."""
@@ -51,7 +51,7 @@ def workload_c1(api):
                            ramp_steps=int(5 * unit), kick=api.Kick(160.0, 151.0, 0.4 * u / d, int(5 * unit), int(2 * unit)))
 
 
-def workload_c3(api):
+def workload_circle(api):
     """16384 x 8192, TRT 3/16, circle D = 512 at (4096, 4096), u = 0.03, Re = 200, from the
     uniform stream."""
     u, d, re = 0.03, 512.0, 200.0
@@ -61,11 +61,11 @@ def workload_c3(api):
 
 WORKLOADS = {
     # name: (spec builder, reference length, storage precision, default steps, description)
-    "c2": (lambda api, scale: workload(api, nx=8192 // scale, ny=4096 // scale, d=256.0 / scale), 256.0, "f32", 20000,
+    "notched_8192x4096": (lambda api, scale: workload_notched(api, nx=8192 // scale, ny=4096 // scale, d=256.0 / scale), 256.0, "f32", 20000,
            "D2Q9 TRT 3/16 {nx}x{ny}, notched square D={d:g}, Re=1000, u=0.04, {prec}-deviation storage"),
-    "c1": (lambda api, scale: workload_c1(api), 20.0, "f64", 35660,
+    "wake_662x302": (lambda api, scale: workload_wake(api), 20.0, "f64", 35660,
            "D2Q9 BGK 662x302 wake, circle D=20 at (160,151), Re=150, u=0.05, {prec} storage"),
-    "c3": (lambda api, scale: workload_c3(api), 512.0, "f16", 50000,
+    "circle_16384x8192": (lambda api, scale: workload_circle(api), 512.0, "f16", 50000,
            "D2Q9 TRT 3/16 16384x8192, circle D=512, Re=200, u=0.03, {prec}-deviation storage (x2^10)"),
 }
 
@@ -77,7 +77,7 @@ def log(msg):
 def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
     builder, d, _, _, _ = WORKLOADS[wl]
     spec = builder(api, scale)
-    d = d / scale if wl == "c2" else d
+    d = d / scale if wl == "notched_8192x4096" else d
     t0 = time.time()
     ch = vl.make_channel(spec, precision)
     info = ch.sim.info()
@@ -96,7 +96,7 @@ def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
             log(f"[bench] {PREC_NAME[precision]}: {done}/{steps} steps, {time.perf_counter() - t0:.1f} s")
     secs = time.perf_counter() - t0
     coef = np.concatenate(pieces)
-    state_hash = ch.sim.state_hash() if wl == "c1" else None
+    state_hash = ch.sim.state_hash() if wl == "wake_662x302" else None
     # kernel pass: the same step launched directly with HIP events around every bulk kernel on
     # the stream it runs on (events cannot sit inside a replayed graph)
     ch.sim.set_timing(True)
@@ -140,7 +140,7 @@ def cpu_baseline(api, wl, seconds_target, scale, steps_full, gpu_hash=None):
     threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
     builder, d, _, _, _ = WORKLOADS[wl]
     spec = builder(api, scale)
-    d = d / scale if wl == "c2" else d
+    d = d / scale if wl == "notched_8192x4096" else d
     t0 = time.time()
     ch = api.Channel(be, spec, threads)
     fluid = ch.sim.fluid_nodes
@@ -174,26 +174,17 @@ def batch_bench(vl, api, ncases, seed, sample):
     and FP64 for the drag check)."""
     from vortexlab_amd import workloads
     cases = workloads.sweep_cases(seed, ncases)
-    be = vl.backend()
-    t0 = time.perf_counter()
-    b = api.Batch(be, api.F32)
-    for c in cases:
-        b.add(c.spec, c.steps, c.d)
-    b.build()
-    t_build = time.perf_counter() - t0
+    b, t_build, t_run = workloads.run_batch(vl.backend(), cases, api.F32)
     info = b.info()
     log(f"[batch] {ncases} cases, {info.fluid_nodes / 1e6:.1f} M fluid nodes, {info.lattice_updates / 1e12:.3f} T "
         f"updates; set-up {t_build:.1f} s; tiles {info.tile_w}x{info.tile_h}, grid {info.grid_bulk}")
-    t0 = time.perf_counter()
-    b.run()
-    t_run = time.perf_counter() - t0
     results = [b.result(k) for k in range(ncases)]
     diverged = sum(1 for r in results if r[2] != 0)
     log(f"[batch] run {t_run:.2f} s = {ncases / t_run:.2f} cases/s, {info.lattice_updates / t_run / 1e9:.1f} GLUPS; "
         f"{diverged} diverged")
     rng = np.random.default_rng(seed + 1)
     pick = sorted(rng.choice(ncases, size=min(sample, ncases), replace=False).tolist())
-    seq_secs, dcd = 0.0, []
+    seq_secs, dcd, rows, times = 0.0, [], [], []
     for k in pick:
         c = cases[k]
         ch = vl.make_channel(c.spec, api.F32)
@@ -202,7 +193,11 @@ def batch_bench(vl, api, ncases, seed, sample):
             ch.run(c.steps, c.d)
         except api.Diverged:
             pass
-        seq_secs += time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        seq_secs += dt
+        done = int(ch.sim.info().t)
+        rows.append((done, done * ch.sim.fluid_nodes))
+        times.append(dt)
         ch.close()
         coef, done, st = results[k]
         if st == 0:
@@ -215,30 +210,39 @@ def batch_bench(vl, api, ncases, seed, sample):
                 pass
             ref.close()
     seq_rate = len(pick) / seq_secs
+    # one case alone costs a per-step launch time plus a per-update time: fitted on the sample,
+    # summed over every case of the batch (with the steps each ran in the batch)
+    coeff = np.linalg.lstsq(np.array(rows, dtype=float), np.array(times), rcond=None)[0]
+    fluid = [b.case_fluid_nodes(k) for k in range(ncases)]
+    seq_total = sum(coeff[0] * results[k][1] + coeff[1] * results[k][1] * fluid[k] for k in range(ncases))
+    seq_rate_all = ncases / seq_total
     log(f"[batch] sequential single-case runs: {len(pick)} cases in {seq_secs:.2f} s = {seq_rate:.2f} cases/s; "
+        f"model {coeff[0] * 1e6:.2f} us/step + {1e-9 / coeff[1]:.1f} GLUPS -> {seq_rate_all:.2f} cases/s over the batch; "
         f"max |dCd| vs FP64 {max(dcd) if dcd else float('nan'):.2e} over {len(dcd)} cases")
     b.close()
     return {"batched_cases_per_s": round(ncases / t_run, 3), "batched_glups": round(info.lattice_updates / t_run / 1e9, 2),
             "batched_cases": ncases, "batched_diverged": diverged, "batched_seconds": round(t_run, 2),
             "batched_setup_seconds": round(t_build, 2), "batched_tile": f"{info.tile_w}x{info.tile_h}",
-            "sequential_cases_per_s": round(seq_rate, 3), "sequential_sample": len(pick),
-            "batched_vs_sequential": round((ncases / t_run) / seq_rate, 2),
+            "sequential_cases_per_s": round(seq_rate_all, 3), "sequential_sample": len(pick),
+            "sequential_sample_cases_per_s": round(seq_rate, 3),
+            "sequential_step_us": round(coeff[0] * 1e6, 2), "sequential_update_glups": round(1e-9 / coeff[1], 1),
+            "batched_vs_sequential": round((ncases / t_run) / seq_rate_all, 2),
             "batched_max_dcd_vs_fp64": (max(dcd) if dcd else None), "batched_dcd_checked": len(dcd)}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="notched_8192x4096")
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's full run)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 100; 0 for c1)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 100; 0 for the wake case)")
     ap.add_argument("--precision", choices=["f32", "f16", "f64"], default=None)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=None,
-                    help="CPU-baseline sample length (default 15 s; 90 s for c1, which covers its whole run; 0 = skip)")
-    ap.add_argument("--scale", type=int, default=1, help="divide the c2 domain by this (testing only)")
+                    help="CPU-baseline sample length (default 15 s; 90 s for the wake case, which covers its whole run; 0 = skip)")
+    ap.add_argument("--scale", type=int, default=1, help="divide the notched-square domain by this (testing only)")
     ap.add_argument("--batch-cases", type=int, default=None, help="cases of the batched sweep (0 = skip)")
-    ap.add_argument("--batch-sample", type=int, default=8, help="cases re-run one at a time for the comparison")
+    ap.add_argument("--batch-sample", type=int, default=32, help="cases re-run one at a time for the comparison")
     ap.add_argument("--batch-seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -253,15 +257,15 @@ def main():
     wl = args.workload
     _, _, prec_default, steps_default, desc = WORKLOADS[wl]
     steps = args.steps if args.steps is not None else steps_default
-    warmup = args.warmup if args.warmup is not None else (0 if wl == "c1" else 100)
-    batch_cases = args.batch_cases if args.batch_cases is not None else (512 if wl == "c2" else 0)
+    warmup = args.warmup if args.warmup is not None else (0 if wl == "wake_662x302" else 100)
+    batch_cases = args.batch_cases if args.batch_cases is not None else (512 if wl == "notched_8192x4096" else 0)
     pname = args.precision or prec_default
     prec = {"f64": api.F64, "f32": api.F32, "f16": api.F16}[pname]
     compare = not args.no_fp64 and prec != api.F64
-    main_run = run_gpu(vl, api, wl, prec, steps, warmup, args.scale, keep=compare and wl == "c3")
+    main_run = run_gpu(vl, api, wl, prec, steps, warmup, args.scale, keep=compare and wl == "circle_16384x8192")
     extra = {}
     if compare:
-        ref = run_gpu(vl, api, wl, api.F64, steps, warmup, args.scale, keep=wl == "c3")
+        ref = run_gpu(vl, api, wl, api.F64, steps, warmup, args.scale, keep=wl == "circle_16384x8192")
         s_main, s_ref = coef_stats(vl, main_run), coef_stats(vl, ref)
         cl_ref = s_ref.cl_rms if s_ref.cl_rms != 0 else 1.0
         extra = {
@@ -274,7 +278,7 @@ def main():
             "fp64_bulk_frac_hbm": round(BYTES_PER_NODE[0] * (ref["fluid"] - ref["bnodes"]) / (ref["bulk_ms"] * 1e-3)
                                         / 1e9 / HBM_PEAK_GBS, 4),
         }
-        if wl == "c3":
+        if wl == "circle_16384x8192":
             extra["velocity_l2_error_vs_fp64"] = velocity_error(main_run["channel"], ref["channel"])
             main_run["channel"].close()
             ref["channel"].close()
@@ -293,10 +297,10 @@ def main():
     log(f"[bench] plain device copy (1 GiB): {copy.value:.0f} GB/s")
     if batch_cases > 0:
         extra.update(batch_bench(vl, api, batch_cases, args.batch_seed, args.batch_sample))
-    # the CPU run covers the whole c1 job when it fits its time budget: then its final state
+    # the CPU run covers the whole wake job when it fits its time budget: then its final state
     # hash is compared with the GPU run's
     gpu_hash = main_run["state_hash"] if prec == api.F64 and warmup == 0 else None
-    cpu_seconds = args.cpu_seconds if args.cpu_seconds is not None else (90.0 if wl == "c1" else 15.0)
+    cpu_seconds = args.cpu_seconds if args.cpu_seconds is not None else (90.0 if wl == "wake_662x302" else 15.0)
     cpu = (cpu_baseline(api, wl, cpu_seconds, args.scale, steps + warmup, gpu_hash)
            if cpu_seconds > 0 else None)
     kname = f"k_bulk<{PREC_NAME[prec]}>" if main_run["split"] else f"k_step<{PREC_NAME[prec]}>"

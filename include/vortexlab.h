@@ -251,6 +251,9 @@ vl_status vl_batch_info_get(vl_batch* b, vl_batch_info* out);
  * (0 running or finished, 1 diverged). */
 vl_status vl_batch_case_result(vl_batch* b, int id, double* coef, int64_t* steps_done, int* status);
 vl_status vl_batch_read_case(vl_batch* b, int id, double* f);
+/* Grid, fluid-node count and requested steps of case `id` (after vl_batch_build); any output
+ * pointer may be NULL.  Like Sim::fluid_nodes (src/sim.rs:640) for one case of the batch. */
+vl_status vl_batch_case_info(vl_batch* b, int id, int64_t* nx, int64_t* ny, int64_t* fluid_nodes, int64_t* steps);
 void vl_batch_destroy(vl_batch* b);
 
 /* ------------------------------------------------------------------------------------

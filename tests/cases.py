@@ -154,7 +154,7 @@ def tunnel_spec(d=20, re=100.0, u=0.1, upstream=6.0, downstream=12.0, height=10.
                            kick=api.Kick(cx, cy, 0.4 * u / dd, int(5.0 * unit), int(2.0 * unit)))
 
 
-def config1_spec(steps_ramp=2000, kick_steps=800):
+def wake_spec(steps_ramp=2000, kick_steps=800):
     """Wake case on 662x302: BGK, uniform inflow u=0.05, circle D=20 at (160,151), Re=150,
     radiating ("free-slip") sides, absorbing pressure outlet; 5-unit ramp and a 2-unit kick."""
     u, d, re = 0.05, 20.0, 150.0
@@ -166,7 +166,7 @@ def config1_spec(steps_ramp=2000, kick_steps=800):
 
 def wake_662x302_spec():
     """The 662x302 wake case at its full definition (ramp 2000 steps = 5 D/U, kick 800 steps)."""
-    return config1_spec(2000, 800)
+    return wake_spec(2000, 800)
 
 
 def fnv1a(data: bytes) -> int:

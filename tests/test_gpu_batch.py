@@ -50,6 +50,7 @@ def test_batch_equals_single_case_runs(gpu, prec):
         coef, done, status = b.result(k)
         assert status == 0 and done == c.steps
         ch = api.Channel(gpu, c.spec, prec)
+        assert b.case_info(k) == (c.spec.nx, c.spec.ny, ch.sim.fluid_nodes, c.steps)
         ref = ch.run(c.steps, c.d)
         assert np.array_equal(coef, ref), f"case {k}: coefficient history differs"
         assert np.array_equal(b.read_case(k, c.spec.nx, c.spec.ny), ch.sim.read()), f"case {k}: populations differ"

@@ -3,7 +3,7 @@
 The FP64 kernels keep the reference's operation order and the outlet controller's serial sum,
 so populations are expected to be bit-identical to the oracle (which itself reproduces the
 reference's published state hashes).  Forces use a fixed-order tree instead of the
-reference's serial sum: they must agree to 1e-9 relative and be
+reference's serial sum: they must agree to 1e-9 relative (the required tolerance) and be
 bitwise reproducible run to run.
 """
 import json
@@ -21,7 +21,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
-POP_RTOL = 1e-12  # north-star bound; the FP64 path is in fact bit-exact (asserted separately)
+POP_RTOL = 1e-12  # required bound; the FP64 path is in fact bit-exact (asserted separately)
 
 
 def assert_pops(g, o, label, exact=True):

@@ -1,7 +1,7 @@
 """FP32 / FP16 deviation storage against the FP64 oracle.
 
 The reduced-precision paths store d_i = f_i - w_i (FP16 scaled by 2^10) and collide in FP32,
-so they cannot be bit-exact.  Bounds: the north star's |dCd| < 0.5 % of the FP64 drag on a
+so they cannot be bit-exact.  Bounds: the required |dCd| < 0.5 % of the FP64 drag on a
 developed wake, plus per-node velocity and population error bounds and the invariants that
 do not depend on precision (a rest state and a uniform stream stay exact; mass is conserved
 to FP32 rounding).

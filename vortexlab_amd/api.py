@@ -149,6 +149,8 @@ _SIGS = [
     ("batch_info_get", C.c_int, [_VP, C.POINTER(BatchInfoDesc)]),
     ("batch_case_result", C.c_int, [_VP, C.c_int, _D, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     ("batch_read_case", C.c_int, [_VP, C.c_int, _D]),
+    ("batch_case_info", C.c_int, [_VP, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("batch_destroy", None, [_VP]),
 ]
 
@@ -618,6 +620,15 @@ class Batch:
         i = BatchInfoDesc()
         self.be.check(self.be.batch_info_get(self.h, C.byref(i)))
         return i
+
+    def case_info(self, k: int):
+        """(nx, ny, fluid_nodes, steps) of case k."""
+        v = [C.c_int64() for _ in range(4)]
+        self.be.check(self.be.batch_case_info(self.h, k, *(C.byref(x) for x in v)))
+        return tuple(int(x.value) for x in v)
+
+    def case_fluid_nodes(self, k: int) -> int:
+        return self.case_info(k)[2]
 
     def result(self, k: int):
         """(coef [steps, 2], steps_done, status) of case k."""

@@ -849,6 +849,16 @@ vl_status vl_batch_case_result(vl_batch* b, int id, double* coef, int64_t* steps
   return VL_OK;
 }
 
+vl_status vl_batch_case_info(vl_batch* b, int id, int64_t* nx, int64_t* ny, int64_t* fluid_nodes, int64_t* steps) {
+  if (!b || !b->built || id < 0 || (size_t)id >= b->hcases.size()) return set_error(VL_EINVAL, "bad case id");
+  const BatchCase& c = b->hcases[id];
+  if (nx) *nx = c.g.nx;
+  if (ny) *ny = c.g.ny;
+  if (fluid_nodes) *fluid_nodes = b->tabs[id].fluid_nodes;
+  if (steps) *steps = c.steps;
+  return VL_OK;
+}
+
 // Populations of one case in the reference layout (FP64 f_i).
 vl_status vl_batch_read_case(vl_batch* b, int id, double* f) {
   if (!b || !b->built || id < 0 || (size_t)id >= b->hcases.size() || !f) return set_error(VL_EINVAL, "bad case id");

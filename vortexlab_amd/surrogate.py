@@ -2,8 +2,8 @@
 
 The labels come from the batched solver (vl_batch); the network is trained and evaluated by
 the HIP kernels behind vl_nn_* (vortexlab_amd/csrc/nn.hip).  A 4-member ensemble answers a
-query when its members agree (relative spread <= 2 % on every output), otherwise the query
-falls back to the solver, so the ensemble is also the judge of its own error.
+query when its members agree on the drag (relative spread of the mean Cd <= 2 %), otherwise
+the query falls back to the solver, so the ensemble is also the judge of its own error.
 
 Usage (one GPU):  python -m vortexlab_amd.surrogate --train 8192 --test 1024 --out report.json
 """

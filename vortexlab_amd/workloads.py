@@ -63,3 +63,50 @@ def sweep_cases(seed: int, n: int = 512, w=(200, 1400), h=(80, 600), d=(10.0, 60
         sub = np.random.default_rng(rng.integers(1 << 62))
         cases.append(tank_case(nx, ny, dn, r, u, coll, lambda cx, cy: star_polygon(sub, dn, cx, cy), units=units))
     return cases
+
+
+def run_batch(be, cases: list, precision: int = api.F32):
+    """All cases in one batch; returns (batch, set-up seconds, run seconds).  The caller closes
+    the batch."""
+    import time
+    t0 = time.perf_counter()
+    b = api.Batch(be, precision)
+    for c in cases:
+        b.add(c.spec, c.steps, c.d)
+    b.build()
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.run()
+    return b, t_build, time.perf_counter() - t0
+
+
+def main(argv=None):
+    """Runs one seeded sweep as a batch and prints its throughput as one JSON line."""
+    import argparse
+    import json
+    import sys
+    from . import backend
+    ap = argparse.ArgumentParser(description="batched towing-tank sweep on one GPU")
+    ap.add_argument("--cases", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--units", type=float, default=20.0, help="convective units per case")
+    ap.add_argument("--precision", choices=["f32", "f16", "f64"], default="f32")
+    args = ap.parse_args(argv)
+    prec = {"f64": api.F64, "f32": api.F32, "f16": api.F16}[args.precision]
+    cases = sweep_cases(args.seed, args.cases, units=args.units)
+    b, t_build, t_run = run_batch(backend(), cases, prec)
+    info = b.info()
+    status = [b.result(k)[2] for k in range(len(cases))]
+    out = {"cases": len(cases), "precision": args.precision, "fluid_nodes": int(info.fluid_nodes),
+           "lattice_updates": int(info.lattice_updates), "max_steps": int(info.max_steps),
+           "setup_s": round(t_build, 3), "run_s": round(t_run, 3), "cases_per_s": round(len(cases) / t_run, 3),
+           "glups": round(info.lattice_updates / t_run / 1e9, 2), "diverged": int(sum(1 for s in status if s)),
+           "tile": f"{info.tile_w}x{info.tile_h}", "grid_bulk": int(info.grid_bulk),
+           "grid_boundary": int(info.grid_boundary), "boundary_tiles": int(info.boundary_tiles),
+           "bulk_tiles": int(info.bulk_tiles)}
+    b.close()
+    print(json.dumps(out), file=sys.stdout, flush=True)
+
+
+if __name__ == "__main__":
+    main()
