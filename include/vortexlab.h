@@ -247,6 +247,13 @@ vl_status vl_batch_build(vl_batch* b);
  * whose drag turns non-finite is retired (status 1) while the others go on. */
 vl_status vl_batch_run(vl_batch* b, int64_t max_steps);
 vl_status vl_batch_info_get(vl_batch* b, vl_batch_info* out);
+/* Kernel timing of the batch's own stepping (no reference counterpart: the reference's
+ * `bench` times whole runs only, src/main.rs:448).  While on, vl_batch_run launches every step
+ * directly with HIP events around the boundary kernel (side stream), the bulk kernel (main
+ * stream) and the whole step; vl_batch_get_timing returns the sums in ms since the last
+ * vl_batch_set_timing call. */
+vl_status vl_batch_set_timing(vl_batch* b, int on);
+vl_status vl_batch_get_timing(vl_batch* b, double* boundary_ms, double* bulk_ms, double* step_ms, int64_t* steps);
 /* (cd, cl) per step of case `id` (2 * steps doubles), the steps completed, and its status
  * (0 running or finished, 1 diverged). */
 vl_status vl_batch_case_result(vl_batch* b, int id, double* coef, int64_t* steps_done, int* status);

@@ -100,8 +100,17 @@ def test_graph_windows_equal_direct_launches(gpu):
         d = run_batch(gpu, cases, api.F32)
     finally:
         del os.environ["VL_GRAPHS"]
+    t = api.Batch(gpu, api.F32)
+    for c in cases:
+        t.add(c.spec, c.steps, c.d)
+    t.build()
+    t.set_timing(True)
+    t.run()
+    bnd, bulk, step, n = t.timing()
+    assert n == max(c.steps for c in cases) + 1 and 0.0 < bulk <= step and 0.0 < bnd <= step
     for k in range(len(cases)):
         assert np.array_equal(a.result(k)[0], d.result(k)[0])
+        assert np.array_equal(a.result(k)[0], t.result(k)[0])
 
 
 def test_read_case_mid_run(gpu):

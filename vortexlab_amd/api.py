@@ -151,6 +151,8 @@ _SIGS = [
     ("batch_read_case", C.c_int, [_VP, C.c_int, _D]),
     ("batch_case_info", C.c_int, [_VP, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    ("batch_set_timing", C.c_int, [_VP, C.c_int]),
+    ("batch_get_timing", C.c_int, [_VP, _D, _D, _D, C.POINTER(C.c_int64)]),
     ("batch_destroy", None, [_VP]),
 ]
 
@@ -626,6 +628,16 @@ class Batch:
         v = [C.c_int64() for _ in range(4)]
         self.be.check(self.be.batch_case_info(self.h, k, *(C.byref(x) for x in v)))
         return tuple(int(x.value) for x in v)
+
+    def set_timing(self, on: bool):
+        self.be.check(self.be.batch_set_timing(self.h, 1 if on else 0))
+
+    def timing(self):
+        """(boundary_ms, bulk_ms, step_ms, steps) summed since set_timing."""
+        v = [C.c_double() for _ in range(3)]
+        n = C.c_int64()
+        self.be.check(self.be.batch_get_timing(self.h, *(C.byref(x) for x in v), C.byref(n)))
+        return v[0].value, v[1].value, v[2].value, int(n.value)
 
     def case_fluid_nodes(self, k: int) -> int:
         return self.case_info(k)[2]

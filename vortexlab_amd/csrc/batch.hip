@@ -242,6 +242,15 @@ struct vl_batch {
   double autotune_ms[3][3] = {};
   double last_run_seconds = 0.0;
   int64_t device_bytes = 0;
+  // kernel timing (vl_batch_set_timing): events around each kernel of each step, read back
+  // every TIMING_W steps
+  struct StepEvents {
+    hipEvent_t bs, be, ks, ke, ej;
+  };
+  bool timing = false;
+  std::vector<StepEvents> tev;
+  double tm_bnd = 0.0, tm_bulk = 0.0, tm_step = 0.0;
+  int64_t tm_steps = 0;
 };
 
 namespace {
@@ -312,32 +321,59 @@ void bulk_tiles(const vl_batch* b, const std::vector<int>& order, int tw, int th
   }
 }
 
+constexpr int TIMING_W = 64;
+
+// One step: the boundary kernel on the side stream and the bulk on the main stream, forked
+// after the previous step and joined before the next.  `ev` (timing runs only, never inside a
+// captured graph) brackets each kernel on its own stream.
 template <class S>
-vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local) {
+vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, const vl_batch::StepEvents* ev) {
   using T = typename S::T;
   const T* A = (const T*)b->pool[b->cur];
   T* B = (T*)b->pool[b->cur ^ 1];
   VL_HIP(hipEventRecord(b->ev_fork, b->stream));
   VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+  if (ev) VL_HIP(hipEventRecord(ev->bs, b->side));
   hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, b->side, A, B, b->dcases,
                      b->d_bcase, b->d_bb0, b->d_bact, tbase, t_local, b->d_kick, b->d_hist, b->d_status, b->d_done,
                      SQRT_CS2_B);
+  if (ev) VL_HIP(hipEventRecord(ev->be, b->side));
   VL_HIP(hipEventRecord(b->ev_join, b->side));
+  if (ev) VL_HIP(hipEventRecord(ev->ks, b->stream));
   hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)b->grid_bulk), dim3(BT), 0, b->stream, A, B, b->dflags,
                      b->dcases, b->d_kcase, b->d_ky0, b->d_kx0, b->d_kact, tbase, t_local, b->tw_log2, b->d_status);
+  if (ev) VL_HIP(hipEventRecord(ev->ke, b->stream));
   VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+  if (ev) VL_HIP(hipEventRecord(ev->ej, b->stream));
   VL_HIP(hipGetLastError());
   b->cur ^= 1;
   b->t += 1;
   return VL_OK;
 }
 
-vl_status batch_step(vl_batch* b, const int64_t* tbase, int t_local) {
+vl_status batch_step(vl_batch* b, const int64_t* tbase, int t_local, const vl_batch::StepEvents* ev = nullptr) {
   switch (b->precision) {
-    case VL_F64: return launch_batch_step<StoreF64>(b, tbase, t_local);
-    case VL_F16: return launch_batch_step<StoreF16>(b, tbase, t_local);
-    default: return launch_batch_step<StoreF32>(b, tbase, t_local);
+    case VL_F64: return launch_batch_step<StoreF64>(b, tbase, t_local, ev);
+    case VL_F16: return launch_batch_step<StoreF16>(b, tbase, t_local, ev);
+    default: return launch_batch_step<StoreF32>(b, tbase, t_local, ev);
   }
+}
+
+// Adds the kernel and step times of the first n steps' events to the running totals.
+vl_status read_timing(vl_batch* b, int n) {
+  VL_HIP(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < n; ++i) {
+    const auto& e = b->tev[(size_t)i];
+    float bnd = 0.0f, bulk = 0.0f, step = 0.0f;
+    VL_HIP(hipEventElapsedTime(&bnd, e.bs, e.be));
+    VL_HIP(hipEventElapsedTime(&bulk, e.ks, e.ke));
+    VL_HIP(hipEventElapsedTime(&step, e.ks, e.ej));
+    b->tm_bnd += bnd;
+    b->tm_bulk += bulk;
+    b->tm_step += step;
+    b->tm_steps += 1;
+  }
+  return VL_OK;
 }
 
 template <class S>
@@ -371,6 +407,9 @@ void free_batch(vl_batch* b) {
   if (b->pool[1]) (void)hipFree(b->pool[1]);
   for (auto& g : b->graph)
     if (g) (void)hipGraphExecDestroy(g);
+  for (auto& e : b->tev)
+    for (hipEvent_t h : {e.bs, e.be, e.ks, e.ke, e.ej})
+      if (h) (void)hipEventDestroy(h);
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->ev_join) (void)hipEventDestroy(b->ev_join);
   if (b->side) (void)hipStreamDestroy(b->side);
@@ -789,7 +828,21 @@ vl_status vl_batch_run(vl_batch* b, int64_t max_steps) {
   if (todo <= 0) return VL_OK;
   const auto t0 = std::chrono::steady_clock::now();
   int64_t k = 0;
-  if (b->use_graphs && todo >= GRAPH_W) {
+  if (b->timing) {
+    // direct launches with events around every kernel
+    VL_HIP(hipMemcpyAsync(b->d_tbase, &b->t, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
+    VL_HIP(hipStreamSynchronize(b->stream));
+    int pending = 0;
+    for (int64_t kk = 0; kk < todo; ++kk) {
+      VL_TRYB(batch_step(b, b->d_tbase, (int)kk, &b->tev[(size_t)pending]));
+      if (++pending == TIMING_W || kk + 1 == todo) {
+        VL_TRYB(read_timing(b, pending));
+        pending = 0;
+      }
+    }
+    k = todo;
+  }
+  if (b->use_graphs && todo - k >= GRAPH_W) {
     // the window counter holds the global step of the window's first step
     VL_HIP(hipMemcpyAsync(b->d_tbase, &b->t, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
     VL_HIP(hipStreamSynchronize(b->stream));
@@ -809,6 +862,33 @@ vl_status vl_batch_run(vl_batch* b, int64_t max_steps) {
   }
   VL_HIP(hipStreamSynchronize(b->stream));
   b->last_run_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return VL_OK;
+}
+
+vl_status vl_batch_set_timing(vl_batch* b, int on) {
+  if (!b) return set_error(VL_EINVAL, "null batch");
+  if (on && b->tev.empty()) {
+    b->tev.resize(TIMING_W);
+    for (auto& e : b->tev) {
+      VL_HIP(hipEventCreate(&e.bs));
+      VL_HIP(hipEventCreate(&e.be));
+      VL_HIP(hipEventCreate(&e.ks));
+      VL_HIP(hipEventCreate(&e.ke));
+      VL_HIP(hipEventCreate(&e.ej));
+    }
+  }
+  b->timing = on != 0;
+  b->tm_bnd = b->tm_bulk = b->tm_step = 0.0;
+  b->tm_steps = 0;
+  return VL_OK;
+}
+
+vl_status vl_batch_get_timing(vl_batch* b, double* boundary_ms, double* bulk_ms, double* step_ms, int64_t* steps) {
+  if (!b) return set_error(VL_EINVAL, "null batch");
+  if (boundary_ms) *boundary_ms = b->tm_bnd;
+  if (bulk_ms) *bulk_ms = b->tm_bulk;
+  if (step_ms) *step_ms = b->tm_step;
+  if (steps) *steps = b->tm_steps;
   return VL_OK;
 }
 

@@ -80,6 +80,40 @@ def run_batch(be, cases: list, precision: int = api.F32):
     return b, t_build, time.perf_counter() - t0
 
 
+def run_phases(be, cases: list, precision: int, pieces: int):
+    """Runs the batch with kernel timing on, in `pieces` equal spans of steps, and prints per
+    span the active cases, node updates and the boundary / bulk / whole-step time per step."""
+    import sys
+    import time
+    b = api.Batch(be, precision)
+    for c in cases:
+        b.add(c.spec, c.steps, c.d)
+    b.build()
+    info = b.info()
+    steps = np.array([c.steps for c in cases])
+    fluid = np.array([b.case_fluid_nodes(k) for k in range(len(cases))])
+    end = int(info.max_steps) + 1
+    span = -(-end // pieces)
+    t = 0
+    while t < end:
+        n = min(span, end - t)
+        b.set_timing(True)
+        w0 = time.perf_counter()
+        b.run(n)
+        wall = time.perf_counter() - w0
+        bnd, bulk, step, ns = b.timing()
+        lo, hi = t, t + n
+        active = int(np.sum(steps > lo))
+        lu = int(np.sum(fluid * np.clip(np.minimum(steps, hi) - lo, 0, None)))
+        print(f"[phase] steps {lo}-{hi}: {active} cases active, {lu / 1e9:.1f} G updates; per step "
+              f"boundary {bnd / ns * 1e3:.1f} us, bulk {bulk / ns * 1e3:.1f} us, step {step / ns * 1e3:.1f} us; "
+              f"{lu / (step * 1e-3) / 1e9:.1f} GLUPS on the step time, {lu / wall / 1e9:.1f} on the wall clock",
+              file=sys.stderr, flush=True)
+        t += n
+    b.set_timing(False)
+    return b
+
+
 def main(argv=None):
     """Runs one seeded sweep as a batch and prints its throughput as one JSON line."""
     import argparse
@@ -91,10 +125,16 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--units", type=float, default=20.0, help="convective units per case")
     ap.add_argument("--precision", choices=["f32", "f16", "f64"], default="f32")
+    ap.add_argument("--phases", type=int, default=0,
+                    help="if > 0: run with kernel timing on, in this many pieces, one line per piece")
     args = ap.parse_args(argv)
     prec = {"f64": api.F64, "f32": api.F32, "f16": api.F16}[args.precision]
     cases = sweep_cases(args.seed, args.cases, units=args.units)
-    b, t_build, t_run = run_batch(backend(), cases, prec)
+    if args.phases > 0:
+        b = run_phases(backend(), cases, prec, args.phases)
+        t_build, t_run = 0.0, b.info().last_run_seconds
+    else:
+        b, t_build, t_run = run_batch(backend(), cases, prec)
     info = b.info()
     status = [b.result(k)[2] for k in range(len(cases))]
     out = {"cases": len(cases), "precision": args.precision, "fluid_nodes": int(info.fluid_nodes),
