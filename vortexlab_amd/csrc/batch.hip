@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <thread>
@@ -235,11 +236,16 @@ struct vl_batch {
   double* d_kick = nullptr;
   int nbnd_tiles = 0, nbulk_tiles = 0;
   int tw_log2 = 6, th = 4, grid_bulk = 0, grid_bnd = 0;
+  // bulk grid policy: 0 = one workgroup per active tile (the grid follows the active-tile count
+  // on a geometric ladder), m > 0 = persistent grid of m workgroups per CU
+  int bulk_mult = 0;
+  std::vector<int32_t> h_kact;  // active bulk tiles per step (host copy of d_kact)
+  std::vector<int32_t> ladder;  // grid sizes used when bulk_mult == 0
   int cur = 0;
   int64_t t = 0;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (A/B parity, bulk grid) -> window
   bool use_graphs = true;
-  double autotune_ms[3][3] = {};
+  double autotune_ms[3][4] = {};
   double last_run_seconds = 0.0;
   int64_t device_bytes = 0;
   // kernel timing (vl_batch_set_timing): events around each kernel of each step, read back
@@ -327,7 +333,8 @@ constexpr int TIMING_W = 64;
 // after the previous step and joined before the next.  `ev` (timing runs only, never inside a
 // captured graph) brackets each kernel on its own stream.
 template <class S>
-vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, const vl_batch::StepEvents* ev) {
+vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, int grid_bulk,
+                            const vl_batch::StepEvents* ev) {
   using T = typename S::T;
   const T* A = (const T*)b->pool[b->cur];
   T* B = (T*)b->pool[b->cur ^ 1];
@@ -340,7 +347,7 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, cons
   if (ev) VL_HIP(hipEventRecord(ev->be, b->side));
   VL_HIP(hipEventRecord(b->ev_join, b->side));
   if (ev) VL_HIP(hipEventRecord(ev->ks, b->stream));
-  hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)b->grid_bulk), dim3(BT), 0, b->stream, A, B, b->dflags,
+  hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)grid_bulk), dim3(BT), 0, b->stream, A, B, b->dflags,
                      b->dcases, b->d_kcase, b->d_ky0, b->d_kx0, b->d_kact, tbase, t_local, b->tw_log2, b->d_status);
   if (ev) VL_HIP(hipEventRecord(ev->ke, b->stream));
   VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
@@ -351,12 +358,35 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, cons
   return VL_OK;
 }
 
-vl_status batch_step(vl_batch* b, const int64_t* tbase, int t_local, const vl_batch::StepEvents* ev = nullptr) {
+vl_status batch_step(vl_batch* b, const int64_t* tbase, int t_local, int grid_bulk,
+                     const vl_batch::StepEvents* ev = nullptr) {
   switch (b->precision) {
-    case VL_F64: return launch_batch_step<StoreF64>(b, tbase, t_local, ev);
-    case VL_F16: return launch_batch_step<StoreF16>(b, tbase, t_local, ev);
-    default: return launch_batch_step<StoreF32>(b, tbase, t_local, ev);
+    case VL_F64: return launch_batch_step<StoreF64>(b, tbase, t_local, grid_bulk, ev);
+    case VL_F16: return launch_batch_step<StoreF16>(b, tbase, t_local, grid_bulk, ev);
+    default: return launch_batch_step<StoreF32>(b, tbase, t_local, grid_bulk, ev);
   }
+}
+
+// Geometric ladder of grid sizes (each about 9 % above the last, the last one = all tiles).
+void make_ladder(vl_batch* b) {
+  b->ladder.clear();
+  int64_t v = 1;
+  while (v < b->nbulk_tiles) {
+    b->ladder.push_back((int32_t)v);
+    v = std::max(v + 1, (v * 109 + 99) / 100);
+  }
+  b->ladder.push_back(std::max(1, b->nbulk_tiles));
+}
+
+// Bulk grid for a span of steps starting at global step t: the active-tile count can only
+// shrink with t, so the count at t covers the span.  The kernel loops over tiles when the
+// grid is smaller than the count, so any grid is correct; this one only sets the speed.
+int bulk_grid_at(const vl_batch* b, int64_t t) {
+  if (b->bulk_mult > 0) return b->grid_bulk;
+  const size_t i = (size_t)std::min<int64_t>(std::max<int64_t>(t, 0), (int64_t)b->h_kact.size() - 1);
+  const int32_t need = std::max<int32_t>(b->h_kact[i], 1);
+  auto it = std::lower_bound(b->ladder.begin(), b->ladder.end(), need);
+  return it == b->ladder.end() ? b->ladder.back() : *it;
 }
 
 // Adds the kernel and step times of the first n steps' events to the running totals.
@@ -405,8 +435,8 @@ void free_batch(vl_batch* b) {
   for (void* p : b->allocs) (void)hipFree(p);
   if (b->pool[0]) (void)hipFree(b->pool[0]);
   if (b->pool[1]) (void)hipFree(b->pool[1]);
-  for (auto& g : b->graph)
-    if (g) (void)hipGraphExecDestroy(g);
+  for (auto& kv : b->graphs)
+    if (kv.second) (void)hipGraphExecDestroy(kv.second);
   for (auto& e : b->tev)
     for (hipEvent_t h : {e.bs, e.be, e.ks, e.ke, e.ej})
       if (h) (void)hipEventDestroy(h);
@@ -684,14 +714,14 @@ vl_status build(vl_batch* b) {
     VL_HIP(hipGetLastError());
   }
   VL_HIP(hipStreamSynchronize(b->stream));
-  // ---- autotune the bulk tile shape and the persistent grid on the real batch ----
+  // ---- autotune the bulk tile shape and the grid policy on the real batch ----
   hipDeviceProp_t prop;
   VL_HIP(hipGetDeviceProperties(&prop, 0));
   const int cus = std::max(1, prop.multiProcessorCount);
   b->grid_bnd = std::max(1, std::min(b->nbnd_tiles, cus * 2));
   if (const char* e = std::getenv("VL_BATCH_GRID_BND")) b->grid_bnd = std::max(1, std::min(b->nbnd_tiles, std::atoi(e)));
   const int shapes[3][2] = {{6, 4}, {7, 2}, {8, 1}};  // (log2 TW, TH)
-  const int mults[3] = {4, 8, 16};
+  const int mults[4] = {0, 4, 8, 16};  // 0: one workgroup per tile; else workgroups per CU
   double best = 1e30;
   std::vector<int32_t> kc, ky, kx, kend;
   hipEvent_t e0, e1;
@@ -706,8 +736,9 @@ vl_status build(vl_batch* b) {
     VL_TRYB(dupload(b, tky, ky));
     VL_TRYB(dupload(b, tkx, kx));
     VL_TRYB(dupload(b, tact, act1));
-    for (int mi = 0; mi < 3; ++mi) {
-      const int grid = std::max(1, std::min((int)kc.size(), cus * mults[mi]));
+    for (int mi = 0; mi < 4; ++mi) {
+      const int grid = mults[mi] == 0 ? std::max(1, (int)kc.size())
+                                      : std::max(1, std::min((int)kc.size(), cus * mults[mi]));
       // A -> B with the same A every time: the state does not change
       VL_TRYB(bulk_only(b, grid, shapes[si][0], tkc, tky, tkx, tact));
       VL_HIP(hipEventRecord(e0, b->stream));
@@ -730,7 +761,11 @@ vl_status build(vl_batch* b) {
   b->th = shapes[best_s][1];
   bulk_tiles(b, order, 1 << b->tw_log2, b->th, kc, ky, kx, kend);
   b->nbulk_tiles = (int)kc.size();
-  b->grid_bulk = std::max(1, std::min(b->nbulk_tiles, cus * mults[best_m]));
+  b->bulk_mult = mults[best_m];
+  if (const char* e = std::getenv("VL_BATCH_BULK_MULT")) b->bulk_mult = std::max(0, std::atoi(e));
+  b->grid_bulk = b->bulk_mult == 0 ? std::max(1, b->nbulk_tiles)
+                                   : std::max(1, std::min(b->nbulk_tiles, cus * b->bulk_mult));
+  make_ladder(b);
   std::vector<int32_t> kact((size_t)b->max_steps + 2, 0);
   for (int64_t tt = 0; tt <= b->max_steps + 1; ++tt) {
     int32_t n = 0;
@@ -742,6 +777,7 @@ vl_status build(vl_batch* b) {
   VL_TRYB(dupload(b, b->d_ky0, ky));
   VL_TRYB(dupload(b, b->d_kx0, kx));
   VL_TRYB(dupload(b, b->d_kact, kact));
+  b->h_kact = std::move(kact);
   b->use_graphs = true;
   if (const char* e = std::getenv("VL_GRAPHS")) b->use_graphs = e[0] != '0';
   b->built = true;
@@ -749,9 +785,11 @@ vl_status build(vl_batch* b) {
 }
 
 vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
-  const int key = b->cur & 1;
-  if (b->graph[key]) {
-    *out = b->graph[key];
+  const int grid = bulk_grid_at(b, b->t);
+  const auto key = std::make_pair(b->cur & 1, grid);
+  auto found = b->graphs.find(key);
+  if (found != b->graphs.end()) {
+    *out = found->second;
     return VL_OK;
   }
   const int cur0 = b->cur;
@@ -759,7 +797,7 @@ vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
   hipGraph_t g = nullptr;
   VL_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   vl_status st = VL_OK;
-  for (int k = 0; k < GRAPH_W && st == VL_OK; ++k) st = batch_step(b, b->d_tbase, k);
+  for (int k = 0; k < GRAPH_W && st == VL_OK; ++k) st = batch_step(b, b->d_tbase, k, grid);
   if (st == VL_OK) hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, b->stream, b->d_tbase, (int64_t)GRAPH_W);
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   b->cur = cur0;
@@ -769,13 +807,12 @@ vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
     return st;
   }
   if (e != hipSuccess) return set_error(VL_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  e = hipGraphInstantiate(&b->graph[key], g, nullptr, nullptr, 0);
+  hipGraphExec_t ge = nullptr;
+  e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
-  if (e != hipSuccess) {
-    b->graph[key] = nullptr;
-    return set_error(VL_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-  }
-  *out = b->graph[key];
+  if (e != hipSuccess) return set_error(VL_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  b->graphs[key] = ge;
+  *out = ge;
   return VL_OK;
 }
 
@@ -834,7 +871,7 @@ vl_status vl_batch_run(vl_batch* b, int64_t max_steps) {
     VL_HIP(hipStreamSynchronize(b->stream));
     int pending = 0;
     for (int64_t kk = 0; kk < todo; ++kk) {
-      VL_TRYB(batch_step(b, b->d_tbase, (int)kk, &b->tev[(size_t)pending]));
+      VL_TRYB(batch_step(b, b->d_tbase, (int)kk, bulk_grid_at(b, b->t), &b->tev[(size_t)pending]));
       if (++pending == TIMING_W || kk + 1 == todo) {
         VL_TRYB(read_timing(b, pending));
         pending = 0;
@@ -858,7 +895,7 @@ vl_status vl_batch_run(vl_batch* b, int64_t max_steps) {
     VL_HIP(hipStreamSynchronize(b->stream));
     VL_HIP(hipMemcpyAsync(b->d_tbase, &b->t, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
     VL_HIP(hipStreamSynchronize(b->stream));
-    for (int kk = 0; k < todo; ++k, ++kk) VL_TRYB(batch_step(b, b->d_tbase, kk));
+    for (int kk = 0; k < todo; ++k, ++kk) VL_TRYB(batch_step(b, b->d_tbase, kk, bulk_grid_at(b, b->t)));
   }
   VL_HIP(hipStreamSynchronize(b->stream));
   b->last_run_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -910,7 +947,7 @@ vl_status vl_batch_info_get(vl_batch* b, vl_batch_info* o) {
   o->last_run_seconds = b->last_run_seconds;
   o->device_bytes = b->device_bytes;
   for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) o->autotune_ms[i][j] = b->autotune_ms[i][j];
+    for (int j = 0; j < 4; ++j) o->autotune_ms[i][j] = b->autotune_ms[i][j];
   return VL_OK;
 }
 
