@@ -718,7 +718,7 @@ vl_status build(vl_batch* b) {
   hipDeviceProp_t prop;
   VL_HIP(hipGetDeviceProperties(&prop, 0));
   const int cus = std::max(1, prop.multiProcessorCount);
-  b->grid_bnd = std::max(1, std::min(b->nbnd_tiles, cus * 2));
+  b->grid_bnd = std::max(1, b->nbnd_tiles);  // one workgroup per boundary tile
   if (const char* e = std::getenv("VL_BATCH_GRID_BND")) b->grid_bnd = std::max(1, std::min(b->nbnd_tiles, std::atoi(e)));
   const int shapes[3][2] = {{6, 4}, {7, 2}, {8, 1}};  // (log2 TW, TH)
   const int mults[4] = {0, 4, 8, 16};  // 0: one workgroup per tile; else workgroups per CU

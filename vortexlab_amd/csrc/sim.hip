@@ -99,6 +99,7 @@ static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool 
   DevTables& t = s->dt;
   StepArgs sa;
   sa.nlb = s->nlb;
+  sa.fblk = s->fblk;
   sa.xblocks = (g.nx + STEP_T - 1) / STEP_T;
   sa.sqrt_cs2 = SQRT_CS2;
   const int64_t ns2 = (int64_t)std::max(t.nslots, 1) * 2;
@@ -359,7 +360,7 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
     VL_HIP(hipMemset(t.fcontrib, 0, fcb));
   }
   s->nlb = (t.nb + STEP_T - 1) / STEP_T;
-  if (s->nlb == 0 && t.ngroups > 0) s->nlb = 1;
+  s->fblk = t.ngroups > 0 ? s->nlb++ : -1;
   // Split the step into two concurrent kernels once the bulk is large enough for the
   // per-step fork/join to be negligible (VL_STEP_MODE=fused|split overrides).
   s->split = (int64_t)d.nx * d.ny >= ((int64_t)1 << 21);

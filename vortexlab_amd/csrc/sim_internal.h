@@ -72,7 +72,8 @@ struct vl_sim {
   double outlet_density_offset = 0.0;
   int64_t t = 0;
   int device = 0;
-  int nlb = 0;                       // boundary workgroups of a step
+  int nlb = 0;                       // boundary workgroups of a step (force workgroup included)
+  int fblk = -1;                     // the force workgroup, last of the boundary workgroups
   bool split = false;                // boundary and bulk as two concurrent kernels
   hipStream_t side = nullptr;        // stream of the boundary kernel in split mode
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;

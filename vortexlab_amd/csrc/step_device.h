@@ -81,7 +81,8 @@ __device__ inline void pull(const typename S::T* __restrict__ A, const Geom& g, 
 constexpr int STEP_T = 256;
 
 struct StepArgs {
-  int nlb;        // boundary workgroups
+  int nlb;        // boundary workgroups (the force workgroup included)
+  int fblk;       // workgroup that reduces the previous step's force terms (-1: none)
   int xblocks;    // bulk tiles per row (fused kernel)
   double sqrt_cs2;
   double* fc_cur;        // force terms written this step (per force slot: x, y)
@@ -202,8 +203,12 @@ __device__ inline void boundary_node(const typename S::T* __restrict__ A, typena
   using C = typename S::C;
   const int x = t.bx[b], y = t.by[b];
   const int64_t idx = (int64_t)y * g.pitch + x;
-  C f[Q];
+  C f[Q], own[Q];
   pull<S>(A, g, x, y, f);
+  // This node's own populations in A (what leaves it along each direction), loaded together
+  // with the pull so that no link waits on a load of its own.
+#pragma unroll
+  for (int k = 0; k < Q; ++k) own[k] = S::ld(A + k * g.plane + idx);
   // Local velocity for anti-bounce-back links, computed once (the reference recomputes the
   // same value per link).
   bool have_u = false;
@@ -212,7 +217,7 @@ __device__ inline void boundary_node(const typename S::T* __restrict__ A, typena
   for (int l = t.bfirst[b]; l < l1; ++l) {
     const int j = t.dir[l];
     const int i = opp(j);
-    const C out = S::ld(A + j * g.plane + idx);
+    const C out = own[j];
     const double scale = ctl.vscale[t.vgroup[l]];
     C back;
     if (t.kind[l] == LINK_WALL) {
@@ -221,7 +226,7 @@ __device__ inline void boundary_node(const typename S::T* __restrict__ A, typena
         const double corr = t.cu6[l] * scale;
         if (q >= 0.5) {
           const double k = 0.5 / q;
-          back = k * (out - corr) + (1.0 - k) * A[i * g.plane + idx];
+          back = k * (out - corr) + (1.0 - k) * own[i];
         } else if (t.lflags[l] & 2) {
           const int bxn = wrapi(x - cx(j), g.nx), byn = wrapi(y - cy(j), g.ny);
           back = 2.0 * q * out + (1.0 - 2.0 * q) * A[j * g.plane + (int64_t)byn * g.pitch + bxn] - corr;
@@ -235,7 +240,7 @@ __device__ inline void boundary_node(const typename S::T* __restrict__ A, typena
         const float qf = (float)q;
         if (q >= 0.5) {
           const float k = (float)(0.5 / q);
-          back = k * (out - corr) + (1.0f - k) * S::ld(A + i * g.plane + idx);
+          back = k * (out - corr) + (1.0f - k) * own[i];
         } else if (t.lflags[l] & 2) {
           const int bxn = wrapi(x - cx(j), g.nx), byn = wrapi(y - cy(j), g.ny);
           back = 2.0f * qf * out + (1.0f - 2.0f * qf) * S::ld(A + j * g.plane + (int64_t)byn * g.pitch + bxn) -
@@ -248,19 +253,13 @@ __device__ inline void boundary_node(const typename S::T* __restrict__ A, typena
       if (!have_u) {
         have_u = true;
         if constexpr (S::exact) {
-          double fa[Q];
-#pragma unroll
-          for (int k = 0; k < Q; ++k) fa[k] = A[k * g.plane + idx];
           double rho, jx, jy;
-          Exact::moments(fa, rho, jx, jy);
+          Exact::moments(own, rho, jx, jy);
           ux = jx / rho - 0.5 * c.gx;
           uy = jy / rho - 0.5 * c.gy;
         } else {
-          float da[Q];
-#pragma unroll
-          for (int k = 0; k < Q; ++k) da[k] = S::ld(A + k * g.plane + idx);
           float dr, jx, jy;
-          Dev::moments(da, dr, jx, jy);
+          Dev::moments(own, dr, jx, jy);
           const double rho = 1.0 + (double)dr;
           ux = (double)jx / rho - 0.5 * c.gx;
           uy = (double)jy / rho - 0.5 * c.gy;
@@ -308,9 +307,13 @@ __device__ inline void boundary_block(int blk, const typename S::T* __restrict__
                                       const Geom& g, const Coll& c, const DevTables& t, const StepArgs& sa,
                                       double* red) {
   const int64_t k = *sa.kbase + sa.k_local;
+  if (blk == sa.fblk) {
+    // a workgroup of its own: the force terms of the previous step of this chunk go to its
+    // history slot, off the path of the boundary nodes
+    if (sa.hist && k > 0) reduce_forces(sa.fc_prev, t.gfirst, t.ngroups, sa.hist + (k - 1) * 2 * t.ngroups, red);
+    return;
+  }
   const StepCtl ctl = sa.ctls[k];
-  // the force terms of the previous step of this chunk go to its history slot
-  if (blk == 0 && sa.hist && k > 0) reduce_forces(sa.fc_prev, t.gfirst, t.ngroups, sa.hist + (k - 1) * 2 * t.ngroups, red);
   double drho = 0.0;
   if (ctl.has_target && t.np > 0 && blk * STEP_T < t.npb) {
     drho = outlet_offset<S>(sa.pv_cur, t.np, ctl.target, sa.sqrt_cs2, red);
