@@ -55,10 +55,13 @@ __global__ __launch_bounds__(STEP_T) void k_boundary(const typename S::T* __rest
 
 constexpr int BULK_T = 512;
 
-template <class S, int M>
+template <class S, int M, int V>
 __global__ __launch_bounds__(BULK_T) void k_bulk(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
                                                  const uint8_t* __restrict__ flag, Geom g, Coll c, int xblocks) {
-  bulk_node<S, M>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, g, c);
+  if constexpr (V == 1)
+    bulk_node<S, M>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, g, c);
+  else
+    bulk_vec<S, M, V>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, g, c);
 }
 
 // Advances the window counter at the end of a replayed window of steps.
@@ -126,9 +129,24 @@ static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool 
       VL_HIP(hipEventRecord(s->ev_join, s->side));
     }
     if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
-    const int xb = (g.nx + BULK_T - 1) / BULK_T;
-    hipLaunchKernelGGL((k_bulk<S, M>), dim3((unsigned)(xb * g.ny)), dim3(BULK_T), 0, s->stream, A, B, s->dflag, g,
-                       s->coll, xb);
+    const int v = s->bulk_v;
+    const int xb = (g.nx + BULK_T * v - 1) / (BULK_T * v);
+    const dim3 grid((unsigned)(xb * g.ny));
+    switch (v) {
+      case 1:
+        hipLaunchKernelGGL((k_bulk<S, M, 1>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, g, s->coll, xb);
+        break;
+      case 2:
+        hipLaunchKernelGGL((k_bulk<S, M, 2>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, g, s->coll, xb);
+        break;
+      case 4:
+        if constexpr (sizeof(T) <= 4)
+          hipLaunchKernelGGL((k_bulk<S, M, 4>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, g, s->coll, xb);
+        break;
+      default:
+        if constexpr (sizeof(T) == 2)
+          hipLaunchKernelGGL((k_bulk<S, M, 8>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, g, s->coll, xb);
+    }
   }
   if (timed) {
     VL_HIP(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
@@ -365,6 +383,13 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   // per-step fork/join to be negligible (VL_STEP_MODE=fused|split overrides).
   s->split = (int64_t)d.nx * d.ny >= ((int64_t)1 << 21);
   if (const char* e = std::getenv("VL_STEP_MODE")) s->split = std::string(e) == "split";
+  // nodes per thread of the split-mode bulk: 16-byte vectors by default (VL_BULK_V = 1, 2, 4
+  // or 8 overrides; 8 only for FP16, whose elements are 2 bytes)
+  s->bulk_v = precision == VL_F64 ? 2 : (precision == VL_F32 ? 4 : 8);
+  if (const char* e = std::getenv("VL_BULK_V")) {
+    const int v = std::atoi(e);
+    if (v == 1 || v == 2 || (v == 4 && precision != VL_F64) || (v == 8 && precision == VL_F16)) s->bulk_v = v;
+  }
   if (s->split) {
     VL_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
     VL_HIP(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));

@@ -75,6 +75,7 @@ struct vl_sim {
   int nlb = 0;                       // boundary workgroups of a step (force workgroup included)
   int fblk = -1;                     // the force workgroup, last of the boundary workgroups
   bool split = false;                // boundary and bulk as two concurrent kernels
+  int bulk_v = 1;                    // nodes per thread of the split-mode bulk kernel
   hipStream_t side = nullptr;        // stream of the boundary kernel in split mode
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipStream_t stream = nullptr;

@@ -12,10 +12,15 @@ namespace vl {
 // ------------------------------------------------------------------------------------------
 // Storage classes
 // ------------------------------------------------------------------------------------------
+// T: stored type; C: compute type; V: nodes per thread of the vectorised bulk (16-byte loads);
+// cv / enc convert one stored value to / from the compute type.
 struct StoreF64 {
   using T = double;
   using C = double;
   static constexpr bool exact = true;
+  static constexpr int V = 2;
+  __device__ static inline C cv(T v) { return v; }
+  __device__ static inline T enc(C v) { return v; }
   __device__ static inline C ld(const T* p) { return *p; }
   __device__ static inline void st(T* p, C v) { *p = v; }
   __device__ static inline void st_stream(T* p, C v) { __builtin_nontemporal_store(v, p); }
@@ -25,6 +30,9 @@ struct StoreF32 {
   using T = float;
   using C = float;
   static constexpr bool exact = false;
+  static constexpr int V = 4;
+  __device__ static inline C cv(T v) { return v; }
+  __device__ static inline T enc(C v) { return v; }
   __device__ static inline C ld(const T* p) { return *p; }
   __device__ static inline void st(T* p, C v) { *p = v; }
   __device__ static inline void st_stream(T* p, C v) { __builtin_nontemporal_store(v, p); }
@@ -37,6 +45,9 @@ struct StoreF16 {
   using T = _Float16;
   using C = float;
   static constexpr bool exact = false;
+  static constexpr int V = 8;
+  __device__ static inline C cv(T v) { return (float)v * F16_INV; }
+  __device__ static inline T enc(C v) { return (_Float16)(v * F16_SCALE); }
   __device__ static inline C ld(const T* p) { return (float)*p * F16_INV; }
   __device__ static inline void st(T* p, C v) { *p = (_Float16)(v * F16_SCALE); }
   __device__ static inline void st_stream(T* p, C v) { __builtin_nontemporal_store((_Float16)(v * F16_SCALE), p); }
@@ -338,6 +349,87 @@ __device__ inline void bulk_node(int tile, int width, int xblocks, const typenam
   collide<S, M>(f, c);
 #pragma unroll
   for (int i = 0; i < Q; ++i) S::st_stream(B + i * g.plane + idx, f[i]);
+}
+
+// V consecutive interior nodes of one row per thread (the split-mode bulk).  Each population
+// plane is read as one aligned vector per thread (16 bytes at the storage's default V), plus one element beyond the vector's
+// left or right end for the directions that shift along x, and written as one vector.  Threads
+// whose V nodes are not all interior, or whose neighbours would wrap around a periodic x edge,
+// take the one-node path for each of their nodes; the arithmetic per node is the same in both.
+template <int V>
+struct FlagWord;
+template <>
+struct FlagWord<2> {
+  using T = uint16_t;
+};
+template <>
+struct FlagWord<4> {
+  using T = uint32_t;
+};
+template <>
+struct FlagWord<8> {
+  using T = uint64_t;
+};
+
+template <class S, int M, int V>
+__device__ inline void bulk_vec(int tile, int width, int xblocks, const typename S::T* __restrict__ A,
+                                typename S::T* __restrict__ B, const uint8_t* __restrict__ flag, const Geom& g,
+                                const Coll& c) {
+  using T = typename S::T;
+  using C = typename S::C;
+  using VT = T __attribute__((ext_vector_type(V)));
+  const int y = tile / xblocks;
+  const int x0 = ((tile - y * xblocks) * width + (int)threadIdx.x) * V;
+  if (x0 >= g.nx) return;
+  const int64_t r0 = (int64_t)y * g.pitch;
+  const auto fw = *(const typename FlagWord<V>::T*)(flag + r0 + x0);
+  if (fw == 0 && x0 >= 1 && x0 + V < g.nx) {
+    const int ym = wrapi(y - 1, g.ny), yp = wrapi(y + 1, g.ny);
+    const int64_t rm = (int64_t)ym * g.pitch + x0, rp = (int64_t)yp * g.pitch + x0, rc = r0 + x0;
+    const int64_t P = g.plane;
+    const VT v0 = *(const VT*)(A + rc);
+    const VT v1 = *(const VT*)(A + P + rc);
+    const VT v2 = *(const VT*)(A + 2 * P + rm);
+    const VT v3 = *(const VT*)(A + 3 * P + rc);
+    const VT v4 = *(const VT*)(A + 4 * P + rp);
+    const VT v5 = *(const VT*)(A + 5 * P + rm);
+    const VT v6 = *(const VT*)(A + 6 * P + rm);
+    const VT v7 = *(const VT*)(A + 7 * P + rp);
+    const VT v8 = *(const VT*)(A + 8 * P + rp);
+    const T l1 = A[P + rc - 1], r3 = A[3 * P + rc + V], l5 = A[5 * P + rm - 1], r6 = A[6 * P + rm + V];
+    const T r7 = A[7 * P + rp + V], l8 = A[8 * P + rp - 1];
+    VT o[Q];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const int km = k > 0 ? k - 1 : 0, kp = k < V - 1 ? k + 1 : V - 1;
+      C f[Q];
+      f[0] = S::cv(v0[k]);
+      f[1] = S::cv(k == 0 ? l1 : v1[km]);
+      f[2] = S::cv(v2[k]);
+      f[3] = S::cv(k == V - 1 ? r3 : v3[kp]);
+      f[4] = S::cv(v4[k]);
+      f[5] = S::cv(k == 0 ? l5 : v5[km]);
+      f[6] = S::cv(k == V - 1 ? r6 : v6[kp]);
+      f[7] = S::cv(k == V - 1 ? r7 : v7[kp]);
+      f[8] = S::cv(k == 0 ? l8 : v8[km]);
+      collide<S, M>(f, c);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) o[q][k] = S::enc(f[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(o[q], (VT*)(B + q * P + rc));
+    return;
+  }
+#pragma unroll 1
+  for (int k = 0; k < V; ++k) {
+    const int x = x0 + k;
+    if (x >= g.nx || flag[r0 + x] != 0) continue;
+    C f[Q];
+    pull<S>(A, g, x, y, f);
+    collide<S, M>(f, c);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) S::st_stream(B + i * g.plane + r0 + x, f[i]);
+  }
 }
 
 }  // namespace vl
