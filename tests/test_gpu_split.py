@@ -61,14 +61,14 @@ def test_split_equals_fused_reduced_precision(gpu, monkeypatch, prec):
     spec = tunnel_spec(d=16, re=100.0)
     make(monkeypatch, "fused")
     ref = api.Channel(gpu, spec, prec)
-    ref_coef = ref.run(600, 16.0)
+    ref_coef = ref.run(1200, 16.0)
     ref_pops = ref.sim.read()
     ref.close()
     for v in WIDTHS[prec]:
         print(f"[split] prec={prec} V={v}", flush=True)
         make(monkeypatch, "split", v)
         ch = api.Channel(gpu, spec, prec)
-        coef = ch.run(600, 16.0)
+        coef = ch.run(1200, 16.0)
         assert np.array_equal(ch.sim.read(), ref_pops), f"V={v}: populations differ"
         assert np.array_equal(coef, ref_coef), f"V={v}: forces differ"
         ch.close()

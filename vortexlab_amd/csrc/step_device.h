@@ -41,17 +41,25 @@ struct StoreF32 {
 // FP16 deviations scaled by 2^10: keeps |d| ~ 1e-6..1e-1 inside the normal half range.
 constexpr float F16_SCALE = 1024.0f;
 constexpr float F16_INV = 1.0f / 1024.0f;
+// An FP32 value rounded to FP32 and then to FP16 (two roundings, as the source says).  The
+// empty asm keeps the compiler from folding the FMA that produced `v` into a mixed-precision
+// FMA with an FP16 result (one rounding): that fold happens in some unrollings and not in
+// others, and would make the same node's bits depend on the kernel variant.
+__device__ inline _Float16 to_f16(float v) {
+  asm("" : "+v"(v));
+  return (_Float16)(v * F16_SCALE);
+}
 struct StoreF16 {
   using T = _Float16;
   using C = float;
   static constexpr bool exact = false;
   static constexpr int V = 8;
   __device__ static inline C cv(T v) { return (float)v * F16_INV; }
-  __device__ static inline T enc(C v) { return (_Float16)(v * F16_SCALE); }
+  __device__ static inline T enc(C v) { return to_f16(v); }
   __device__ static inline C ld(const T* p) { return (float)*p * F16_INV; }
-  __device__ static inline void st(T* p, C v) { *p = (_Float16)(v * F16_SCALE); }
-  __device__ static inline void st_stream(T* p, C v) { __builtin_nontemporal_store((_Float16)(v * F16_SCALE), p); }
-  __device__ static inline C round(C v) { return (float)(_Float16)(v * F16_SCALE) * F16_INV; }  // = ld(st(v))
+  __device__ static inline void st(T* p, C v) { *p = to_f16(v); }
+  __device__ static inline void st_stream(T* p, C v) { __builtin_nontemporal_store(to_f16(v), p); }
+  __device__ static inline C round(C v) { return (float)to_f16(v) * F16_INV; }  // = ld(st(v))
 };
 
 template <class S, int M>
