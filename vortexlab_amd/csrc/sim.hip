@@ -32,14 +32,14 @@ namespace vl {
 // Small domains: the whole step in one launch (boundary workgroups first).
 template <class S, int M>
 __global__ __launch_bounds__(STEP_T) void k_step(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
-                                                 const uint8_t* __restrict__ flag, Geom g, Coll c, DevTables t,
-                                                 StepArgs sa) {
+                                                 const uint8_t* __restrict__ flag, const int2* __restrict__ rows,
+                                                 Geom g, Coll c, DevTables t, StepArgs sa) {
   __shared__ double red[STEP_T];
   const int blk = (int)blockIdx.x;
   if (blk < sa.nlb)
     boundary_block<S, M>(blk, A, B, g, c, t, sa, red);
   else
-    bulk_node<S, M>(blk - sa.nlb, STEP_T, sa.xblocks, A, B, flag, g, c);
+    bulk_node<S, M>(blk - sa.nlb, STEP_T, sa.xblocks, A, B, flag, rows, g, c);
 }
 
 // Large domains: the boundary part and the bulk as two kernels on two streams, so each gets
@@ -57,11 +57,12 @@ constexpr int BULK_T = 512;
 
 template <class S, int M, int V>
 __global__ __launch_bounds__(BULK_T) void k_bulk(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
-                                                 const uint8_t* __restrict__ flag, Geom g, Coll c, int xblocks) {
+                                                 const uint8_t* __restrict__ flag, const int2* __restrict__ rows,
+                                                 Geom g, Coll c, int xblocks) {
   if constexpr (V == 1)
-    bulk_node<S, M>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, g, c);
+    bulk_node<S, M>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, rows, g, c);
   else
-    bulk_vec<S, M, V>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, g, c);
+    bulk_vec<S, M, V>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, rows, g, c);
 }
 
 // Advances the window counter at the end of a replayed window of steps.
@@ -119,7 +120,7 @@ static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool 
   if (!s->split) {
     if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
     hipLaunchKernelGGL((k_step<S, M>), dim3((unsigned)(sa.nlb + sa.xblocks * g.ny)), dim3(STEP_T), 0, s->stream, A,
-                       B, s->dflag, g, s->coll, t, sa);
+                       B, s->dflag, s->drows, g, s->coll, t, sa);
   } else {
     if (sa.nlb > 0) {
       VL_HIP(hipEventRecord(s->ev_fork, s->stream));
@@ -134,18 +135,22 @@ static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool 
     const dim3 grid((unsigned)(xb * g.ny));
     switch (v) {
       case 1:
-        hipLaunchKernelGGL((k_bulk<S, M, 1>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, g, s->coll, xb);
+        hipLaunchKernelGGL((k_bulk<S, M, 1>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, s->drows, g,
+                           s->coll, xb);
         break;
       case 2:
-        hipLaunchKernelGGL((k_bulk<S, M, 2>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, g, s->coll, xb);
+        hipLaunchKernelGGL((k_bulk<S, M, 2>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, s->drows, g,
+                           s->coll, xb);
         break;
       case 4:
         if constexpr (sizeof(T) <= 4)
-          hipLaunchKernelGGL((k_bulk<S, M, 4>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, g, s->coll, xb);
+          hipLaunchKernelGGL((k_bulk<S, M, 4>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, s->drows, g,
+                           s->coll, xb);
         break;
       default:
         if constexpr (sizeof(T) == 2)
-          hipLaunchKernelGGL((k_bulk<S, M, 8>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, g, s->coll, xb);
+          hipLaunchKernelGGL((k_bulk<S, M, 8>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, s->drows, g,
+                           s->coll, xb);
     }
   }
   if (timed) {
@@ -282,6 +287,7 @@ static void free_sim(vl_sim* s) {
   f(s->buf[0]);
   f(s->buf[1]);
   f(s->dflag);
+  f(s->drows);
   DevTables& t = s->dt;
   for (void* p : {(void*)t.bx, (void*)t.by, (void*)t.bfirst, (void*)t.dir, (void*)t.kind, (void*)t.vgroup,
                   (void*)t.lflags, (void*)t.lnx, (void*)t.lny, (void*)t.fslot, (void*)t.q, (void*)t.uwx,
@@ -343,6 +349,23 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   for (int64_t y = 0; y < d.ny; ++y)
     std::memcpy(&pflag[y * s->g.pitch], &s->tab.flag[y * d.nx], (size_t)d.nx);
   VL_TRY(upload(s, s->dflag, pflag));
+  // interior span of every row (see row_span)
+  {
+    std::vector<int2> rows((size_t)d.ny);
+    for (int64_t y = 0; y < d.ny; ++y) {
+      const uint8_t* fr = &s->tab.flag[y * d.nx];
+      int lo = 0, hi = -1;
+      for (int64_t x = 0; x < d.nx; ++x)
+        if (fr[x] == 0) {
+          if (hi < lo) lo = (int)x;
+          hi = (int)x;
+        }
+      bool mixed = false;
+      for (int x = lo; x <= hi && !mixed; ++x) mixed = fr[x] != 0;
+      rows[(size_t)y] = make_int2(mixed ? -lo - 1 : lo, hi);
+    }
+    VL_TRY(upload(s, s->drows, rows));
+  }
   // Outlet-controller nodes first: only the leading boundary workgroups need the offset.
   Tables tb = s->tab;
   s->dt.npb = outlet_nodes_first(tb);

@@ -84,6 +84,7 @@ struct vl_sim {
   size_t elem = 8;
   size_t pop_bytes = 0;  // bytes of one population array
   uint8_t* dflag = nullptr;
+  int2* drows = nullptr;  // interior span of every row
   vl::DevTables dt;
   double* dhist = nullptr;  // device force history of a run chunk (hist_cap steps)
   vl::StepCtl* d_ctl = nullptr;  // controls of a run chunk (device)

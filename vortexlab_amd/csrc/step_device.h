@@ -342,16 +342,35 @@ __device__ inline void boundary_block(int blk, const typename S::T* __restrict__
   if (b < t.nb) boundary_node<S, M>(A, B, g, c, t, ctl, drho, sa.sqrt_cs2, b, sa.fc_cur, sa.pv_next);
 }
 
+// Interior span of a row, {lo, hi}: its first and last interior node (hi < lo: none).  A row
+// whose span also holds boundary or solid nodes is stored with lo as -lo - 1, and only such
+// rows read the per-node flag; elsewhere the span alone says which nodes are interior.
+__device__ inline bool row_span(const int2* __restrict__ rows, int y, int& lo, int& hi) {
+  const int2 r = rows[y];
+  lo = r.x < 0 ? -r.x - 1 : r.x;
+  hi = r.y;
+  return r.x < 0;
+}
+
+__device__ inline bool is_interior(const int2* __restrict__ rows, const uint8_t* __restrict__ flag, int64_t idx,
+                                   int x, int y) {
+  if (!rows) return flag[idx] == 0;
+  int lo, hi;
+  const bool mixed = row_span(rows, y, lo, hi);
+  return x >= lo && x <= hi && (!mixed || flag[idx] == 0);
+}
+
 // One interior node of bulk tile `tile` (tiles are `width` consecutive nodes of one row).
+// `rows` (optional) holds each row's interior span; without it every node's flag is read.
 template <class S, int M>
 __device__ inline void bulk_node(int tile, int width, int xblocks, const typename S::T* __restrict__ A,
-                                 typename S::T* __restrict__ B, const uint8_t* __restrict__ flag, const Geom& g,
-                                 const Coll& c) {
+                                 typename S::T* __restrict__ B, const uint8_t* __restrict__ flag,
+                                 const int2* __restrict__ rows, const Geom& g, const Coll& c) {
   const int y = tile / xblocks;
   const int x = (tile - y * xblocks) * width + (int)threadIdx.x;
   if (x >= g.nx) return;
   const int64_t idx = (int64_t)y * g.pitch + x;
-  if (flag[idx] != 0) return;
+  if (!is_interior(rows, flag, idx, x, y)) return;
   typename S::C f[Q];
   pull<S>(A, g, x, y, f);
   collide<S, M>(f, c);
@@ -381,8 +400,8 @@ struct FlagWord<8> {
 
 template <class S, int M, int V>
 __device__ inline void bulk_vec(int tile, int width, int xblocks, const typename S::T* __restrict__ A,
-                                typename S::T* __restrict__ B, const uint8_t* __restrict__ flag, const Geom& g,
-                                const Coll& c) {
+                                typename S::T* __restrict__ B, const uint8_t* __restrict__ flag,
+                                const int2* __restrict__ rows, const Geom& g, const Coll& c) {
   using T = typename S::T;
   using C = typename S::C;
   using VT = T __attribute__((ext_vector_type(V)));
@@ -390,8 +409,16 @@ __device__ inline void bulk_vec(int tile, int width, int xblocks, const typename
   const int x0 = ((tile - y * xblocks) * width + (int)threadIdx.x) * V;
   if (x0 >= g.nx) return;
   const int64_t r0 = (int64_t)y * g.pitch;
-  const auto fw = *(const typename FlagWord<V>::T*)(flag + r0 + x0);
-  if (fw == 0 && x0 >= 1 && x0 + V < g.nx) {
+  bool all_interior;
+  if (rows) {
+    int lo, hi;
+    const bool mixed = row_span(rows, y, lo, hi);
+    if (x0 + V - 1 < lo || x0 > hi) return;
+    all_interior = x0 >= lo && x0 + V - 1 <= hi && (!mixed || *(const typename FlagWord<V>::T*)(flag + r0 + x0) == 0);
+  } else {
+    all_interior = *(const typename FlagWord<V>::T*)(flag + r0 + x0) == 0;
+  }
+  if (all_interior && x0 >= 1 && x0 + V < g.nx) {
     const int ym = wrapi(y - 1, g.ny), yp = wrapi(y + 1, g.ny);
     const int64_t rm = (int64_t)ym * g.pitch + x0, rp = (int64_t)yp * g.pitch + x0, rc = r0 + x0;
     const int64_t P = g.plane;
@@ -431,7 +458,7 @@ __device__ inline void bulk_vec(int tile, int width, int xblocks, const typename
 #pragma unroll 1
   for (int k = 0; k < V; ++k) {
     const int x = x0 + k;
-    if (x >= g.nx || flag[r0 + x] != 0) continue;
+    if (x >= g.nx || !is_interior(rows, flag, r0 + x, x, y)) continue;
     C f[Q];
     pull<S>(A, g, x, y, f);
     collide<S, M>(f, c);
