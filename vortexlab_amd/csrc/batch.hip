@@ -239,6 +239,7 @@ struct vl_batch {
   // bulk grid policy: 0 = one workgroup per active tile (the grid follows the active-tile count
   // on a geometric ladder), m > 0 = persistent grid of m workgroups per CU
   int bulk_mult = 0;
+  bool serial = false;  // boundary kernel before the bulk on one stream (else the two run side by side)
   std::vector<int32_t> h_kact;  // active bulk tiles per step (host copy of d_kact)
   std::vector<int32_t> ladder;  // grid sizes used when bulk_mult == 0
   int cur = 0;
@@ -338,19 +339,23 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, int 
   using T = typename S::T;
   const T* A = (const T*)b->pool[b->cur];
   T* B = (T*)b->pool[b->cur ^ 1];
-  VL_HIP(hipEventRecord(b->ev_fork, b->stream));
-  VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
-  if (ev) VL_HIP(hipEventRecord(ev->bs, b->side));
-  hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, b->side, A, B, b->dcases,
+  // serial: the boundary kernel first, then the bulk, both on the main stream
+  hipStream_t bs = b->serial ? b->stream : b->side;
+  if (!b->serial) {
+    VL_HIP(hipEventRecord(b->ev_fork, b->stream));
+    VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+  }
+  if (ev) VL_HIP(hipEventRecord(ev->bs, bs));
+  hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, bs, A, B, b->dcases,
                      b->d_bcase, b->d_bb0, b->d_bact, tbase, t_local, b->d_kick, b->d_hist, b->d_status, b->d_done,
                      SQRT_CS2_B);
-  if (ev) VL_HIP(hipEventRecord(ev->be, b->side));
-  VL_HIP(hipEventRecord(b->ev_join, b->side));
+  if (ev) VL_HIP(hipEventRecord(ev->be, bs));
+  if (!b->serial) VL_HIP(hipEventRecord(b->ev_join, b->side));
   if (ev) VL_HIP(hipEventRecord(ev->ks, b->stream));
   hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)grid_bulk), dim3(BT), 0, b->stream, A, B, b->dflags,
                      b->dcases, b->d_kcase, b->d_ky0, b->d_kx0, b->d_kact, tbase, t_local, b->tw_log2, b->d_status);
   if (ev) VL_HIP(hipEventRecord(ev->ke, b->stream));
-  VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+  if (!b->serial) VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
   if (ev) VL_HIP(hipEventRecord(ev->ej, b->stream));
   VL_HIP(hipGetLastError());
   b->cur ^= 1;
@@ -397,7 +402,7 @@ vl_status read_timing(vl_batch* b, int n) {
     float bnd = 0.0f, bulk = 0.0f, step = 0.0f;
     VL_HIP(hipEventElapsedTime(&bnd, e.bs, e.be));
     VL_HIP(hipEventElapsedTime(&bulk, e.ks, e.ke));
-    VL_HIP(hipEventElapsedTime(&step, e.ks, e.ej));
+    VL_HIP(hipEventElapsedTime(&step, e.bs, e.ej));
     b->tm_bnd += bnd;
     b->tm_bulk += bulk;
     b->tm_step += step;
@@ -780,6 +785,7 @@ vl_status build(vl_batch* b) {
   b->h_kact = std::move(kact);
   b->use_graphs = true;
   if (const char* e = std::getenv("VL_GRAPHS")) b->use_graphs = e[0] != '0';
+  if (const char* e = std::getenv("VL_BATCH_SERIAL")) b->serial = e[0] == '1';
   b->built = true;
   return VL_OK;
 }
