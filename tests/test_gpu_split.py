@@ -1,9 +1,9 @@
 """The split step (boundary kernel beside a vectorised bulk kernel) against the oracle and the
 fused step.
 
-Large domains run the bulk with V nodes per thread and 16-byte vector loads; nodes whose
-neighbours wrap around a periodic edge, or that share a vector with a non-interior node, take
-the one-node path.  The arithmetic per node is the same in every variant, so FP64 must stay
+Large domains run the bulk with V nodes per thread (vector loads), or with the x-shifted
+populations staged through LDS; nodes whose neighbours wrap around a periodic edge, or that
+share a vector with a non-interior node, take the one-node path.  The arithmetic per node is the same in every variant, so FP64 must stay
 bit-identical to the oracle and to the reference's published hash, and FP32 / FP16 must give
 the same bits as the fused kernel.  VL_STEP_MODE forces the mode and VL_BULK_V the vector
 width (both read when a simulation is created).
@@ -17,7 +17,8 @@ from cases import cylinder_rig_spec, periodic_box, tunnel_spec
 
 pytestmark = pytest.mark.gpu
 
-WIDTHS = {api.F64: ["1", "2"], api.F32: ["1", "2", "4"], api.F16: ["2", "4", "8"]}
+# "0": one node per thread with the x-shifted populations staged through LDS
+WIDTHS = {api.F64: ["0", "1", "2"], api.F32: ["0", "1", "2", "4"], api.F16: ["0", "2", "4", "8"]}
 
 
 def make(monkeypatch, mode, v=None):

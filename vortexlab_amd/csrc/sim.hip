@@ -55,11 +55,14 @@ __global__ __launch_bounds__(STEP_T) void k_boundary(const typename S::T* __rest
 
 constexpr int BULK_T = 512;
 
+// V = 0: one node per thread with the x-shifted populations staged through LDS (bulk_lds).
 template <class S, int M, int V>
 __global__ __launch_bounds__(BULK_T) void k_bulk(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
                                                  const uint8_t* __restrict__ flag, const int2* __restrict__ rows,
                                                  Geom g, Coll c, int xblocks) {
-  if constexpr (V == 1)
+  if constexpr (V == 0)
+    bulk_lds<S, M, BULK_T>((int)blockIdx.x, xblocks, A, B, flag, rows, g, c);
+  else if constexpr (V == 1)
     bulk_node<S, M>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, rows, g, c);
   else
     bulk_vec<S, M, V>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, rows, g, c);
@@ -131,9 +134,13 @@ static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool 
     }
     if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
     const int v = s->bulk_v;
-    const int xb = (g.nx + BULK_T * v - 1) / (BULK_T * v);
+    const int xb = (g.nx + BULK_T * std::max(v, 1) - 1) / (BULK_T * std::max(v, 1));
     const dim3 grid((unsigned)(xb * g.ny));
     switch (v) {
+      case 0:
+        hipLaunchKernelGGL((k_bulk<S, M, 0>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, s->drows, g,
+                           s->coll, xb);
+        break;
       case 1:
         hipLaunchKernelGGL((k_bulk<S, M, 1>), grid, dim3(BULK_T), 0, s->stream, A, B, s->dflag, s->drows, g,
                            s->coll, xb);
@@ -407,13 +414,14 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   s->split = (int64_t)d.nx * d.ny >= ((int64_t)1 << 21);
   if (const char* e = std::getenv("VL_STEP_MODE")) s->split = std::string(e) == "split";
   // nodes per thread of the split-mode bulk (VL_BULK_V = 1, 2, 4 or 8 overrides; 4 not for
-  // FP64, 8 only for FP16).  Measured on MI355X at 8192 x 4096: FP32 and FP64 are fastest with
+  // FP64, 8 only for FP16; 0 = one node per thread with LDS-staged x neighbours).  Measured on MI355X at 8192 x 4096: FP32 and FP64 are fastest with
   // one node per thread (4- and 8-byte loads; wider vectors cost occupancy), FP16 with two
   // (4-byte loads instead of 2-byte ones).
   s->bulk_v = precision == VL_F16 ? 2 : 1;
   if (const char* e = std::getenv("VL_BULK_V")) {
     const int v = std::atoi(e);
-    if (v == 1 || v == 2 || (v == 4 && precision != VL_F64) || (v == 8 && precision == VL_F16)) s->bulk_v = v;
+    if (v == 0 || v == 1 || v == 2 || (v == 4 && precision != VL_F64) || (v == 8 && precision == VL_F16))
+      s->bulk_v = v;
   }
   if (s->split) {
     VL_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));

@@ -467,4 +467,70 @@ __device__ inline void bulk_vec(int tile, int width, int xblocks, const typename
   }
 }
 
+// One row segment of W nodes per workgroup of W threads, with the six populations that shift
+// along x staged through LDS: every global load of the segment is then aligned to the segment
+// (plus one element on each side for the halo), and each thread reads its x - 1 / x + 1
+// neighbours from LDS.  Nodes at x = 0 or x = nx - 1 (interior only in x-periodic domains) pull
+// from global memory with the wrap.  Same arithmetic per node as bulk_node.
+template <class S, int M, int W>
+__device__ inline void bulk_lds(int tile, int xblocks, const typename S::T* __restrict__ A,
+                                typename S::T* __restrict__ B, const uint8_t* __restrict__ flag,
+                                const int2* __restrict__ rows, const Geom& g, const Coll& c) {
+  using T = typename S::T;
+  using C = typename S::C;
+  __shared__ T sh[6][W + 2];  // planes 1, 3, 5, 6, 7, 8; element k holds x = xs - 1 + k
+  const int y = tile / xblocks;
+  const int xs = (tile - y * xblocks) * W;
+  const int tid = (int)threadIdx.x;
+  const int x = xs + tid;
+  int lo = 0, hi = g.nx - 1;
+  bool mixed = true;
+  if (rows) {
+    mixed = row_span(rows, y, lo, hi);
+    if (xs > hi || xs + W - 1 < lo) return;  // the same for the whole workgroup
+  }
+  const int ym = wrapi(y - 1, g.ny), yp = wrapi(y + 1, g.ny);
+  const int64_t r0 = (int64_t)y * g.pitch, rm = (int64_t)ym * g.pitch, rp = (int64_t)yp * g.pitch;
+  const int64_t P = g.plane;
+  const int64_t off[6] = {P + r0, 3 * P + r0, 5 * P + rm, 6 * P + rm, 7 * P + rp, 8 * P + rp};
+  const bool in_row = x < g.nx;
+  T d0 = T(0), d2 = T(0), d4 = T(0);
+  if (in_row) {
+    d0 = A[r0 + x];
+    d2 = A[2 * P + rm + x];
+    d4 = A[4 * P + rp + x];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sh[k][tid + 1] = in_row ? A[off[k] + x] : T(0);
+  if (tid == 0) {
+    const int xl = xs > 0 ? xs - 1 : 0;  // x = 0 itself pulls through the global path
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sh[k][0] = A[off[k] + xl];
+  }
+  if (tid == W - 1) {
+    const int xr = xs + W < g.nx ? xs + W : g.nx - 1;  // likewise x = nx - 1
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sh[k][W + 1] = A[off[k] + xr];
+  }
+  __syncthreads();
+  if (!in_row || x < lo || x > hi || (mixed && flag[r0 + x] != 0)) return;
+  C f[Q];
+  if (x == 0 || x == g.nx - 1) {
+    pull<S>(A, g, x, y, f);
+  } else {
+    f[0] = S::cv(d0);
+    f[1] = S::cv(sh[0][tid]);
+    f[2] = S::cv(d2);
+    f[3] = S::cv(sh[1][tid + 2]);
+    f[4] = S::cv(d4);
+    f[5] = S::cv(sh[2][tid]);
+    f[6] = S::cv(sh[3][tid + 2]);
+    f[7] = S::cv(sh[4][tid + 2]);
+    f[8] = S::cv(sh[5][tid]);
+  }
+  collide<S, M>(f, c);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) S::st_stream(B + i * P + r0 + x, f[i]);
+}
+
 }  // namespace vl
