@@ -303,6 +303,9 @@ vl_status vl_series_stats(const double* cd, const double* cl, size_t n, double d
                           double from, vl_stats* out);
 /* signal::dominant_frequency (signal.rs:92-107): out2 = (frequency, amplitude). */
 vl_status vl_dominant_frequency(const double* x, size_t n, double dt, double* out2);
+/* signal::peaks (signal.rs:109-123): local maxima above the mean with parabolic refinement,
+ * (fractional index, value); `count` = all peaks, the first `cap` are written. */
+vl_status vl_peaks(const double* x, size_t n, double* pos, double* val, size_t cap, size_t* count);
 /* signal::crossing_frequency (signal.rs:127-140); returns 0 when fewer than two crossings. */
 vl_status vl_crossing_frequency(const double* x, size_t n, double dt, double* out, int* found);
 /* In-place radix-2 FFT (signal.rs:6-43); n must be a power of two. */

@@ -776,6 +776,22 @@ std::array<double, 2> dominant_frequency(const double* x, size_t n, double dt) {
   return {s[k][0] + shift * df, std::exp(b - 0.25 * (a - c) * shift)};
 }
 
+// signal.rs:109-123
+std::vector<std::array<double, 2>> peaks(const double* x, size_t n) {
+  std::vector<std::array<double, 2>> out;
+  if (n < 3) return out;
+  const double m = mean(x, n);
+  for (size_t i = 1; i < n - 1; ++i) {
+    if (x[i] > m && x[i] > x[i - 1] && x[i] >= x[i + 1]) {
+      double a = x[i - 1], b = x[i], c = x[i + 1];
+      double denom = a - 2.0 * b + c;
+      double shift = denom != 0.0 ? 0.5 * (a - c) / denom : 0.0;
+      out.push_back({(double)i + shift, b - 0.25 * (a - c) * shift});
+    }
+  }
+  return out;
+}
+
 // signal.rs:127-140
 bool crossing_frequency(const double* x, size_t n, double dt, double& f) {
   double m = mean(x, n);
@@ -1269,6 +1285,15 @@ vl_status or_dominant_frequency(const double* x, size_t n, double dt, double* ou
   auto r = dominant_frequency(x, n, dt);
   out2[0] = r[0];
   out2[1] = r[1];
+  return VL_OK;
+}
+vl_status or_peaks(const double* x, size_t n, double* pos, double* val, size_t cap, size_t* count) {
+  auto p = peaks(x, n);
+  for (size_t k = 0; k < p.size() && k < cap; ++k) {
+    pos[k] = p[k][0];
+    val[k] = p[k][1];
+  }
+  *count = p.size();
   return VL_OK;
 }
 vl_status or_crossing_frequency(const double* x, size_t n, double dt, double* out, int* found) {

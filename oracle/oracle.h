@@ -78,6 +78,7 @@ vl_status or_channel_coefficients(or_channel* c, double d, double* cd, double* c
 vl_status or_series_stats(const double* cd, const double* cl, size_t n, double dt,
                           double from, vl_stats* out);
 vl_status or_dominant_frequency(const double* x, size_t n, double dt, double* out2);
+vl_status or_peaks(const double* x, size_t n, double* pos, double* val, size_t cap, size_t* count);
 vl_status or_crossing_frequency(const double* x, size_t n, double dt, double* out, int* found);
 vl_status or_fft(double* re, double* im, size_t n);
 vl_status or_dft_naive(const double* re, const double* im, size_t n, double* ore, double* oim);

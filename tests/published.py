@@ -1,0 +1,36 @@
+"""Figures the reference publishes for its validation set, as printed
+(README.md:133-200, docs/report/index.html sections 1 and 3)."""
+
+# Poiseuille relative L2 error: (scheme, q, rows, published)
+POISEUILLE = [
+    ("bgk", 0.5, 8, 1.112e-2), ("bgk", 0.5, 16, 2.781e-3), ("bgk", 0.5, 32, 6.954e-4),
+    ("trt14", 0.5, 8, 7.131e-3), ("trt14", 0.5, 16, 1.783e-3),
+    ("trt", 0.25, 8, 3.398e-2), ("trt", 0.25, 16, 7.818e-3),
+    ("trt", 0.8, 8, 1.499e-2), ("trt", 0.8, 16, 4.098e-3),
+]
+
+# Schaefer-Turek 2D-1 (Re 20): (cells per D, lattice U, c_D, c_L, L_a in m, dP in Pa), as printed
+ST_2D1 = [
+    (10, 0.04, "5.6132", "0.01050", "0.0802", "0.1133"),
+    (20, 0.04, "5.5891", "0.01052", "0.0834", "0.1159"),
+    (40, 0.04, "5.5789", "0.01079", "0.0838", "0.1170"),
+    (80, 0.04, "5.5774", "0.01079", "0.0840", "0.1173"),
+    (20, 0.02, "5.5928", "0.01037", "0.0838", "0.1162"),
+]
+
+# Schaefer-Turek 2D-2 (Re 100): (cells per D, lattice U, c_D max, c_D min, c_L max, c_L min, St),
+# as printed
+ST_2D2 = [
+    (20, 0.05, "3.3244", "3.2719", "0.9807", "-1.0175", "0.2988"),
+    (40, 0.05, "3.2423", "3.1919", "0.9830", "-1.0181", "0.3006"),
+    (80, 0.05, "3.2354", "3.1840", "0.9947", "-1.0295", "0.3007"),
+    (20, 0.025, "3.3268", "3.2633", "0.9797", "-1.0167", "0.2995"),
+    (40, 0.025, "3.2431", "3.1790", "0.9792", "-1.0144", "0.3013"),
+]
+
+
+def printed(value: float, published: str) -> bool:
+    """`value` rounds to the printed figure `published` (a decimal string) at its number of
+    decimals."""
+    decimals = len(published.split(".")[1])
+    return f"{value:.{decimals}f}" == published or abs(value - float(published)) <= 0.5 * 10.0 ** (-decimals)

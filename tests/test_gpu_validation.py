@@ -1,0 +1,66 @@
+"""The reference's validation set on the MI355X FP64 path, against its printed figures.
+
+The case drivers (vortexlab_amd/cases.py) restate the reference's step loops, stopping rules
+and post-processing; run on the oracle they reproduce the printed rows (tests/
+test_cases_oracle.py).  Here they run on the GPU library: every row must print as published.
+Poiseuille: the published L2 errors (4 significant digits), TRT Lambda = 3/16 with halfway
+walls exact to 1e-11, second-order convergence, force balance to 1e-9 (tests/validation.rs:
+19-38).  Schaefer-Turek 2D-1 and 2D-2 (README.md:161-185): c_D, c_L, L_a, dP and c_D/c_L
+extrema with St, at 10-80 cells per diameter; 2D-2 at 10 cells diverges, as the reference
+reports.
+"""
+import pytest
+
+from vortexlab_amd import api, cases
+
+from published import POISEUILLE, ST_2D1, ST_2D2, printed
+
+pytestmark = pytest.mark.gpu
+
+MODEL = {"bgk": (api.BGK, 0.0), "trt": (api.TRT, 3.0 / 16.0), "trt14": (api.TRT, 0.25)}
+
+
+@pytest.mark.parametrize("scheme,q,rows,published", POISEUILLE)
+def test_poiseuille_published_l2(gpu, scheme, q, rows, published):
+    print(f"[poiseuille] {scheme} q={q} rows={rows}", flush=True)
+    o = cases.poiseuille_run(gpu, rows, 0.8, MODEL[scheme], q)
+    assert abs(o.l2 / published - 1.0) < 6e-4, f"l2 {o.l2:.4e} vs {published:.3e}"
+    assert abs(o.force_balance - 1.0) < 1e-9
+
+
+def test_poiseuille_trt_magic_exact_and_second_order_elsewhere(gpu):
+    print("[poiseuille] exactness and order", flush=True)
+    for rows in (8, 16, 32):
+        o = cases.poiseuille_run(gpu, rows, 0.8, MODEL["trt"], 0.5)
+        assert o.l2 < 1e-11, f"rows {rows}: {o.l2:.3e}"
+        assert abs(o.force_balance - 1.0) < 1e-9
+    for scheme, q in (("bgk", 0.5), ("trt14", 0.5), ("trt", 0.25), ("trt", 0.8)):
+        out = [cases.poiseuille_run(gpu, h, 0.8, MODEL[scheme], q) for h in (8, 16, 32)]
+        for order in cases.observed_orders(out):
+            assert abs(order - 2.0) < 0.05, f"{scheme} q={q}: order {order}"
+        assert out[2].l2 < 2.5e-3
+
+
+@pytest.mark.parametrize("n,u,cd,cl,la,dp", ST_2D1)
+def test_schaefer_turek_2d1(gpu, n, u, cd, cl, la, dp):
+    print(f"[2D-1] n={n} u={u}", flush=True)
+    o = cases.run_2d1(gpu, n, u)
+    assert o.converged
+    got = (o.cd, o.cl, o.la, o.dp)
+    assert all(printed(v, p) for v, p in zip(got, (cd, cl, la, dp))), f"{got} vs {(cd, cl, la, dp)}"
+
+
+@pytest.mark.parametrize("n,u,cd_max,cd_min,cl_max,cl_min,st", ST_2D2)
+def test_schaefer_turek_2d2(gpu, n, u, cd_max, cd_min, cl_max, cl_min, st):
+    print(f"[2D-2] n={n} u={u}", flush=True)
+    o = cases.run_2d2(gpu, n, u)
+    got = (o.cd_max, o.cd_min, o.cl_max, o.cl_min, o.st)
+    row = (cd_max, cd_min, cl_max, cl_min, st)
+    assert all(printed(v, p) for v, p in zip(got, row)), f"{got} vs {row}"
+
+
+def test_schaefer_turek_2d2_diverges_at_10_cells(gpu):
+    """README.md:173: at 10 cells per diameter (tau 0.515) the periodic case diverges."""
+    print("[2D-2] n=10 divergence", flush=True)
+    with pytest.raises(api.Diverged):
+        cases.run_2d2(gpu, 10, 0.05)

@@ -39,13 +39,7 @@ def test_hash_independent_of_thread_count():
     assert len(hashes) == 1
 
 
-# docs/report/index.html section 1: scheme, rows, published L2 (4 significant digits)
-POISEUILLE_TABLE = [
-    ("bgk", 0.5, 8, 1.112e-2), ("bgk", 0.5, 16, 2.781e-3), ("bgk", 0.5, 32, 6.954e-4),
-    ("trt14", 0.5, 8, 7.131e-3), ("trt14", 0.5, 16, 1.783e-3),
-    ("trt", 0.25, 8, 3.398e-2), ("trt", 0.25, 16, 7.818e-3),
-    ("trt", 0.8, 8, 1.499e-2), ("trt", 0.8, 16, 4.098e-3),
-]
+from published import POISEUILLE as POISEUILLE_TABLE  # noqa: E402
 MODEL = {"bgk": (api.BGK, 0.0), "trt": (api.TRT, 3.0 / 16.0), "trt14": (api.TRT, 0.25)}
 
 

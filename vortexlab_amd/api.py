@@ -141,6 +141,7 @@ _SIGS = [
     ("series_stats", C.c_int, [_D, _D, C.c_size_t, C.c_double, C.c_double, C.POINTER(StatsDesc)]),
     ("dominant_frequency", C.c_int, [_D, C.c_size_t, C.c_double, _D]),
     ("crossing_frequency", C.c_int, [_D, C.c_size_t, C.c_double, _D, C.POINTER(C.c_int)]),
+    ("peaks", C.c_int, [_D, C.c_size_t, _D, _D, C.c_size_t, C.POINTER(C.c_size_t)]),
     ("fft", C.c_int, [_D, _D, C.c_size_t]),
     ("batch_create", C.c_int, [C.c_int, C.POINTER(_VP)]),
     ("batch_add", C.c_int, [_VP, C.POINTER(ChannelSpecDesc), _VP, C.c_int64, C.c_double]),
@@ -693,6 +694,18 @@ def dominant_frequency(be: Backend, x, dt: float):
     out = np.zeros(2)
     be.check(be.dominant_frequency(_dptr(x), len(x), dt, _dptr(out)))
     return float(out[0]), float(out[1])
+
+
+def peaks(be: Backend, x) -> np.ndarray:
+    """signal::peaks: [k, 2] array of (fractional index, refined value) of the local maxima
+    above the mean."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    cap = max(len(x) // 2, 1)
+    pos, val = np.zeros(cap), np.zeros(cap)
+    n = C.c_size_t()
+    be.check(be.peaks(_dptr(x), len(x), _dptr(pos), _dptr(val), cap, C.byref(n)))
+    k = min(int(n.value), cap)
+    return np.stack([pos[:k], val[:k]], axis=1)
 
 
 def crossing_frequency(be: Backend, x, dt: float) -> Optional[float]:

@@ -1,0 +1,379 @@
+"""Case drivers of the reference's `cases` module, on the mirrored API.
+
+Each driver states a reference set-up once and runs it on any backend: the MI355X library
+(``arg`` = storage precision) or, in tests, the CPU oracle (``arg`` = thread count).  The
+step loops, stopping rules, divergence checks and post-processing follow the reference line
+by line, so a driver's figures can be compared with the reference's published tables.
+
+* Poiseuille flow (src/cases/poiseuille.rs:41-102)
+* Schaefer-Turek cylinder rig, 2D-1 / 2D-2 and the bench (src/cases/cylinder.rs:35-232)
+* Towing tank (src/cases/tunnel.rs:35-92)
+* Output rows in the reference's CSV formats (src/main.rs:197-208, 448-476)
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import api
+
+CS2 = 1.0 / 3.0
+TRT = (api.TRT, 3.0 / 16.0)
+
+
+def viscosity(tau: float) -> float:
+    """lattice.rs:48-50."""
+    return (tau - 0.5) * CS2
+
+
+def tau_for(nu: float) -> float:
+    """lattice.rs:53-55."""
+    return nu / CS2 + 0.5
+
+
+def _serial_sum(x) -> float:
+    """Left-to-right sum, as the reference's iterator sums."""
+    s = 0.0
+    for v in x:
+        s += float(v)
+    return s
+
+
+def _mean(x) -> float:
+    """signal.rs:61-63."""
+    return _serial_sum(x) / len(x)
+
+
+def sample_at(x, pos: float) -> float:
+    """signal.rs:143-147: linear interpolation at a fractional index."""
+    i = min(int(math.floor(pos)), len(x) - 2)
+    f = pos - i
+    return x[i] * (1.0 - f) + x[i + 1] * f
+
+
+# ------------------------------------------------------------------------------------------
+# Poiseuille flow (src/cases/poiseuille.rs)
+# ------------------------------------------------------------------------------------------
+@dataclass
+class PoiseuilleOutcome:
+    rows: int
+    q: float
+    l2: float
+    steps: int
+    force_balance: float
+    profile: list = field(default_factory=list)
+
+
+def poiseuille_run(be: api.Backend, rows: int, tau: float, collision=TRT, q: float = 0.5, arg: int = api.F64,
+                   max_steps: int = 4_000_000) -> PoiseuilleOutcome:
+    """poiseuille::run (poiseuille.rs:41-89): a periodic channel of `rows` fluid rows driven by
+    a body force, run until the profile stops changing; `q` places both walls q cells beyond
+    the outermost fluid rows (0.5: halfway bounce-back, otherwise interpolated)."""
+    nx, ny = 4, rows + 2
+    y0, y1 = 1.0 - q, rows + q
+    width = y1 - y0
+    nu = viscosity(tau)
+    umax = 0.1 * 16.0 / rows
+    g = 8.0 * nu * umax / (width * width)
+    b = api.Builder(be, nx, ny)
+    b.periodic_x = True
+    b.tau = tau
+    b.collision, b.magic = collision
+    b.accel = (g, 0.0)
+    shape = None if q == 0.5 else api.outside_band(y0, y1)
+    wall = b.add_wall(fgroup=0, shape=shape)
+    b.fill_rect(0, nx - 1, 0, 0, wall)
+    b.fill_rect(0, nx - 1, ny - 1, ny - 1, wall)
+    sim = b.build(arg)
+
+    def exact(y):
+        return g / (2.0 * nu) * (y - y0) * (y1 - y)
+
+    prev = [0.0] * ny
+    check = 200
+    t = 0
+    while True:
+        sim.step(check)
+        t += check
+        _, ux, _ = sim.macros_all()
+        change = 0.0
+        for y in range(1, rows + 1):
+            u = float(ux[y, 1])
+            change = max(change, abs(u - prev[y]))
+            prev[y] = u
+        if change < 1e-13 * umax or t > max_steps:
+            break
+    _, ux, _ = sim.macros_all()
+    num = den = 0.0
+    profile = []
+    for y in range(1, rows + 1):
+        u, e = float(ux[y, 1]), exact(y)
+        num += (u - e) * (u - e)
+        den += e * e
+        profile.append(((y - y0) / width, u / umax, e / umax))
+    driving = g * sim.total_mass()
+    balance = float(sim.force[0][0]) / driving
+    out = PoiseuilleOutcome(rows, q, math.sqrt(num / den), int(sim.info().t), balance, profile)
+    sim.close()
+    return out
+
+
+def observed_orders(outcomes) -> list:
+    """poiseuille::observed_orders (poiseuille.rs:93-102)."""
+    out = []
+    for a, b in zip(outcomes, outcomes[1:]):
+        h0 = a.rows - 1.0 + 2.0 * a.q
+        h1 = b.rows - 1.0 + 2.0 * b.q
+        out.append(math.log(a.l2 / b.l2) / math.log(h1 / h0))
+    return out
+
+
+# ------------------------------------------------------------------------------------------
+# Schaefer-Turek cylinder rig (src/cases/cylinder.rs)
+# ------------------------------------------------------------------------------------------
+def cylinder_rig_spec(n: int, u_mean: float = 0.05, re: float = 100.0, ramp_units: float = 8.0,
+                      collision=TRT) -> api.ChannelSpec:
+    """rig (cylinder.rs:35-50): channel 22n x 4.1n with a parabolic inflow and a circle of
+    diameter n at (2n + 0.5, 2n + 0.5)."""
+    if n < 10 or n % 10:
+        raise ValueError("cells per diameter must be a multiple of 10")
+    nn = float(n)
+    xc = yc = 0.5 + 2.0 * nn
+    return api.ChannelSpec(nx=22 * n + 2, ny=41 * n // 10 + 2, inflow=api.INFLOW_PARABOLIC, u=u_mean,
+                           nu=u_mean * nn / re, body=api.circle(xc, yc, 0.5 * nn), collision=collision[0],
+                           magic=collision[1], ramp_steps=int(ramp_units * nn / u_mean))
+
+
+def probe_density(rho: np.ndarray, solid: np.ndarray, x: float, y: float) -> float:
+    """Sim::probe_density (sim.rs:624-643): bilinear density over the non-solid corners of the
+    cell holding (x, y), weights renormalised."""
+    ny, nx = rho.shape
+    i, j = math.floor(x), math.floor(y)
+    fx, fy = x - i, y - j
+    s = ws = 0.0
+    for di, dj, w in ((0, 0, (1.0 - fx) * (1.0 - fy)), (1, 0, fx * (1.0 - fy)), (0, 1, (1.0 - fx) * fy),
+                      (1, 1, fx * fy)):
+        xi, yj = i + di, j + dj
+        if xi < 0 or yj < 0 or xi >= nx or yj >= ny:
+            continue
+        if not solid[yj, xi] and w > 0.0:
+            s += w * float(rho[yj, xi])
+            ws += w
+    return s / ws if ws > 0.0 else math.nan
+
+
+def probe_density_at_wall(rho, solid, x, y, nx, ny) -> float:
+    """Sim::probe_density_at_wall (sim.rs:647-651): quadratic extrapolation from 1, 2 and 3
+    lattice units along the outward normal."""
+    def p(d):
+        return probe_density(rho, solid, x + d * nx, y + d * ny)
+    return 3.0 * p(1.0) - 3.0 * p(2.0) + p(3.0)
+
+
+def _solid_mask(spec: api.ChannelSpec) -> np.ndarray:
+    """Cells of the rig's circle: centre strictly inside (Circle::inside, geometry.rs:22-27, as
+    Builder::fill_shape paints them).  The probes sit next to the body, far from the walls."""
+    b = spec.body
+    cx, cy, r = b.args
+    y, x = np.mgrid[0:spec.ny, 0:spec.nx]
+    return (x - cx) ** 2 + (y - cy) ** 2 < r * r
+
+
+@dataclass
+class Steady:
+    n: int
+    cd: float
+    cl: float
+    dp: float
+    la: float
+    residual: float
+    steps: int
+    converged: bool
+    seconds: float
+
+
+def _pressure_drop(rho, solid, n, xc, yc, u_ref):
+    """Rig::pressure_drop (cylinder.rs:57-62), in units of rho U^2."""
+    r = 0.5 * n
+    front = probe_density_at_wall(rho, solid, xc - r, yc, -1.0, 0.0)
+    back = probe_density_at_wall(rho, solid, xc + r, yc, 1.0, 0.0)
+    return CS2 * (front - back) / (u_ref * u_ref)
+
+
+def _recirculation_length(ux, n, xc, yc):
+    """Rig::recirculation_length (cylinder.rs:66-82), in diameters."""
+    j0 = int(math.floor(yc))
+    j1 = j0 + 1
+
+    def u(x):
+        return 0.5 * (float(ux[j0, x]) + float(ux[j1, x]))
+
+    rear = xc + 0.5 * n
+    start = int(math.ceil(rear)) + 1
+    if u(start) >= 0.0:
+        return 0.0
+    for x in range(start, ux.shape[1] - 2):
+        a, b = u(x), u(x + 1)
+        if a < 0.0 and b >= 0.0:
+            return (x + a / (a - b) - rear) / n
+    return math.nan
+
+
+def run_2d1(be: api.Backend, n: int, u_mean: float = 0.04, max_units: float = 300.0, tol: float = 1e-6,
+            arg: int = api.F64, collision=TRT) -> Steady:
+    """Case 2D-1, Re = 20 (cylinder.rs:100-143): windows of 5 convective units until the drag
+    varies by less than `tol` over a window and its window mean has stopped drifting."""
+    start = time.time()
+    spec = cylinder_rig_spec(n, u_mean, 20.0, 8.0, collision)
+    ch = api.Channel(be, spec, arg)
+    nn = float(n)
+    unit = int(round(nn / u_mean))
+    window = 5 * unit
+    prev_mean, residual, converged = math.nan, math.nan, False
+    cd = cl = 0.0
+    t = 0
+    while t < max_units * unit:
+        coef = ch.run(window, nn)
+        t += window
+        lo, hi = float(np.min(coef[:, 0])), float(np.max(coef[:, 0]))
+        cd = _serial_sum(coef[:, 0]) / window
+        cl = _serial_sum(coef[:, 1]) / window
+        residual = (hi - lo) / abs(cd)
+        if not math.isfinite(cd):
+            break
+        if residual < tol and abs((cd - prev_mean) / cd) < tol:
+            converged = True
+            break
+        prev_mean = cd
+    rho, ux, _ = ch.sim.macros_all()
+    solid = _solid_mask(spec)
+    xc = yc = 0.5 + 2.0 * nn
+    out = Steady(n, cd, cl, _pressure_drop(rho, solid, nn, xc, yc, u_mean) * 0.2 * 0.2,
+                 _recirculation_length(ux, nn, xc, yc) * 0.1, residual, t, converged, time.time() - start)
+    ch.close()
+    return out
+
+
+@dataclass
+class Unsteady:
+    n: int
+    cd_max: float
+    cd_min: float
+    cl_max: float
+    cl_min: float
+    st: float
+    st_crossings: float
+    peak_spread: float
+    periods: int
+    steps: int
+    seconds: float
+    cd: np.ndarray
+    cl: np.ndarray
+    dt: float
+    window_start: int
+
+
+def run_2d2(be: api.Backend, n: int, u_mean: float = 0.05, total_units: float = 110.0, window_units: float = 20.0,
+            arg: int = api.F64, collision=TRT) -> Unsteady:
+    """Case 2D-2, Re = 100 (cylinder.rs:169-222): `total_units` convective units, the last
+    `window_units` analysed.  Raises api.Diverged if the run blows up.  The pressure
+    difference needs a density probe every step and is not computed."""
+    start = time.time()
+    spec = cylinder_rig_spec(n, u_mean, 100.0, 8.0, collision)
+    ch = api.Channel(be, spec, arg)
+    nn = float(n)
+    unit = nn / u_mean
+    steps = int(total_units * unit)
+    coef = ch.run(steps, nn)  # raises api.Diverged on the first non-finite drag
+    ch.close()
+    dt = 1.0 / unit
+    i0 = steps - min(int(window_units * unit), steps)
+    cd, cl = coef[i0:, 0], coef[i0:, 1]
+    f, _ = api.dominant_frequency(be, cl, dt)
+    pk = api.peaks(be, cl)
+    tops = pk[:, 1] if len(pk) else np.zeros(0)
+    cl_max = max([-math.inf] + [float(v) for v in tops])
+    spread = (cl_max - min(float(v) for v in tops)) / _mean(tops) if len(tops) else math.nan
+    stc = api.crossing_frequency(be, cl, dt)
+    return Unsteady(n, float(np.max(cd)), float(np.min(cd)), cl_max, float(np.min(cl)), f,
+                    stc if stc is not None else math.nan, spread, len(tops), steps, time.time() - start,
+                    coef[:, 0].copy(), coef[:, 1].copy(), dt, i0)
+
+
+def cylinder_bench(be: api.Backend, n: int, steps: int, arg: int = api.F64):
+    """cylinder::bench (cylinder.rs:224-232): `steps` steps of the 2D-2 set-up at u = 0.05,
+    TRT; returns (channel, seconds).  The caller closes the channel."""
+    ch = api.Channel(be, cylinder_rig_spec(n, 0.05, 100.0, 8.0), arg)
+    t0 = time.perf_counter()
+    ch.run(steps, float(n))
+    return ch, time.perf_counter() - t0
+
+
+def bench_row(ch, steps: int, seconds: float, threads: int = 1) -> list:
+    """A row of the CLI's bench table (main.rs:452-474): threads, grid, fluid_nodes, steps,
+    seconds, mlups, state_hash."""
+    info = ch.sim.info()
+    nodes = int(info.fluid_nodes)
+    mlups = nodes * steps / seconds / 1e6
+    return [str(threads), f"{info.nx}x{info.ny}", str(nodes), str(steps), f"{seconds:.3f}", f"{mlups:.1f}",
+            f"{ch.sim.state_hash():016x}"]
+
+
+BENCH_HEADER = ["threads", "grid", "fluid_nodes", "steps", "seconds", "mlups", "state_hash"]
+
+
+def forces_csv(cd, cl, dt: float) -> str:
+    """forces.csv as the CLI writes it (main.rs:197-208): t, cd, cl, at most ~5000 rows."""
+    lines = ["t,cd,cl"]
+    stride = max(len(cd) // 5000, 1)
+    for i in range(0, len(cd), stride):
+        lines.append(f"{i * dt:.4f},{cd[i]:.6f},{cl[i]:.6f}")
+    return "\n".join(lines) + "\n"
+
+
+# ------------------------------------------------------------------------------------------
+# Towing tank (src/cases/tunnel.rs)
+# ------------------------------------------------------------------------------------------
+@dataclass
+class Tunnel:
+    """tunnel.rs:14-31: a body of width `d` cells in a tank `upstream + downstream` widths
+    long and `height` widths high, stream speed `u`, Reynolds number `re`.  The defaults are
+    the demo's tank (main.rs:245-256)."""
+    d: int = 30
+    re: float = 150.0
+    u: float = 0.08
+    upstream: float = 6.0
+    downstream: float = 16.0
+    height: float = 10.0
+    collision: tuple = TRT
+    open_sides: bool = True
+
+    def grid(self):
+        d = float(self.d)
+        return int(round((self.upstream + self.downstream) * d)) + 2, int(round(self.height * d)) + 2
+
+    def centre(self):
+        d = float(self.d)
+        return 0.5 + round(self.upstream * d), 0.5 + 0.5 * round(self.height * d)
+
+    def steps_per_unit(self) -> float:
+        return self.d / self.u
+
+    def spec(self, body: api.ShapeSpec) -> api.ChannelSpec:
+        """Tunnel::build (tunnel.rs:50-68): 5-unit inflow ramp, then a 2-unit twist of the body."""
+        nx, ny = self.grid()
+        cx, cy = self.centre()
+        unit = self.steps_per_unit()
+        return api.ChannelSpec(nx=nx, ny=ny, inflow=api.INFLOW_UNIFORM_OPEN if self.open_sides else api.INFLOW_UNIFORM,
+                               u=self.u, nu=self.u * self.d / self.re, body=body, collision=self.collision[0],
+                               magic=self.collision[1], ramp_steps=int(5.0 * unit),
+                               kick=api.Kick(cx, cy, 0.4 * self.u / self.d, int(5.0 * unit), int(2.0 * unit)))
+
+    def run(self, ch: api.Channel, units: float) -> np.ndarray:
+        """Tunnel::run (tunnel.rs:72-92): `units` convective units of (cd, cl) per step;
+        api.Diverged when the drag stops being finite."""
+        return ch.run(int(units * self.steps_per_unit()), float(self.d))

@@ -123,6 +123,31 @@ vl_status vl_dominant_frequency(const double* x, size_t n, double dt, double* ou
   return VL_OK;
 }
 
+// Local maxima above the mean, each refined by the parabola through it and its two
+// neighbours: (fractional index, refined value).  All `count` peaks are counted; the first
+// `cap` are written.
+vl_status vl_peaks(const double* x, size_t n, double* pos, double* val, size_t cap, size_t* count) {
+  if (!x || !count || (cap > 0 && (!pos || !val))) return vl::set_error(VL_EINVAL, "null argument");
+  *count = 0;
+  if (n < 3) return VL_OK;
+  const double m = avg(x, n);
+  size_t k = 0;
+  for (size_t i = 1; i + 1 < n; ++i) {
+    if (x[i] > m && x[i] > x[i - 1] && x[i] >= x[i + 1]) {
+      const double a = x[i - 1], b = x[i], c = x[i + 1];
+      const double denom = a - 2.0 * b + c;
+      const double shift = denom != 0.0 ? 0.5 * (a - c) / denom : 0.0;
+      if (k < cap) {
+        pos[k] = (double)i + shift;
+        val[k] = b - 0.25 * (a - c) * shift;
+      }
+      ++k;
+    }
+  }
+  *count = k;
+  return VL_OK;
+}
+
 vl_status vl_crossing_frequency(const double* x, size_t n, double dt, double* out, int* found) {
   if (!x || !out || !found) return vl::set_error(VL_EINVAL, "null argument");
   double f = 0.0;
