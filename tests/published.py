@@ -34,3 +34,27 @@ def printed(value: float, published: str) -> bool:
     decimals."""
     decimals = len(published.split(".")[1])
     return f"{value:.{decimals}f}" == published or abs(value - float(published)) <= 0.5 * 10.0 ** (-decimals)
+
+# Lid-driven cavity (README.md:147-155): (Re, n, max dev from Ghia u, v, u_min, v_max, v_min),
+# as printed
+CAVITY = [
+    (100, 65, "0.0048", "0.0079", "-0.21354", "0.17882", "-0.25245"),
+    (100, 129, "0.0050", "0.0084", "-0.21384", "0.17918", "-0.25301"),
+    (1000, 65, "0.0167", "0.0087", "-0.37995", "0.36766", "-0.51217"),
+    (1000, 129, "0.0045", "0.0115", "-0.38676", "0.37501", "-0.52401"),
+    (1000, 257, "0.0061", "0.0158", "-0.38798", "0.37638", "-0.52600"),
+]
+
+
+def ghia(re: int):
+    """Ghia's centreline tables from the committed fixture: (u rows, v rows) of (coordinate,
+    value), sorted by coordinate."""
+    import csv
+    import os
+    col = {100: 3, 1000: 4}[re]
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ghia1982.csv")
+    with open(path) as fh:
+        rows = list(csv.reader(line for line in fh if not line.startswith("#")))[1:]
+    u = sorted((float(r[2]), float(r[col])) for r in rows if r[0] == "u")
+    v = sorted((float(r[2]), float(r[col])) for r in rows if r[0] == "v")
+    return u, v

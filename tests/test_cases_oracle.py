@@ -64,3 +64,23 @@ def test_output_rows_in_reference_formats():
     text = cases.forces_csv(np.array([1.0, 2.0]), np.array([0.5, -0.5]), 0.25)
     assert text == "t,cd,cl\n0.0000,1.000000,0.500000\n0.2500,2.000000,-0.500000\n"
     assert cases.BENCH_HEADER == ["threads", "grid", "fluid_nodes", "steps", "seconds", "mlups", "state_hash"]
+
+
+def test_cavity_driver_on_the_oracle(orc):
+    """tests/validation.rs:41-52 at 33 x 33, Re 100: converged; deviations from Ghia below
+    0.006 / 0.008 (the reference measured 0.0039 / 0.0052); u_min within 1.5 % of the spectral
+    value."""
+    from published import ghia
+    o = cases.cavity_run(orc, 33, 100.0, 0.1, arg=8)
+    c = cases.cavity_compare(o, *ghia(100))
+    assert o.converged
+    assert printed(c.max_dev_u, "0.0039") and printed(c.max_dev_v, "0.0052")
+    assert abs(c.u_min[1] / -0.2140424 - 1.0) < 0.015
+
+
+def test_cavity_interpolation_exact_for_cubics():
+    """cavity.rs:103-113: cubic Lagrange interpolation reproduces a cubic."""
+    f = lambda x: 1.0 - 2.0 * x + 0.5 * x * x * x  # noqa: E731
+    prof = [(x, f(x)) for x in (0.0, 0.07, 0.2, 0.31, 0.5, 0.74, 0.9, 1.0)]
+    for s in (0.0, 0.01, 0.25, 0.5, 0.62, 0.95, 1.0):
+        assert abs(cases.interpolate(prof, s) - f(s)) < 1e-12

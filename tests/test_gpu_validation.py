@@ -7,13 +7,14 @@ Poiseuille: the published L2 errors (4 significant digits), TRT Lambda = 3/16 wi
 walls exact to 1e-11, second-order convergence, force balance to 1e-9 (tests/validation.rs:
 19-38).  Schaefer-Turek 2D-1 and 2D-2 (README.md:161-185): c_D, c_L, L_a, dP and c_D/c_L
 extrema with St, at 10-80 cells per diameter; 2D-2 at 10 cells diverges, as the reference
-reports.
+reports.  Lid-driven cavity (README.md:147-155): deviations from Ghia's tables (committed as a
+fixture) and the centreline extrema at Re 100 and 1000, 65^2 to 257^2.
 """
 import pytest
 
 from vortexlab_amd import api, cases
 
-from published import POISEUILLE, ST_2D1, ST_2D2, printed
+from published import CAVITY, POISEUILLE, ST_2D1, ST_2D2, ghia, printed
 
 pytestmark = pytest.mark.gpu
 
@@ -64,3 +65,15 @@ def test_schaefer_turek_2d2_diverges_at_10_cells(gpu):
     print("[2D-2] n=10 divergence", flush=True)
     with pytest.raises(api.Diverged):
         cases.run_2d2(gpu, 10, 0.05)
+
+
+@pytest.mark.parametrize("re,n,dev_u,dev_v,u_min,v_max,v_min", CAVITY)
+def test_lid_driven_cavity(gpu, re, n, dev_u, dev_v, u_min, v_max, v_min):
+    """README.md:147-155: deviations from Ghia's tables and the centreline extrema."""
+    print(f"[cavity] Re={re} n={n}", flush=True)
+    o = cases.cavity_run(gpu, n, float(re), 0.1)
+    assert o.converged
+    c = cases.cavity_compare(o, *ghia(re))
+    got = (c.max_dev_u, c.max_dev_v, c.u_min[1], c.v_max[1], c.v_min[1])
+    row = (dev_u, dev_v, u_min, v_max, v_min)
+    assert all(printed(v, p) for v, p in zip(got, row)), f"{got} vs {row}"

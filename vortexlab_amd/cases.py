@@ -6,6 +6,7 @@ step loops, stopping rules, divergence checks and post-processing follow the ref
 by line, so a driver's figures can be compared with the reference's published tables.
 
 * Poiseuille flow (src/cases/poiseuille.rs:41-102)
+* Lid-driven cavity against Ghia's tables (src/cases/cavity.rs, src/suite.rs:95-179)
 * Schaefer-Turek cylinder rig, 2D-1 / 2D-2 and the bench (src/cases/cylinder.rs:35-232)
 * Towing tank (src/cases/tunnel.rs:35-92)
 * Output rows in the reference's CSV formats (src/main.rs:197-208, 448-476)
@@ -130,6 +131,138 @@ def observed_orders(outcomes) -> list:
         h1 = b.rows - 1.0 + 2.0 * b.q
         out.append(math.log(a.l2 / b.l2) / math.log(h1 / h0))
     return out
+
+
+# ------------------------------------------------------------------------------------------
+# Lid-driven cavity (src/cases/cavity.rs, src/suite.rs:95-179)
+# ------------------------------------------------------------------------------------------
+@dataclass
+class CavityOutcome:
+    n: int
+    re: float
+    steps: int
+    converged: bool
+    seconds: float
+    u_vertical: list   # (y, u / U_lid) on the vertical centreline, walls included
+    v_horizontal: list  # (x, v / U_lid) on the horizontal centreline, walls included
+
+
+def cavity_run(be: api.Backend, n: int, re: float, u_lid: float = 0.1, collision=TRT, tol: float = 1e-9,
+               max_steps: int = 3_000_000, arg: int = api.F64) -> CavityOutcome:
+    """cavity::run (cavity.rs:23-79): n x n fluid cells, no-slip walls half a cell outside, the
+    top one sliding at `u_lid`; runs until the largest velocity change over 1000 steps is
+    below `tol` x u_lid."""
+    start = time.time()
+    b = api.Builder(be, n + 2, n + 2)
+    b.tau = tau_for(u_lid * n / re)
+    b.collision, b.magic = collision
+    wall = b.add_static_wall()
+    lid = b.add_wall(velocity=api.vel_const(u_lid, 0.0))
+    b.fill_rect(0, n + 1, 0, n + 1, wall)
+    b.fill_rect(1, n, n + 1, n + 1, lid)
+    b.fill_rect(1, n, 1, n, 0)
+    sim = b.build(arg)
+
+    def snapshot():
+        _, ux, uy = sim.macros_all()
+        return ux[1:n + 1, 1:n + 1].copy(), uy[1:n + 1, 1:n + 1].copy()
+
+    prev = snapshot()
+    converged = False
+    t = 0
+    while t < max_steps:
+        sim.step(1000)
+        t += 1000
+        now = snapshot()
+        change = float(max(np.max(np.abs(now[0] - prev[0])), np.max(np.abs(now[1] - prev[1]))))
+        prev = now
+        if math.isnan(change):
+            break
+        if change < tol * u_lid:
+            converged = True
+            break
+    _, ux, uy = sim.macros_all()
+    sim.close()
+    c0, c1 = ((n + 1) // 2, (n + 1) // 2) if n % 2 == 1 else (n // 2, n // 2 + 1)
+    u_vertical, v_horizontal = [(0.0, 0.0)], [(0.0, 0.0)]
+    for k in range(1, n + 1):
+        sk = (k - 0.5) / n
+        u_vertical.append((sk, 0.5 * (float(ux[k, c0]) + float(ux[k, c1])) / u_lid))
+        v_horizontal.append((sk, 0.5 * (float(uy[c0, k]) + float(uy[c1, k])) / u_lid))
+    u_vertical.append((1.0, 1.0))
+    v_horizontal.append((1.0, 0.0))
+    return CavityOutcome(n, re, t, converged, time.time() - start, u_vertical, v_horizontal)
+
+
+def interpolate(profile, s: float) -> float:
+    """cavity::interpolate (cavity.rs:83-99): cubic Lagrange through the four nearest samples."""
+    n = len(profile)
+    k = 0
+    while k < n and profile[k][0] < s:
+        k += 1
+    k = min(max(k, 2), n - 2)
+    pts = profile[k - 2:k + 2]
+    out = 0.0
+    for i in range(4):
+        w = 1.0
+        for j in range(4):
+            if i != j:
+                w *= (s - pts[j][0]) / (pts[i][0] - pts[j][0])
+        out += w * pts[i][1]
+    return out
+
+
+def refine_extremum(p, k: int):
+    """suite.rs:95-108: (position, value) of the parabola through samples k - 1, k, k + 1."""
+    if k == 0 or k + 1 >= len(p):
+        return p[k]
+    (x0, y0), (x1, y1), (x2, y2) = p[k - 1], p[k], p[k + 1]
+    d1 = (y1 - y0) / (x1 - x0)
+    d2 = (y2 - y1) / (x2 - x1)
+    c = (d2 - d1) / (x2 - x0)
+    if c == 0.0:
+        return p[k]
+    x = 0.5 * (x0 + x1) - 0.5 * d1 / c
+    return x, y0 + d1 * (x - x0) + c * (x - x0) * (x - x1)
+
+
+@dataclass
+class CavityComparison:
+    max_dev_u: float
+    max_dev_v: float
+    rms_dev: float
+    u_min: tuple  # (position, value)
+    v_max: tuple
+    v_min: tuple
+
+
+def cavity_compare(o: CavityOutcome, ghia_u, ghia_v) -> CavityComparison:
+    """suite::cavity_run's comparison (suite.rs:137-158): deviations at the interior points of
+    Ghia's tables (lists of (coordinate, value)) and the refined centreline extrema."""
+    max_u = max_v = sq = 0.0
+    count = 0
+    for table, profile in ((ghia_u, o.u_vertical), (ghia_v, o.v_horizontal)):
+        for sk, r in table:
+            if sk == 0.0 or sk == 1.0:
+                continue
+            d = interpolate(profile, sk) - r
+            if table is ghia_u:
+                max_u = max(max_u, abs(d))
+            else:
+                max_v = max(max_v, abs(d))
+            sq += d * d
+            count += 1
+
+    def arg(p, sign):
+        best = 0
+        for i in range(1, len(p)):
+            if sign * p[i][1] > sign * p[best][1]:
+                best = i
+        return best
+
+    return CavityComparison(max_u, max_v, math.sqrt(sq / count), refine_extremum(o.u_vertical, arg(o.u_vertical, -1.0)),
+                            refine_extremum(o.v_horizontal, arg(o.v_horizontal, 1.0)),
+                            refine_extremum(o.v_horizontal, arg(o.v_horizontal, -1.0)))
 
 
 # ------------------------------------------------------------------------------------------
