@@ -8,13 +8,14 @@ walls exact to 1e-11, second-order convergence, force balance to 1e-9 (tests/val
 19-38).  Schaefer-Turek 2D-1 and 2D-2 (README.md:161-185): c_D, c_L, L_a, dP and c_D/c_L
 extrema with St, at 10-80 cells per diameter; 2D-2 at 10 cells diverges, as the reference
 reports.  Lid-driven cavity (README.md:147-155): deviations from Ghia's tables (committed as a
-fixture) and the centreline extrema at Re 100 and 1000, 65^2 to 257^2.
+fixture) and the centreline extrema at Re 100 and 1000, 65^2 to 257^2.  Corner study
+(README.md:205-229): C_D, C_L' and St of the five sections in all four published sets.
 """
 import pytest
 
 from vortexlab_amd import api, cases
 
-from published import CAVITY, POISEUILLE, ST_2D1, ST_2D2, ghia, printed
+from published import CAVITY, CORNERS, POISEUILLE, ST_2D1, ST_2D2, ghia, printed
 
 pytestmark = pytest.mark.gpu
 
@@ -77,3 +78,12 @@ def test_lid_driven_cavity(gpu, re, n, dev_u, dev_v, u_min, v_max, v_min):
     got = (c.max_dev_u, c.max_dev_v, c.u_min[1], c.v_max[1], c.v_min[1])
     row = (dev_u, dev_v, u_min, v_max, v_min)
     assert all(printed(v, p) for v, p in zip(got, row)), f"{got} vs {row}"
+
+
+@pytest.mark.parametrize("tag,d,re,u,corner,cd,clr,st", CORNERS)
+def test_corner_study(gpu, tag, d, re, u, corner, cd, clr, st):
+    """README.md:205-229: mean drag, RMS lift and Strouhal number of the five sections."""
+    print(f"[corners] {tag} {corner} D={d} Re={re} u={u}", flush=True)
+    s = cases.corner_run(gpu, corner, d=d, re=re, u=u)
+    got = (s.cd_mean, s.cl_rms, s.st)
+    assert all(printed(v, p) for v, p in zip(got, (cd, clr, st))), f"{got} vs {(cd, clr, st)}"

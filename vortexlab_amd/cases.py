@@ -510,3 +510,33 @@ class Tunnel:
         """Tunnel::run (tunnel.rs:72-92): `units` convective units of (cd, cl) per step;
         api.Diverged when the drag stops being finite."""
         return ch.run(int(units * self.steps_per_unit()), float(self.d))
+
+    def experiment(self, ch: api.Channel, units: float, discard: float):
+        """Tunnel::experiment's run and statistics (tunnel.rs:152-169): `units` in 20 equal
+        parts, then Series::stats from time `discard` on.  Returns (stats, cd, cl)."""
+        unit = self.steps_per_unit()
+        parts = [ch.run(int(units / 20.0 * unit), float(self.d)) for _ in range(20)]
+        coef = np.concatenate(parts)
+        stats = api.series_stats(ch.be, coef[:, 0], coef[:, 1], 1.0 / unit, discard)
+        return stats, coef[:, 0], coef[:, 1]
+
+
+# ------------------------------------------------------------------------------------------
+# Corner study (src/suite.rs:246-290; the "corners" command, main.rs:389-420)
+# ------------------------------------------------------------------------------------------
+CORNERS = ("square", "chamfered", "rounded", "recessed", "double-recessed")
+
+
+def corner_run(be: api.Backend, corner: str, d: int = 60, re: float = 200.0, u: float = 0.1, b: float = 0.1,
+               units: float = 140.0, discard: Optional[float] = None, arg: int = api.F64):
+    """One section of the corner study: a square of side `d` with `corner` corners (size b D;
+    the double recess uses two steps of b/2) in a tank 10 + 18 D long and 20 D high with
+    radiating sides; returns Series::stats of the run after `discard` (default: half)."""
+    t = Tunnel(d=d, re=re, u=u, upstream=10.0, downstream=18.0, height=20.0, collision=TRT, open_sides=True)
+    cx, cy = t.centre()
+    size = 0.5 * b if corner == "double-recessed" else b
+    ch = api.Channel(be, t.spec(api.section(corner, size, cx, cy, float(d), 0.0)), arg)
+    stats, _, _ = t.experiment(ch, units, 0.5 * units if discard is None else discard)
+    ch.close()
+    return stats
+
