@@ -9,7 +9,7 @@ walls exact to 1e-11, second-order convergence, force balance to 1e-9 (tests/val
 extrema with St, at 10-80 cells per diameter; 2D-2 at 10 cells diverges, as the reference
 reports.  Lid-driven cavity (README.md:147-155): deviations from Ghia's tables (committed as a
 fixture) and the centreline extrema at Re 100 and 1000, 65^2 to 257^2.  Corner study
-(README.md:205-229): C_D, C_L' and St of the five sections in all four published sets.
+(docs/report/index.html section 4): C_D, C_L' and St of the five sections in all four sets.
 """
 import pytest
 
@@ -82,7 +82,8 @@ def test_lid_driven_cavity(gpu, re, n, dev_u, dev_v, u_min, v_max, v_min):
 
 @pytest.mark.parametrize("tag,d,re,u,corner,cd,clr,st", CORNERS)
 def test_corner_study(gpu, tag, d, re, u, corner, cd, clr, st):
-    """README.md:205-229: mean drag, RMS lift and Strouhal number of the five sections."""
+    """docs/report/index.html section 4: mean drag, RMS lift and Strouhal number of the five
+    sections."""
     print(f"[corners] {tag} {corner} D={d} Re={re} u={u}", flush=True)
     s = cases.corner_run(gpu, corner, d=d, re=re, u=u)
     got = (s.cd_mean, s.cl_rms, s.st)
