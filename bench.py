@@ -164,6 +164,25 @@ def cpu_baseline(api, wl, seconds_target, scale, steps_full, gpu_hash=None):
                      f"{spec.nx}x{spec.ny} domain from t=0, {threads} threads, {secs:.1f} s timed"}
     if match is not None:
         out["full_run_state_hash_matches_gpu"] = match
+    # one thread on the same domain (a few seconds), and the reference's own benchmark (cylinder
+    # rig n = 40, 3000 steps, one thread) against its published per-core rate and hash
+    ch = api.Channel(be, spec, 1)
+    ch.run(1, d)
+    t0 = time.perf_counter()
+    n1 = 0
+    while time.perf_counter() - t0 < 3.0 and n1 + 1 < steps_full:
+        ch.run(1, d)
+        n1 += 1
+    g1 = fluid * n1 / (time.perf_counter() - t0) / 1e9
+    ch.close()
+    h, fl, _, _, s40 = oracle.cylinder_bench(40, 3000, 1)
+    out["value_1thread"] = round(g1, 5)
+    out["bench_n40_mlups_1thread"] = round(fl * 3000 / s40 / 1e6, 1)
+    out["bench_n40_state_hash_is_published"] = h == 0x6B6814B976BBBAFF
+    out["reference_published_mlups_per_core"] = 130.0
+    log(f"[cpu] 1 thread: {g1 * 1e3:.1f} MLUPS on the same domain; cylinder bench n=40 (3000 steps, 1 thread): "
+        f"{out['bench_n40_mlups_1thread']} MLUPS (published 130 per Apple M5 core), hash published: "
+        f"{out['bench_n40_state_hash_is_published']}")
     return out
 
 
