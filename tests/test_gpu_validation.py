@@ -88,3 +88,16 @@ def test_corner_study(gpu, tag, d, re, u, corner, cd, clr, st):
     s = cases.corner_run(gpu, corner, d=d, re=re, u=u)
     got = (s.cd_mean, s.cl_rms, s.st)
     assert all(printed(v, p) for v, p in zip(got, (cd, clr, st))), f"{got} vs {(cd, clr, st)}"
+
+
+@pytest.mark.parametrize("prec,bound", [(api.F32, 5e-3), (api.F16, 2e-2)])
+def test_schaefer_turek_2d2_reduced_precision(gpu, prec, bound):
+    """The 2D-2 benchmark at 40 cells per diameter with FP32 / FP16 deviation storage: each
+    figure within `bound` (relative) of the FP64 run (FP32: the 0.5 % drag bound)."""
+    print(f"[2D-2] n=40 precision {prec}", flush=True)
+    ref = cases.run_2d2(gpu, 40, 0.05)
+    o = cases.run_2d2(gpu, 40, 0.05, arg=prec)
+    for name in ("cd_max", "cd_min", "cl_max", "cl_min", "st"):
+        a, b = getattr(o, name), getattr(ref, name)
+        print(f"  {name}: {a:.6f} vs FP64 {b:.6f} ({abs(a / b - 1):.2e})", flush=True)
+        assert abs(a / b - 1.0) < bound, name
