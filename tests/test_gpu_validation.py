@@ -103,3 +103,13 @@ def test_schaefer_turek_2d2_reduced_precision(gpu, prec, n, bound):
         a, b = getattr(o, name), getattr(ref, name)
         print(f"  {name}: {a:.6f} vs FP64 {b:.6f} ({abs(a / b - 1):.2e})", flush=True)
         assert abs(a / b - 1.0) < bound, name
+
+
+def test_cylinder_bench_row(gpu):
+    """cylinder::bench through the case driver, as the CLI's bench table row (main.rs:452-474):
+    882x166 grid, 143056 fluid nodes, the published state hash after 3000 steps."""
+    ch, secs = cases.cylinder_bench(gpu, 40, 3000)
+    row = cases.bench_row(ch, 3000, secs)
+    ch.close()
+    assert row[1:4] == ["882x166", "143056", "3000"] and row[6] == "6b6814b976bbbaff"
+    assert float(row[5]) > 0.0
