@@ -113,3 +113,18 @@ def test_cylinder_bench_row(gpu):
     ch.close()
     assert row[1:4] == ["882x166", "143056", "3000"] and row[6] == "6b6814b976bbbaff"
     assert float(row[5]) > 0.0
+
+
+def test_tunnel_demo(gpu):
+    """The CLI demo's vortex street (main.rs:245-262, printed in README.md:45-56): circle
+    D = 30 in the 662x302 tank, Re 150, u = 0.08, 90 units with 50 discarded: Cd 1.456,
+    Cl -0.009, Cl' 0.416, St 0.1954 from the spectrum and from zero crossings."""
+    t = cases.Tunnel()
+    cx, cy = t.centre()
+    ch = api.Channel(gpu, t.spec(api.circle(cx, cy, 15.0)), api.F64)
+    assert (ch.sim.info().nx, ch.sim.info().ny) == (662, 302)
+    s, _, _ = t.experiment(ch, 90.0, 50.0)
+    ch.close()
+    got = (s.cd_mean, s.cl_mean, s.cl_rms, s.st, s.st_crossings)
+    row = ("1.456", "-0.009", "0.416", "0.1954", "0.1954")
+    assert all(printed(v, p) for v, p in zip(got, row)), f"{got} vs {row}"
