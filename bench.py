@@ -293,6 +293,7 @@ def main():
             "cd_mean": s_main.cd_mean, "cd_mean_fp64": s_ref.cd_mean,
             "dcd_rel": abs(s_main.cd_mean / s_ref.cd_mean - 1.0),
             "dcl_rms_rel": abs(s_main.cl_rms / cl_ref - 1.0),
+            "cl_rms": s_main.cl_rms, "cl_rms_fp64": s_ref.cl_rms,
             "st": s_main.st, "st_fp64": s_ref.st,
             "fp64_bulk_frac_hbm": round(BYTES_PER_NODE[0] * (ref["fluid"] - ref["bnodes"]) / (ref["bulk_ms"] * 1e-3)
                                         / 1e9 / HBM_PEAK_GBS, 4),
