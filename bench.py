@@ -238,6 +238,25 @@ def batch_bench(vl, api, ncases, seed, sample):
     log(f"[batch] sequential single-case runs: {len(pick)} cases in {seq_secs:.2f} s = {seq_rate:.2f} cases/s; "
         f"model {coeff[0] * 1e6:.2f} us/step + {1e-9 / coeff[1]:.1f} GLUPS -> {seq_rate_all:.2f} cases/s over the batch; "
         f"max |dCd| vs FP64 {max(dcd) if dcd else float('nan'):.2e} over {len(dcd)} cases")
+    # The same cases driven the way the reference's case loop drives a Channel (tunnel.rs:76-85:
+    # one step, then the force coefficients, per call): 1000 steps each on 8 of the sampled cases,
+    # fitted and summed over the batch in the same way.
+    rows1, times1 = [], []
+    for k in pick[:8]:
+        c = cases[k]
+        ch = vl.make_channel(c.spec, api.F32)
+        n = min(1000, c.steps)
+        t0 = time.perf_counter()
+        for _ in range(n):
+            ch.run(1, c.d)
+        times1.append(time.perf_counter() - t0)
+        rows1.append((n, n * ch.sim.fluid_nodes))
+        ch.close()
+    coeff1 = np.linalg.lstsq(np.array(rows1, dtype=float), np.array(times1), rcond=None)[0]
+    seq1_total = sum(coeff1[0] * results[k][1] + coeff1[1] * results[k][1] * fluid[k] for k in range(ncases))
+    seq1_rate = ncases / seq1_total
+    log(f"[batch] driven one step per call: {coeff1[0] * 1e6:.2f} us/step -> {seq1_rate:.3f} cases/s over the batch; "
+        f"batched / that = {(ncases / t_run) / seq1_rate:.1f}")
     b.close()
     return {"batched_cases_per_s": round(ncases / t_run, 3), "batched_glups": round(info.lattice_updates / t_run / 1e9, 2),
             "batched_cases": ncases, "batched_diverged": diverged, "batched_seconds": round(t_run, 2),
@@ -246,6 +265,9 @@ def batch_bench(vl, api, ncases, seed, sample):
             "sequential_sample_cases_per_s": round(seq_rate, 3),
             "sequential_step_us": round(coeff[0] * 1e6, 2), "sequential_update_glups": round(1e-9 / coeff[1], 1),
             "batched_vs_sequential": round((ncases / t_run) / seq_rate_all, 2),
+            "sequential_one_step_per_call_cases_per_s": round(seq1_rate, 3),
+            "sequential_one_step_per_call_step_us": round(coeff1[0] * 1e6, 2),
+            "batched_vs_sequential_one_step_per_call": round((ncases / t_run) / seq1_rate, 2),
             "batched_max_dcd_vs_fp64": (max(dcd) if dcd else None), "batched_dcd_checked": len(dcd)}
 
 

@@ -235,8 +235,9 @@ typedef struct {
   int32_t boundary_tiles, bulk_tiles, tile_w, tile_h, grid_bulk, grid_boundary;
   double last_run_seconds;
   int64_t device_bytes;
-  double autotune_ms[3][4]; /* bulk step time per (tile shape 64x4 / 128x2 / 256x1,
-                               grid: one workgroup per tile / 4 / 8 / 16 persistent per CU) */
+  double autotune_ms[3][4]; /* time of a trial step 0 (boundary kernel beside the bulk) per
+                               (tile shape 64x4 / 128x2 / 256x1, bulk grid: one workgroup per
+                               tile / 4 / 8 / 16 persistent per CU) */
 } vl_batch_info;
 vl_status vl_batch_create(int precision, vl_batch** out);
 /* Adds a case (the channel of vl_channel_create; `body` consumed) that runs `steps` steps;

@@ -91,7 +91,7 @@ def run_phases(be, cases: list, precision: int, pieces: int):
     b.build()
     info = b.info()
     at = np.array(info.autotune_ms)
-    print(f"[phase] autotune bulk ms (rows 64x4/128x2/256x1; cols one-per-tile/4/8/16 per CU):\n{np.round(at, 3)}",
+    print(f"[phase] autotune trial-step ms (rows 64x4/128x2/256x1; cols one-per-tile/4/8/16 per CU):\n{np.round(at, 3)}",
           file=sys.stderr, flush=True)
     steps = np.array([c.steps for c in cases])
     fluid = np.array([b.case_fluid_nodes(k) for k in range(len(cases))])
