@@ -779,6 +779,10 @@ vl_status build(vl_batch* b) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (const char* e = std::getenv("VL_BATCH_TILE")) {  // force a shape: 0, 1, 2 = 64x4, 128x2, 256x1
+    const int si = std::atoi(e);
+    if (si >= 0 && si < 3) best_s = si;
+  }
   b->tw_log2 = shapes[best_s][0];
   b->th = shapes[best_s][1];
   bulk_tiles(b, order, 1 << b->tw_log2, b->th, kc, ky, kx, kend);
