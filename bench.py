@@ -108,8 +108,9 @@ def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
                glups=info.fluid_nodes * steps / secs / 1e9, bulk_ms=bulk_ms / max(bulk_n, 1),
                step_ms=total_ms / max(tsteps, 1), coef=np.concatenate([coef_w, coef]), d=d, dt=spec.u / d,
                split=bool(info.split), steps_total=int(info.t), nx=int(info.nx), ny=int(info.ny), spec=spec,
-               state_hash=state_hash)
-    kname = "bulk kernel" if res["split"] else "step kernel"
+               state_hash=state_hash, two_step=bool(info.two_step), tb_nodes=int(info.two_step_nodes),
+               tb_tasks=int(info.two_step_tasks), listed=int(info.listed_nodes), tb_rows=int(info.two_step_rows))
+    kname = "two-step kernel" if res["two_step"] else ("bulk kernel" if res["split"] else "step kernel")
     log(f"[bench] {PREC_NAME[precision]}: {steps} steps in {secs:.2f} s = {res['glups']:.2f} GLUPS, "
         f"{kname} {res['bulk_ms'] * 1e3:.1f} us, whole step {res['step_ms'] * 1e3:.1f} us")
     if keep:
@@ -117,6 +118,19 @@ def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
     else:
         ch.close()
     return res
+
+
+def dominant_kernel(r):
+    """(name, units, algorithmic bytes per launch) of the kernel that dominates a step.
+    k_two takes its covered interior nodes two steps per launch and moves each node's nine
+    populations in and out of HBM once per launch; k_bulk moves them once per step for every
+    interior node; fused, k_step does so for every fluid node."""
+    p, b = PREC_NAME[r["precision"]], BYTES_PER_NODE[r["precision"]]
+    if r["two_step"]:
+        return f"k_two<{p}>", r["tb_nodes"], b * r["tb_nodes"]
+    if r["split"]:
+        return f"k_bulk<{p}>", r["fluid"] - r["bnodes"], b * (r["fluid"] - r["bnodes"])
+    return f"k_step<{p}>", r["fluid"], b * r["fluid"]
 
 
 def coef_stats(vl, r):
@@ -317,19 +331,18 @@ def main():
             "dcl_rms_rel": abs(s_main.cl_rms / cl_ref - 1.0),
             "cl_rms": s_main.cl_rms, "cl_rms_fp64": s_ref.cl_rms,
             "st": s_main.st, "st_fp64": s_ref.st,
-            "fp64_bulk_frac_hbm": round(BYTES_PER_NODE[0] * (ref["fluid"] - ref["bnodes"]) / (ref["bulk_ms"] * 1e-3)
-                                        / 1e9 / HBM_PEAK_GBS, 4),
+            "fp64_kernel": dominant_kernel(ref)[0],
+            "fp64_kernel_frac_hbm": round(dominant_kernel(ref)[2] / (ref["bulk_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "fp64_glups_frac_hbm": round(ref["glups"] * BYTES_PER_NODE[0] / HBM_PEAK_GBS, 4),
         }
         if wl == "circle_16384x8192":
             extra["velocity_l2_error_vs_fp64"] = velocity_error(main_run["channel"], ref["channel"])
             main_run["channel"].close()
             ref["channel"].close()
-    # the dominant kernel: k_bulk (interior nodes) in split mode, k_step (every fluid node) fused
-    units = main_run["fluid"] - main_run["bnodes"] if main_run["split"] else main_run["fluid"]
-    traffic_alg = BYTES_PER_NODE[prec] * units
+    kname, units, traffic_alg = dominant_kernel(main_run)
     achieved = traffic_alg / (main_run["bulk_ms"] * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"bulk_traffic_{wl}_{PREC_NAME[prec]}.json")
+    tf = os.path.join(ROOT, "profiles", f"{kname[:kname.index('<')]}_traffic_{wl}_{PREC_NAME[prec]}.json")
     if os.path.exists(tf) and args.scale == 1:
         with open(tf) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
@@ -345,7 +358,6 @@ def main():
     cpu_seconds = args.cpu_seconds if args.cpu_seconds is not None else (90.0 if wl == "wake_662x302" else 15.0)
     cpu = (cpu_baseline(api, wl, cpu_seconds, args.scale, steps + warmup, gpu_hash)
            if cpu_seconds > 0 else None)
-    kname = f"k_bulk<{PREC_NAME[prec]}>" if main_run["split"] else f"k_step<{PREC_NAME[prec]}>"
     out = {
         "metric": METRIC, "value": round(main_run["glups"], 3), "unit": "GLUPS", "n_gpus": 1,
         "steps": steps, "warmup": warmup, "ms_per_step": round(main_run["secs"] / steps * 1e3, 4),
@@ -358,8 +370,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kname, "kernel_us": round(main_run["bulk_ms"] * 1e3, 2),
-                     "algorithmic_bytes_per_launch": traffic_alg,
+                     "algorithmic_bytes_per_launch": traffic_alg, "steps_per_launch": 2 if main_run["two_step"] else 1,
                      "copy_gbs": round(copy.value, 1), "frac_of_copy": round(achieved / copy.value, 4)},
+        # the metric's "%HBM peak in GLUPS": GLUPS x the single-step bytes per update over the
+        # peak; above 1 when a pass carries more than one step per HBM round trip
+        "glups_frac_hbm": round(main_run["glups"] * BYTES_PER_NODE[prec] / HBM_PEAK_GBS, 4),
+        "two_step": {"on": main_run["two_step"], "covered_nodes": main_run["tb_nodes"],
+                     "listed_nodes": main_run["listed"], "tasks": main_run["tb_tasks"],
+                     "rows_per_task": main_run["tb_rows"]},
         "cpu_baseline": cpu,
     }
     out.update(extra)

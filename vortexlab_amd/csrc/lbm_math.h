@@ -7,7 +7,9 @@
 //  * FP32 "deviation" operators on d_i = f_i - w_i, the form used by the FP32 and FP16
 //    storage paths.  Working on deviations keeps the O(1) rest weight out of every sum, so
 //    FP32 resolves the O(Ma) part of the populations to ~1e-7 of itself rather than ~1e-7
-//    of w_i.  These let the compiler contract to FMA (pragma below).
+//    of w_i.  These contract a*b+c to FMA within each expression only (`contract(on)`): the
+//    fusion is then fixed by the source text, so every kernel that inlines them (one step or
+//    two steps per pass, single domain or batch) gets the same bits.
 #pragma once
 
 #if defined(__HIPCC__)
@@ -146,7 +148,7 @@ struct Exact {
 // way as the reference's "rho - moving" rest equilibrium.
 struct Dev {
   __device__ static inline void moments(const float* d, float& drho, float& jx, float& jy) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     drho = ((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
     jx = (d[1] - d[3]) + (d[5] - d[6]) + (d[8] - d[7]);
     jy = (d[2] - d[4]) + (d[5] - d[8]) + (d[6] - d[7]);
@@ -154,7 +156,7 @@ struct Dev {
 
   template <int M>
   __device__ static inline void collide(float* d, float op, float om, float gx, float gy) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     float drho, jx, jy;
     moments(d, drho, jx, jy);
     const float rho = 1.0f + drho;

@@ -68,6 +68,23 @@ __global__ __launch_bounds__(BULK_T) void k_bulk(const typename S::T* __restrict
     bulk_vec<S, M, V>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, rows, g, c);
 }
 
+// Two steps of the covered interior in one pass: one task per wave, W waves per workgroup.
+template <class S, int M, int W>
+__global__ __launch_bounds__(W * TB_LANES) void k_two(const typename S::T* __restrict__ A,
+                                                     typename S::T* __restrict__ B, typename S::T* __restrict__ Cn,
+                                                     const int4* __restrict__ tasks, int ntasks, Geom g, Coll c) {
+  const int w = __builtin_amdgcn_readfirstlane((int)blockIdx.x * W + (int)(threadIdx.x / TB_LANES));
+  if (w >= ntasks) return;
+  two_step_strip<S, M>(tasks[w], A, B, Cn, g, c);
+}
+
+// One single step of the interior nodes that no two-step task covers.
+template <class S, int M>
+__global__ __launch_bounds__(256) void k_listed(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
+                                                const uint32_t* __restrict__ nodes, int n, Geom g, Coll c) {
+  listed_node<S, M>((int)blockIdx.x * 256 + (int)threadIdx.x, n, nodes, A, B, g, c);
+}
+
 // Advances the window counter at the end of a replayed window of steps.
 __global__ void k_advance(int64_t* kbase, int64_t by) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *kbase += by;
@@ -90,46 +107,74 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
 // ------------------------------------------------------------------------------------------
 // Host side
 // ------------------------------------------------------------------------------------------
+#define VL_TRY(x)                 \
+  do {                            \
+    vl_status st_ = (x);          \
+    if (st_ != VL_OK) return st_; \
+  } while (0)
+
 static const double SQRT_CS2 = std::sqrt(1.0 / 3.0);
 // Steps per run chunk: controls go up and forces come back once per chunk.
 static constexpr int64_t RUN_CHUNK = 4096;
 
-// Launch one step, step `k_local` of a window whose first chunk index is *kbase.
-// Fused mode: one kernel.  Split mode: the boundary kernel on the side stream and the bulk on
-// the main stream, forked after the previous step and joined before the next one.
+// Step arguments of the step at time `t`, step `k_local` of a window whose first chunk index is
+// *kbase.
+static StepArgs step_args(vl_sim* s, int64_t t, const int64_t* kbase, int k_local) {
+  const DevTables& dt = s->dt;
+  StepArgs sa;
+  sa.nlb = s->nlb;
+  sa.fblk = s->fblk;
+  sa.xblocks = (s->g.nx + STEP_T - 1) / STEP_T;
+  sa.sqrt_cs2 = SQRT_CS2;
+  const int64_t ns2 = (int64_t)std::max(dt.nslots, 1) * 2;
+  sa.fc_cur = dt.fcontrib + (t & 1) * ns2;
+  sa.fc_prev = dt.fcontrib + ((t + 1) & 1) * ns2;
+  sa.ctls = s->d_ctl;
+  sa.kbase = kbase;
+  sa.k_local = k_local;
+  sa.hist = dt.ngroups > 0 ? s->dhist : nullptr;
+  const int64_t np1 = std::max(dt.np, 1);
+  sa.pv_cur = dt.pvel + (t & 1) * np1;
+  sa.pv_next = dt.pvel + ((t + 1) & 1) * np1;
+  return sa;
+}
+
+// Split mode: the boundary kernel of one step on the side stream, forked from the main stream.
+// The caller joins it (ev_join) before anything that reads what it wrote.
+template <class S, int M>
+static vl_status fork_boundary(vl_sim* s, const typename S::T* A, typename S::T* B, const StepArgs& sa) {
+  VL_HIP(hipEventRecord(s->ev_fork, s->stream));
+  VL_HIP(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+  if (sa.nlb > 0)
+    hipLaunchKernelGGL((k_boundary<S, M>), dim3((unsigned)sa.nlb), dim3(STEP_T), 0, s->side, A, B, s->g, s->coll,
+                       s->dt, sa);
+  return VL_OK;
+}
+
+template <class S, int M>
+static void launch_listed(vl_sim* s, hipStream_t st, const typename S::T* A, typename S::T* B) {
+  if (s->nlisted > 0)
+    hipLaunchKernelGGL((k_listed<S, M>), dim3((unsigned)((s->nlisted + 255) / 256)), dim3(256), 0, st, A, B,
+                       s->dlisted, s->nlisted, s->g, s->coll);
+}
+
+// Launch one step.  Fused mode: one kernel.  Split mode: the boundary kernel on the side stream
+// and the bulk on the main stream, forked after the previous step and joined before the next.
 template <class S, int M>
 static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool timed_ok) {
   using T = typename S::T;
   const T* A = (const T*)s->buf[s->cur];
-  T* B = (T*)s->buf[s->cur ^ 1];
+  T* B = (T*)s->buf[(s->cur + 1) % s->nbuf];
   const Geom& g = s->g;
-  DevTables& t = s->dt;
-  StepArgs sa;
-  sa.nlb = s->nlb;
-  sa.fblk = s->fblk;
-  sa.xblocks = (g.nx + STEP_T - 1) / STEP_T;
-  sa.sqrt_cs2 = SQRT_CS2;
-  const int64_t ns2 = (int64_t)std::max(t.nslots, 1) * 2;
-  sa.fc_cur = t.fcontrib + (s->t & 1) * ns2;
-  sa.fc_prev = t.fcontrib + ((s->t + 1) & 1) * ns2;
-  sa.ctls = s->d_ctl;
-  sa.kbase = kbase;
-  sa.k_local = k_local;
-  sa.hist = t.ngroups > 0 ? s->dhist : nullptr;
-  const int64_t np1 = std::max(t.np, 1);
-  sa.pv_cur = t.pvel + (s->t & 1) * np1;
-  sa.pv_next = t.pvel + ((s->t + 1) & 1) * np1;
+  const StepArgs sa = step_args(s, s->t, kbase, k_local);
   const bool timed = timed_ok && s->timing && s->ev_used + 2 <= (int64_t)s->ev.size();
   if (!s->split) {
     if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
     hipLaunchKernelGGL((k_step<S, M>), dim3((unsigned)(sa.nlb + sa.xblocks * g.ny)), dim3(STEP_T), 0, s->stream, A,
-                       B, s->dflag, s->drows, g, s->coll, t, sa);
+                       B, s->dflag, s->drows, g, s->coll, s->dt, sa);
   } else {
     if (sa.nlb > 0) {
-      VL_HIP(hipEventRecord(s->ev_fork, s->stream));
-      VL_HIP(hipStreamWaitEvent(s->side, s->ev_fork, 0));
-      hipLaunchKernelGGL((k_boundary<S, M>), dim3((unsigned)sa.nlb), dim3(STEP_T), 0, s->side, A, B, g, s->coll, t,
-                         sa);
+      VL_TRY((fork_boundary<S, M>(s, A, B, sa)));
       VL_HIP(hipEventRecord(s->ev_join, s->side));
     }
     if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
@@ -166,27 +211,82 @@ static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool 
   }
   if (s->split && sa.nlb > 0) VL_HIP(hipStreamWaitEvent(s->stream, s->ev_join, 0));
   VL_HIP(hipGetLastError());
-  s->cur ^= 1;
+  s->cur = (s->cur + 1) % s->nbuf;
   s->t += 1;
   return VL_OK;
 }
 
+// Two steps, t -> t+2, with A = buf[cur], B = the next buffer (time t+1), C = the one after
+// (time t+2):
+//   phase 1: k_two takes the covered interior from A straight to C (and writes its ring nodes'
+//            t+1 values to B); beside it, on the side stream, the boundary kernel and the
+//            uncovered interior do step t, A -> B;
+//   phase 2: after both, the boundary kernel and the uncovered interior do step t+1, B -> C.
+// Every node of C is written once, by exactly one of these kernels.
+template <class S, int M>
+static vl_status launch_pair(vl_sim* s, const int64_t* kbase, int k_local, bool timed_ok) {
+  using T = typename S::T;
+  const T* A = (const T*)s->buf[s->cur];
+  T* B = (T*)s->buf[(s->cur + 1) % 3];
+  T* Cn = (T*)s->buf[(s->cur + 2) % 3];
+  const StepArgs s0 = step_args(s, s->t, kbase, k_local);
+  const StepArgs s1 = step_args(s, s->t + 1, kbase, k_local + 1);
+  const bool timed = timed_ok && s->timing && s->ev_used + 2 <= (int64_t)s->ev.size();
+  // phase 1
+  VL_TRY((fork_boundary<S, M>(s, A, B, s0)));
+  launch_listed<S, M>(s, s->side, A, B);
+  VL_HIP(hipEventRecord(s->ev_join, s->side));
+  if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
+  if (s->ntasks > 0) {
+    const unsigned grid = (unsigned)((s->ntasks + s->tb_waves - 1) / s->tb_waves);
+    if (s->tb_waves == 8)
+      hipLaunchKernelGGL((k_two<S, M, 8>), dim3(grid), dim3(8 * TB_LANES), 0, s->stream, A, B, Cn, s->dtasks,
+                         s->ntasks, s->g, s->coll);
+    else
+      hipLaunchKernelGGL((k_two<S, M, 4>), dim3(grid), dim3(4 * TB_LANES), 0, s->stream, A, B, Cn, s->dtasks,
+                         s->ntasks, s->g, s->coll);
+  }
+  if (timed) {
+    VL_HIP(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
+    s->ev_used += 2;
+  }
+  VL_HIP(hipStreamWaitEvent(s->stream, s->ev_join, 0));
+  // phase 2
+  VL_TRY((fork_boundary<S, M>(s, B, Cn, s1)));
+  VL_HIP(hipEventRecord(s->ev_join, s->side));
+  launch_listed<S, M>(s, s->stream, B, Cn);
+  VL_HIP(hipStreamWaitEvent(s->stream, s->ev_join, 0));
+  VL_HIP(hipGetLastError());
+  s->cur = (s->cur + 2) % 3;
+  s->t += 2;
+  return VL_OK;
+}
+
+template <class S, int M>
+static vl_status launch_steps(vl_sim* s, int n, const int64_t* kbase, int k_local, bool timed_ok) {
+  return n == 2 ? launch_pair<S, M>(s, kbase, k_local, timed_ok) : launch_step<S, M>(s, kbase, k_local, timed_ok);
+}
+
 template <class S>
-static vl_status step_dispatch(vl_sim* s, const int64_t* kbase, int k_local, bool timed_ok) {
+static vl_status step_dispatch(vl_sim* s, int n, const int64_t* kbase, int k_local, bool timed_ok) {
   switch (s->coll.model) {
-    case MODEL_FORCED: return launch_step<S, MODEL_FORCED>(s, kbase, k_local, timed_ok);
-    case MODEL_BGK: return launch_step<S, MODEL_BGK>(s, kbase, k_local, timed_ok);
-    default: return launch_step<S, MODEL_TRT>(s, kbase, k_local, timed_ok);
+    case MODEL_FORCED: return launch_steps<S, MODEL_FORCED>(s, n, kbase, k_local, timed_ok);
+    case MODEL_BGK: return launch_steps<S, MODEL_BGK>(s, n, kbase, k_local, timed_ok);
+    default: return launch_steps<S, MODEL_TRT>(s, n, kbase, k_local, timed_ok);
   }
 }
 
-static vl_status one_step(vl_sim* s, const int64_t* kbase, int k_local, bool timed_ok) {
+// Launches the next `n` steps (n = 2 only with two-step passes) as steps k_local.. of a window.
+static vl_status some_steps(vl_sim* s, int n, const int64_t* kbase, int k_local, bool timed_ok) {
   switch (s->precision) {
-    case VL_F32: return step_dispatch<StoreF32>(s, kbase, k_local, timed_ok);
-    case VL_F16: return step_dispatch<StoreF16>(s, kbase, k_local, timed_ok);
-    default: return step_dispatch<StoreF64>(s, kbase, k_local, timed_ok);
+    case VL_F32: return step_dispatch<StoreF32>(s, n, kbase, k_local, timed_ok);
+    case VL_F16: return step_dispatch<StoreF16>(s, n, kbase, k_local, timed_ok);
+    default: return step_dispatch<StoreF64>(s, n, kbase, k_local, timed_ok);
   }
 }
+
+// Steps advanced per launch group when at least two remain.
+static int group_steps(const vl_sim* s) { return s->two_step ? 2 : 1; }
 
 // Outward velocities of the outlet-controller nodes from the current A (start of a run).
 static vl_status prime_outlet(vl_sim* s) {
@@ -231,11 +331,6 @@ static vl_status upload(vl_sim* s, V*& dst, const std::vector<V>& src) {
   return VL_OK;
 }
 
-#define VL_TRY(x)                 \
-  do {                            \
-    vl_status st_ = (x);          \
-    if (st_ != VL_OK) return st_; \
-  } while (0)
 
 static vl_status ensure_hist(vl_sim* s, int64_t steps) {
   if (steps <= s->hist_cap) return VL_OK;
@@ -280,8 +375,8 @@ static vl_status read_raw(vl_sim* s, std::vector<char>& host) {
 
 static vl_status write_raw(vl_sim* s, const std::vector<char>& host) {
   VL_HIP(hipStreamSynchronize(s->stream));
-  VL_HIP(hipMemcpy(s->buf[0], host.data(), s->pop_bytes, hipMemcpyHostToDevice));
-  VL_HIP(hipMemcpy(s->buf[1], host.data(), s->pop_bytes, hipMemcpyHostToDevice));
+  for (int b = 0; b < s->nbuf; ++b) VL_HIP(hipMemcpy(s->buf[b], host.data(), s->pop_bytes, hipMemcpyHostToDevice));
+  s->cur = 0;
   return VL_OK;
 }
 
@@ -291,8 +386,9 @@ static void free_sim(vl_sim* s) {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
-  f(s->buf[0]);
-  f(s->buf[1]);
+  for (void* b : s->buf) f(b);
+  f(s->dtasks);
+  f(s->dlisted);
   f(s->dflag);
   f(s->drows);
   DevTables& t = s->dt;
@@ -319,6 +415,107 @@ static void free_sim(vl_sim* s) {
 }
 
 static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// Two-step task list, from the node flags (0 interior, 1 boundary, 2 solid):
+//  * strips of up to TB_OUT output columns start at x = 2, 62, 122, ...;
+//  * a row is usable for a strip when every node of the strip's columns plus one on each side
+//    is interior and its pulls stay inside the domain without wrapping (rows 1 .. ny-2,
+//    columns 1 .. nx-2);
+//  * a run of usable rows [ya, yb] gives output rows ya+1 .. yb-1 (the t+1 margin), cut into
+//    tasks of at most `rows` rows of near-equal size.
+// Tasks are ordered by (first row, first column) so a workgroup's waves sweep adjacent strips.
+// Every interior node no task covers goes to the single-step list; a covered node is a ring node
+// when one of its eight neighbours is a fluid node that no task covers, and a task marks each of
+// its four edges that holds a ring node.
+static vl_status build_two_step(vl_sim* s, int rows) {
+  const int nx = s->g.nx, ny = s->g.ny;
+  const std::vector<uint8_t>& fl = s->tab.flag;
+  if (nx >= 65536 || ny >= 65536 || (int64_t)s->g.pitch * ny >= ((int64_t)1 << 31) || rows < 1)
+    return set_error(VL_EINVAL, "two-step passes need nx, ny < 65536, a plane below 2^31 nodes and rows >= 1");
+  std::vector<int4> tasks;
+  std::vector<uint8_t> covered((size_t)nx * ny, 0);
+  std::vector<uint8_t> usable((size_t)ny);
+  int xs = s->tb_x0;
+  for (int x0 = xs; x0 < nx; x0 += TB_OUT) {
+    const int w = std::min(TB_OUT, nx - 2 - x0);
+    if (w < 1) break;
+    for (int y = 0; y < ny; ++y) {
+      bool ok = y >= 1 && y <= ny - 2;
+      for (int x = x0 - 1; ok && x <= x0 + w; ++x) ok = fl[(size_t)y * nx + x] == 0;
+      usable[(size_t)y] = ok;
+    }
+    for (int y = 0; y < ny;) {
+      if (!usable[(size_t)y]) {
+        ++y;
+        continue;
+      }
+      int yb = y;
+      while (yb + 1 < ny && usable[(size_t)yb + 1]) ++yb;
+      const int first = y + 1, len = yb - y - 1;
+      if (len >= 1) {
+        const int nseg = (len + rows - 1) / rows;
+        for (int k = 0; k < nseg; ++k) {
+          const int a = first + (int)((int64_t)len * k / nseg), b = first + (int)((int64_t)len * (k + 1) / nseg);
+          tasks.push_back(make_int4(x0, a, b - a, w));
+          for (int yy = a; yy < b; ++yy) std::memset(&covered[(size_t)yy * nx + x0], 1, (size_t)w);
+        }
+      }
+      y = yb + 1;
+    }
+  }
+  std::sort(tasks.begin(), tasks.end(),
+            [](const int4& a, const int4& b) { return a.y != b.y ? a.y < b.y : a.x < b.x; });
+  std::vector<uint32_t> listed;
+  std::vector<uint8_t> ring((size_t)nx * ny, 0);
+  int64_t ncov = 0;
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) {
+      const size_t k = (size_t)y * nx + x;
+      if (fl[k] != 0) continue;
+      if (!covered[k]) {
+        listed.push_back((uint32_t)y << 16 | (uint32_t)x);
+        continue;
+      }
+      ++ncov;
+      bool r = false;  // covered nodes lie at least one node inside the domain edges
+      for (int dy = -1; dy <= 1 && !r; ++dy)
+        for (int dx = -1; dx <= 1 && !r; ++dx) {
+          const size_t kn = (size_t)(y + dy) * nx + (x + dx);
+          r = fl[kn] != 2 && !covered[kn];
+        }
+      ring[k] = r;
+    }
+  for (int4& tk : tasks) {
+    const int xa = tk.x, xb = tk.x + tk.w - 1, ya = tk.y, yb = tk.y + tk.z - 1;
+    int edges = 0;
+    for (int yy = ya; yy <= yb; ++yy)
+      for (int xx = xa; xx <= xb; ++xx) {
+        if (!ring[(size_t)yy * nx + xx]) continue;
+        if (yy != ya && yy != yb && xx != xa && xx != xb)
+          return set_error(VL_EINVAL, "internal: a ring node inside a two-step task");
+        edges |= (yy == ya ? 1 : 0) | (yy == yb ? 2 : 0) | (xx == xa ? 4 : 0) | (xx == xb ? 8 : 0);
+      }
+    tk.w |= edges << 8;
+  }
+  // Lockstep bit: set on the tasks of a workgroup when they cover the same rows, so that its
+  // waves execute the same number of barriers.
+  if (s->tb_sync) {
+    const size_t per = (size_t)s->tb_waves;
+    for (size_t b = 0; b + per <= tasks.size(); b += per) {
+      bool same = true;
+      for (size_t i = 1; i < per; ++i) same = same && tasks[b + i].y == tasks[b].y && tasks[b + i].z == tasks[b].z;
+      if (same)
+        for (size_t i = 0; i < per; ++i) tasks[b + i].w |= 1 << 12;
+    }
+  }
+  s->tb_rows = rows;
+  s->ntasks = (int)tasks.size();
+  s->nlisted = (int)listed.size();
+  s->tb_nodes = ncov;
+  VL_TRY(upload(s, s->dtasks, tasks));
+  VL_TRY(upload(s, s->dlisted, listed));
+  return VL_OK;
+}
 
 static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   if (precision != VL_F64 && precision != VL_F32 && precision != VL_F16)
@@ -423,6 +620,28 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
     if (v == 0 || v == 1 || v == 2 || (v == 4 && precision != VL_F64) || (v == 8 && precision == VL_F16))
       s->bulk_v = v;
   }
+  // Two-step passes on split-mode domains (VL_TWO_STEP=0|1 overrides; 1 also forces split mode);
+  // VL_TB_ROWS sets the output rows per task.
+  s->two_step = s->split;
+  if (const char* e = std::getenv("VL_TWO_STEP")) {
+    s->two_step = e[0] == '1';
+    s->split = s->split || s->two_step;
+  }
+  if (s->two_step) {
+    // Tuning (measured on MI355X, 8192 x 4096 FP32): 12 output rows per task, 8 waves per
+    // workgroup sweeping in step, strips starting at x = 32 so that each workgroup's 480 output
+    // columns cover whole 128-byte lines.  VL_TB_ROWS, VL_TB_WAVES (4 | 8), VL_TB_SYNC (0 | 1)
+    // and VL_TB_X0 (>= 2) override.
+    int rows = 12;
+    if (const char* e = std::getenv("VL_TB_ROWS")) rows = std::atoi(e);
+    if (const char* e = std::getenv("VL_TB_WAVES")) s->tb_waves = std::atoi(e) == 4 ? 4 : 8;
+    if (const char* e = std::getenv("VL_TB_SYNC")) s->tb_sync = e[0] != '0';
+    if (const char* e = std::getenv("VL_TB_X0")) s->tb_x0 = std::max(2, std::atoi(e));
+    VL_TRY(build_two_step(s, rows));
+    s->nbuf = 3;
+    VL_HIP(hipMalloc(&s->buf[2], s->pop_bytes));
+    s->device_bytes += (int64_t)s->pop_bytes;
+  }
   if (s->split) {
     VL_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
     VL_HIP(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
@@ -489,7 +708,7 @@ static vl_status collect_times(vl_sim* s, int64_t steps) {
 static constexpr int GRAPH_K = 32;
 
 static vl_status get_window_graph(vl_sim* s, hipGraphExec_t* out) {
-  const int key = (s->cur & 1) * 2 + (int)(s->t & 1);
+  const int key = s->cur * 2 + (int)(s->t & 1);
   if (s->graph[key]) {
     *out = s->graph[key];
     return VL_OK;
@@ -499,9 +718,11 @@ static vl_status get_window_graph(vl_sim* s, hipGraphExec_t* out) {
   hipGraph_t gph = nullptr;
   VL_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   vl_status st = VL_OK;
-  for (int k = 0; k < GRAPH_K && st == VL_OK; ++k) st = one_step(s, s->d_kbase, k, false);
+  const int gs = group_steps(s);
+  for (int k = 0; k < GRAPH_K && st == VL_OK; k += gs) st = some_steps(s, gs, s->d_kbase, k, false);
   if (st == VL_OK) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s->stream, s->d_kbase, (int64_t)GRAPH_K);
   hipError_t e = hipStreamEndCapture(s->stream, &gph);
+  s->graph_cur[key] = s->cur;
   s->cur = cur0;  // capture recorded the steps without running them
   s->t = t0;
   if (st != VL_OK) {
@@ -555,11 +776,15 @@ static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double*
         hipGraphExec_t ge;
         VL_TRY(get_window_graph(s, &ge));
         VL_HIP(hipGraphLaunch(ge, s->stream));
-        s->cur ^= (GRAPH_K & 1);
+        s->cur = s->graph_cur[s->cur * 2 + (int)(s->t & 1)];
         s->t += GRAPH_K;
       }
     }
-    for (; k < m; ++k) VL_TRY(one_step(s, s->d_kzero, (int)k, true));  // remainder, launched directly
+    while (k < m) {  // remainder, launched directly
+      const int n = (int)std::min<int64_t>(group_steps(s), m - k);
+      VL_TRY(some_steps(s, n, s->d_kzero, (int)k, true));
+      k += n;
+    }
     if (ng > 0) VL_TRY(finish_forces(s, s->dhist + (m - 1) * ng * 2));
     if (s->timing) VL_HIP(hipEventRecord(s->ev_chunk[1], s->stream));
     if (ng > 0) {
@@ -669,6 +894,11 @@ vl_status vl_sim_info_get(vl_sim* s, vl_sim_info* o) {
   o->outlet_density_offset = s->outlet_density_offset;
   o->device_bytes = s->device_bytes;
   o->split = s->split ? 1 : 0;
+  o->two_step = s->two_step ? 1 : 0;
+  o->two_step_nodes = s->tb_nodes;
+  o->two_step_tasks = s->ntasks;
+  o->listed_nodes = s->nlisted;
+  o->two_step_rows = s->tb_rows;
   return VL_OK;
 }
 

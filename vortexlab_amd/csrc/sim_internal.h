@@ -79,8 +79,21 @@ struct vl_sim {
   hipStream_t side = nullptr;        // stream of the boundary kernel in split mode
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipStream_t stream = nullptr;
-  void* buf[2] = {nullptr, nullptr};  // population arrays; buf[cur] is A
+  void* buf[3] = {nullptr, nullptr, nullptr};  // population arrays; buf[cur] is A
+  int nbuf = 2;                                 // 3 with two-step passes (A -> B -> C)
   int cur = 0;
+  // Two-step passes (temporal blocking of the bulk): a step pair runs one kernel that takes the
+  // covered interior nodes from t to t+2 beside two single steps of every other fluid node.
+  bool two_step = false;
+  int tb_rows = 0;                   // output rows per task (upper bound)
+  int tb_waves = 8;                  // waves (tasks) per workgroup of the two-step kernel
+  bool tb_sync = true;               // a workgroup's waves sweep in step where their rows agree
+  int tb_x0 = 32;                    // first output column of the first strip
+  int ntasks = 0;                    // one task per wave
+  int nlisted = 0;                   // interior nodes no task covers
+  int64_t tb_nodes = 0;              // interior nodes the tasks cover
+  int4* dtasks = nullptr;
+  uint32_t* dlisted = nullptr;       // y << 16 | x of the uncovered interior nodes
   size_t elem = 8;
   size_t pop_bytes = 0;  // bytes of one population array
   uint8_t* dflag = nullptr;
@@ -92,7 +105,8 @@ struct vl_sim {
   int64_t* d_kbase = nullptr;    // window counter for replayed step graphs
   int64_t* d_kzero = nullptr;    // a device zero (directly launched steps)
   bool use_graphs = true;
-  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // by (buffer, step) parity
+  hipGraphExec_t graph[6] = {};  // by (current buffer, step parity)
+  int graph_cur[6] = {};           // current buffer after the window
   int64_t hist_cap = 0;
   std::vector<double> last_force;  // Sim::force after the last step
   int64_t device_bytes = 0;

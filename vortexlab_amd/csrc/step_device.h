@@ -533,4 +533,187 @@ __device__ inline void bulk_lds(int tile, int xblocks, const typename S::T* __re
   for (int i = 0; i < Q; ++i) S::st_stream(B + i * P + r0 + x, f[i]);
 }
 
+// ------------------------------------------------------------------------------------------
+// Two time steps in one pass (temporal blocking of the bulk)
+// ------------------------------------------------------------------------------------------
+// One wave sweeps a strip of 64 columns upward through a band of rows.  Lane l owns column
+// x0 - 2 + l.  At every row it loads that row's nine populations from A (time t), computes
+// time t+1 one row below, and time t+2 two rows below; the three rows each level needs stay in
+// registers, and the x - 1 / x + 1 neighbours come from the adjacent lanes.  Lanes 1..62 hold
+// valid t+1 values, lanes 2..61 valid t+2 values, so a wave writes up to 60 columns.  Every
+// node the wave computes at t+1 (its output plus a one-node margin) is interior, so no link
+// rule applies and the arithmetic per node is the single-step bulk's: t+1 values are rounded
+// through the storage type exactly as a store and reload would, and the result is bit-identical
+// to two single steps.  Nodes whose t+1 value a single-step neighbour will pull (the task's
+// "ring") are also written to B.
+constexpr int TB_LANES = 64;
+constexpr int TB_OUT = TB_LANES - 4;
+constexpr int TB_DEPTH = 3;  // rows of loads in flight per wave
+
+// Task of one wave: x = first output column, y = first output row, z = output rows,
+// w = output columns (<= TB_OUT) | edge bits << 8.  Ring nodes (covered nodes whose t+1 value
+// a single-step neighbour pulls) lie on the task's edges, since every other output node has all
+// eight neighbours inside the task; edge bit 1 / 2 / 4 / 8 = the first row / last row / first
+// column / last column holds some, and the whole edge then writes its t+1 values to B (values a
+// single step would also produce, so the extra ones are harmless).
+template <class C>
+__device__ inline C from_left(C v) {  // lane l receives lane l - 1's value
+  return __shfl_up(v, 1, TB_LANES);
+}
+template <class C>
+__device__ inline C from_right(C v) {  // lane l receives lane l + 1's value
+  return __shfl_down(v, 1, TB_LANES);
+}
+
+// A copy of v in a register of its own.  The unshifted populations of a row are carried to the
+// next row; without an explicit copy the compiler keeps them in the load's register and moves
+// the reload of that buffer elsewhere instead, which makes it wait for loads still in flight.
+__device__ inline float fresh(float v) {
+  float r;
+  asm volatile("v_mov_b32 %0, %1" : "=&v"(r) : "v"(v));
+  return r;
+}
+__device__ inline double fresh(double v) {
+  double r;
+  asm volatile("v_mov_b64 %0, %1" : "=&v"(r) : "v"(v));
+  return r;
+}
+
+// Element `e` of plane i of a population array, from a wave-uniform base (the same for every
+// lane).  The plane's base goes through a scalar move the optimiser cannot look into, so the
+// address stays (scalar base) + (32-bit lane offset) instead of nine 64-bit vector addresses.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;  // global memory
+template <class T>
+__device__ inline gptr<T> plane_at(T* base, int64_t plane, int i, int e) {
+  T* pb = base + i * plane;
+  asm volatile("s_mov_b64 %0, %1" : "=s"(pb) : "s"(pb));
+  return (gptr<T>)((__attribute__((address_space(1))) char*)pb + (uint32_t)e * (uint32_t)sizeof(T));
+}
+
+template <class S, int M>
+__device__ inline void two_step_strip(const int4 task, const typename S::T* __restrict__ A,
+                                      typename S::T* __restrict__ B, typename S::T* __restrict__ Cn,
+                                      const Geom& g, const Coll& c) {
+  using T = typename S::T;
+  using C = typename S::C;
+  const int lane = (int)(threadIdx.x & (TB_LANES - 1));
+  const int x0 = task.x, y0 = task.y, rows = task.z, w = task.w & 0xff;
+  const int edges = (task.w >> 8) & 15;
+  // the workgroup's four tasks cover the same rows: its waves sweep in step, one barrier a row
+  const bool lockstep = (task.w >> 12) & 1;
+  const int xr = x0 - 2 + lane;                    // this lane's column
+  const int xl = min(max(xr, 0), g.nx - 1);        // clamped for loads (lanes past the strip)
+  const bool out_lane = lane >= 2 && lane < 2 + w;
+  const int64_t P = g.plane;
+  const int pitch = g.pitch;
+  // Every lane reads the same row: the address of (xl, row) in plane i is the wave-uniform
+  // A + i*P + row*pitch + (x0 - 2), plus the lane's constant offset xl - (x0 - 2).
+  const int lane_off = xl - (x0 - 2);
+  const int64_t row0 = (int64_t)(y0 - 2) * pitch + (x0 - 2);  // uniform part of row y0 - 2
+  const int iters = rows + 4;  // rows y0 - 2 .. y0 + rows + 1 of time t
+  // TB_DEPTH rows are loaded ahead: row k sits in buf[k % TB_DEPTH], and consuming it frees its
+  // buffer for row k + TB_DEPTH (the rows past the last one reload the last one, so every load
+  // is in bounds and unconditional).
+  T buf[TB_DEPTH][Q];
+#pragma unroll
+  for (int b = 0; b < TB_DEPTH; ++b)
+#pragma unroll
+    for (int i = 0; i < Q; ++i) buf[b][i] = *plane_at(A + row0 + (int64_t)min(b, iters - 1) * pitch, P, i, lane_off);
+  // level 0 (time t): row r-1's f0, f1, f3 and f2, f5, f6; row r-2's f2, f5, f6
+  C p0 = C(0), p1 = C(0), p3 = C(0), qa2 = C(0), qa5 = C(0), qa6 = C(0), qb2 = C(0), qb5 = C(0), qb6 = C(0);
+  // level 1 (time t+1): the same for the rows of t+1 values
+  C gp0 = C(0), gp1 = C(0), gp3 = C(0), ga2 = C(0), ga5 = C(0), ga6 = C(0), gb2 = C(0), gb5 = C(0), gb6 = C(0);
+  // lanes of the first / last output column that write their t+1 values to B on every row
+  const bool ring_col = out_lane && (((edges & 4) && lane == 2) || ((edges & 8) && lane == 1 + w));
+  // Iteration k consumes row r = y0 - 2 + k: first the row that arrived, then the stores of the
+  // previous iteration's results and the load of row k + 2, then the compute.  The memory
+  // counter that the wait for a load uses also counts stores, and the compiler waits for all
+  // of them whenever the accesses in flight differ between paths; so the loads are
+  // unconditional and follow the stores, and each wait covers accesses issued at least one
+  // whole iteration earlier.  The loop runs in groups of TB_DEPTH iterations; the iterations
+  // past the last one store the last row (the first of them) and otherwise keep nothing.
+  auto iteration = [&](const int k, T (&cur)[Q]) __attribute__((always_inline)) {
+    if (lockstep) __syncthreads();
+    const C f0 = fresh(S::cv(cur[0])), f2 = fresh(S::cv(cur[2])), f4 = fresh(S::cv(cur[4]));
+    const C f1 = from_left(S::cv(cur[1])), f5 = from_left(S::cv(cur[5])), f8 = from_left(S::cv(cur[8]));
+    const C f3 = from_right(S::cv(cur[3])), f6 = from_right(S::cv(cur[6])), f7 = from_right(S::cv(cur[7]));
+    __builtin_amdgcn_sched_barrier(0);
+    {
+      const T* rowp = A + row0 + (int64_t)min(k + TB_DEPTH, iters - 1) * pitch;
+#pragma unroll
+      for (int i = 0; i < Q; ++i) cur[i] = *plane_at(rowp, P, i, lane_off);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const int s = y0 - 3 + k;
+    const bool ring_node = out_lane && s >= y0 && s < y0 + rows &&
+                           (ring_col || ((edges & 1) && s == y0) || ((edges & 2) && s == y0 + rows - 1));
+    if (k >= 2) {
+      // time t+1 at row s = r - 1
+      C e[Q] = {p0, p1, qb2, p3, f4, qb5, qb6, f7, f8};
+      collide<S, M>(e, c);
+#pragma unroll
+      for (int i = 0; i < Q; ++i) e[i] = S::round(e[i]);
+      if (ring_node) {
+        T* rowp = B + (int64_t)s * pitch + (x0 - 2);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) *plane_at(rowp, P, i, lane) = S::enc(e[i]);
+      }
+      const C e1 = from_left(e[1]), e5 = from_left(e[5]), e8 = from_left(e[8]);
+      const C e3 = from_right(e[3]), e6 = from_right(e[6]), e7 = from_right(e[7]);
+      if (k >= 4 && k < iters) {
+        // time t+2 at row s - 1
+        C h[Q] = {gp0, gp1, gb2, gp3, e[4], gb5, gb6, e7, e8};
+        collide<S, M>(h, c);
+        if (out_lane) {
+          T* rowp = Cn + (int64_t)(s - 1) * pitch + (x0 - 2);
+#pragma unroll
+          for (int i = 0; i < Q; ++i) *plane_at(rowp, P, i, lane) = S::enc(h[i]);
+        }
+      }
+      gb2 = ga2;
+      gb5 = ga5;
+      gb6 = ga6;
+      ga2 = e[2];
+      ga5 = e5;
+      ga6 = e6;
+      gp0 = e[0];
+      gp1 = e1;
+      gp3 = e3;
+    }
+    qb2 = qa2;
+    qb5 = qa5;
+    qb6 = qa6;
+    qa2 = f2;
+    qa5 = f5;
+    qa6 = f6;
+    p0 = f0;
+    p1 = f1;
+    p3 = f3;
+  };
+#pragma unroll 1
+  for (int k = 0; k < iters; k += TB_DEPTH) {
+#pragma unroll
+    for (int b = 0; b < TB_DEPTH; ++b) iteration(k + b, buf[b]);
+  }
+}
+
+
+// The single-step update of an interior node from a list (packed y << 16 | x), for the
+// interior nodes that no two-step task covers.
+template <class S, int M>
+__device__ inline void listed_node(int k, int n, const uint32_t* __restrict__ nodes,
+                                   const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
+                                   const Geom& g, const Coll& c) {
+  if (k >= n) return;
+  const uint32_t v = nodes[k];
+  const int x = (int)(v & 0xffffu), y = (int)(v >> 16);
+  typename S::C f[Q];
+  pull<S>(A, g, x, y, f);
+  collide<S, M>(f, c);
+  const int64_t idx = (int64_t)y * g.pitch + x;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) S::st_stream(B + i * g.plane + idx, f[i]);
+}
+
 }  // namespace vl
