@@ -108,9 +108,10 @@ def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
                glups=info.fluid_nodes * steps / secs / 1e9, bulk_ms=bulk_ms / max(bulk_n, 1),
                step_ms=total_ms / max(tsteps, 1), coef=np.concatenate([coef_w, coef]), d=d, dt=spec.u / d,
                split=bool(info.split), steps_total=int(info.t), nx=int(info.nx), ny=int(info.ny), spec=spec,
-               state_hash=state_hash, two_step=bool(info.two_step), tb_nodes=int(info.two_step_nodes),
-               tb_tasks=int(info.two_step_tasks), listed=int(info.listed_nodes), tb_rows=int(info.two_step_rows))
-    kname = "two-step kernel" if res["two_step"] else ("bulk kernel" if res["split"] else "step kernel")
+               state_hash=state_hash, pass_steps=int(info.pass_steps), pass_nodes=int(info.pass_nodes),
+               pass_tasks=int(info.pass_tasks), listed=int(info.listed_nodes), pass_rows=int(info.pass_rows))
+    kname = (f"{res['pass_steps']}-step pass kernel" if res["pass_steps"] > 1
+             else ("bulk kernel" if res["split"] else "step kernel"))
     log(f"[bench] {PREC_NAME[precision]}: {steps} steps in {secs:.2f} s = {res['glups']:.2f} GLUPS, "
         f"{kname} {res['bulk_ms'] * 1e3:.1f} us, whole step {res['step_ms'] * 1e3:.1f} us")
     if keep:
@@ -122,12 +123,12 @@ def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
 
 def dominant_kernel(r):
     """(name, units, algorithmic bytes per launch) of the kernel that dominates a step.
-    k_two takes its covered interior nodes two steps per launch and moves each node's nine
+    k_pass takes its covered interior nodes K steps per launch and moves each node's nine
     populations in and out of HBM once per launch; k_bulk moves them once per step for every
     interior node; fused, k_step does so for every fluid node."""
     p, b = PREC_NAME[r["precision"]], BYTES_PER_NODE[r["precision"]]
-    if r["two_step"]:
-        return f"k_two<{p}>", r["tb_nodes"], b * r["tb_nodes"]
+    if r["pass_steps"] > 1:
+        return f"k_pass<{p},K={r['pass_steps']}>", r["pass_nodes"], b * r["pass_nodes"]
     if r["split"]:
         return f"k_bulk<{p}>", r["fluid"] - r["bnodes"], b * (r["fluid"] - r["bnodes"])
     return f"k_step<{p}>", r["fluid"], b * r["fluid"]
@@ -342,7 +343,8 @@ def main():
     kname, units, traffic_alg = dominant_kernel(main_run)
     achieved = traffic_alg / (main_run["bulk_ms"] * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"{kname[:kname.index('<')]}_traffic_{wl}_{PREC_NAME[prec]}.json")
+    tname = kname[:kname.index("<")] + (f"{main_run['pass_steps']}" if main_run["pass_steps"] > 1 else "")
+    tf = os.path.join(ROOT, "profiles", f"{tname}_traffic_{wl}_{PREC_NAME[prec]}.json")
     if os.path.exists(tf) and args.scale == 1:
         with open(tf) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
@@ -370,14 +372,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kname, "kernel_us": round(main_run["bulk_ms"] * 1e3, 2),
-                     "algorithmic_bytes_per_launch": traffic_alg, "steps_per_launch": 2 if main_run["two_step"] else 1,
+                     "algorithmic_bytes_per_launch": traffic_alg, "steps_per_launch": max(main_run["pass_steps"], 1),
                      "copy_gbs": round(copy.value, 1), "frac_of_copy": round(achieved / copy.value, 4)},
         # the metric's "%HBM peak in GLUPS": GLUPS x the single-step bytes per update over the
         # peak; above 1 when a pass carries more than one step per HBM round trip
         "glups_frac_hbm": round(main_run["glups"] * BYTES_PER_NODE[prec] / HBM_PEAK_GBS, 4),
-        "two_step": {"on": main_run["two_step"], "covered_nodes": main_run["tb_nodes"],
-                     "listed_nodes": main_run["listed"], "tasks": main_run["tb_tasks"],
-                     "rows_per_task": main_run["tb_rows"]},
+        "pass": {"steps": main_run["pass_steps"], "covered_nodes": main_run["pass_nodes"],
+                 "listed_nodes": main_run["listed"], "tasks": main_run["pass_tasks"],
+                 "rows_per_task": main_run["pass_rows"]},
         "cpu_baseline": cpu,
     }
     out.update(extra)

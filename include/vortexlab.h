@@ -134,11 +134,11 @@ typedef struct {
   double outlet_density_offset; /* Sim::outlet_density_offset (last step) */
   int64_t device_bytes;
   int32_t split;  /* 1: boundary and bulk run as two concurrent kernels; 0: one fused kernel */
-  int32_t two_step;        /* 1: step pairs run as one two-step pass over the covered interior */
-  int64_t two_step_nodes;  /* interior nodes the two-step pass covers */
-  int64_t two_step_tasks;  /* its tasks (one wave each) */
+  int32_t pass_steps;      /* K: groups of K steps run one K-step pass over the covered interior (1: off) */
+  int64_t pass_nodes;      /* interior nodes the pass covers */
+  int64_t pass_tasks;      /* its tasks (one wave each) */
   int64_t listed_nodes;    /* interior nodes stepped one step at a time beside it */
-  int32_t two_step_rows;   /* output rows per task, at most */
+  int32_t pass_rows;       /* output rows per task, at most */
   int32_t pad_;
 } vl_sim_info;
 

@@ -1,0 +1,106 @@
+"""K-step passes (temporal blocking of the bulk) against the oracle and the single-step path.
+
+A group of K steps runs one kernel that takes the interior nodes of its tasks (strips of 64 V
+columns, one wave each) from t to t+K, beside K single steps of the boundary nodes and of the
+interior nodes no task covers.  Every node's arithmetic is the single-step bulk's, so FP64 must
+stay bit-identical to the oracle and to the reference's published hash, and FP32 / FP16 must
+give the single-step path's bits.  VL_PASS_STEPS=K forces the mode (and split mode with it) on
+small domains, VL_TB_ROWS sets the output rows per task; short tasks multiply the task edges,
+the ring nodes whose intermediate values single-step neighbours pull, and a run whose length is
+not a multiple of K ends in single steps.
+"""
+import numpy as np
+import pytest
+
+from vortexlab_amd import api
+
+from cases import cylinder_rig_spec, periodic_box, tunnel_spec
+
+pytestmark = pytest.mark.gpu
+
+KS = [2, 3, 4]
+
+
+def passes(monkeypatch, k, rows=None, graphs=True, layout=None):
+    """layout = (waves per workgroup, lockstep, first strip column, columns per lane), or None
+    for the defaults."""
+    monkeypatch.setenv("VL_PASS_STEPS", str(k))
+    monkeypatch.delenv("VL_STEP_MODE", raising=False)
+    monkeypatch.delenv("VL_BULK_V", raising=False)
+    names = ("VL_TB_WAVES", "VL_TB_SYNC", "VL_TB_X0", "VL_TB_V")
+    for name, value in zip(names, layout or (None,) * len(names)):
+        if value is None:
+            monkeypatch.delenv(name, raising=False)
+        else:
+            monkeypatch.setenv(name, str(value))
+    if rows is None:
+        monkeypatch.delenv("VL_TB_ROWS", raising=False)
+    else:
+        monkeypatch.setenv("VL_TB_ROWS", str(rows))
+    if graphs:
+        monkeypatch.delenv("VL_GRAPHS", raising=False)
+    else:
+        monkeypatch.setenv("VL_GRAPHS", "0")
+
+
+def single_step(monkeypatch, mode="split"):
+    monkeypatch.setenv("VL_PASS_STEPS", "1")
+    monkeypatch.setenv("VL_STEP_MODE", mode)
+    monkeypatch.delenv("VL_BULK_V", raising=False)
+    monkeypatch.delenv("VL_GRAPHS", raising=False)
+
+
+@pytest.mark.parametrize("k", KS)
+@pytest.mark.parametrize("rows,graphs,layout", [(64, True, None), (7, True, None), (1, False, None),
+                                                (12, True, (4, 0, 2, None)), (5, True, (8, 1, 7, None))])
+def test_pass_published_hash(gpu, monkeypatch, k, rows, graphs, layout):
+    """cylinder::bench n=40, 3000 steps (docs/report/index.html:59) with K-step passes, in the
+    default layout (8 waves per workgroup in step, strips from x = 32) and others."""
+    print(f"[passes] hash n=40 K={k} rows={rows} graphs={graphs} layout={layout}", flush=True)
+    passes(monkeypatch, k, rows, graphs, layout)
+    ch = api.Channel(gpu, cylinder_rig_spec(40), api.F64)
+    info = ch.sim.info()
+    assert info.pass_steps == k and info.split == 1
+    assert info.pass_nodes > 0.7 * info.fluid_nodes
+    assert info.pass_nodes + info.listed_nodes + info.bnode_count == info.fluid_nodes
+    ch.run(3000, 40.0)
+    assert ch.sim.state_hash() == 0x6B6814B976BBBAFF
+    ch.close()
+
+
+@pytest.mark.parametrize("k", KS)
+@pytest.mark.parametrize("model", ["bgk", "trt"])
+@pytest.mark.parametrize("steps", [300, 301])
+def test_pass_periodic_box_equals_oracle(gpu, orc, monkeypatch, k, model, steps):
+    """Periodic in x and y with a body force: the tasks stay off the wrapping edges, whose nodes
+    take the listed single-step path (with the wrap), and every task edge is a ring."""
+    print(f"[passes] periodic K={k} {model} steps={steps}", flush=True)
+    passes(monkeypatch, k, rows=5)
+    g = periodic_box(gpu, model, api.F64, nx=140, ny=40, accel=(1e-6, 0.0))
+    o = periodic_box(orc, model, 1, nx=140, ny=40, accel=(1e-6, 0.0))
+    assert g.info().pass_tasks > 0
+    g.step(steps)
+    o.step(steps)
+    assert np.array_equal(g.read(), o.read())
+
+
+@pytest.mark.parametrize("k", KS)
+@pytest.mark.parametrize("prec", [api.F32, api.F16])
+@pytest.mark.parametrize("rows,v", [(64, 1), (3, 1), (12, 2)])
+def test_pass_equals_single_step_reduced_precision(gpu, monkeypatch, k, prec, rows, v):
+    """A tunnel with radiating sides, a ramping inflow and a square body: K-step passes give the
+    single-step bits, populations and forces, over a step count that is not a multiple of K."""
+    print(f"[passes] K={k} prec={prec} rows={rows} V={v}", flush=True)
+    spec = tunnel_spec(d=16, re=100.0)
+    single_step(monkeypatch)
+    ref = api.Channel(gpu, spec, prec)
+    ref_coef = ref.run(1201, 16.0)
+    ref_pops = ref.sim.read()
+    ref.close()
+    passes(monkeypatch, k, rows, layout=(None, None, None, v))
+    ch = api.Channel(gpu, spec, prec)
+    assert ch.sim.info().pass_tasks > 0
+    coef = ch.run(1201, 16.0)
+    assert np.array_equal(ch.sim.read(), ref_pops), "populations differ"
+    assert np.array_equal(coef, ref_coef), "forces differ"
+    ch.close()

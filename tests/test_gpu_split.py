@@ -6,7 +6,7 @@ populations staged through LDS; nodes whose neighbours wrap around a periodic ed
 share a vector with a non-interior node, take the one-node path.  The arithmetic per node is the same in every variant, so FP64 must stay
 bit-identical to the oracle and to the reference's published hash, and FP32 / FP16 must give
 the same bits as the fused kernel.  VL_STEP_MODE forces the mode and VL_BULK_V the vector
-width (both read when a simulation is created); VL_TWO_STEP=0 keeps every step a single step.
+width (both read when a simulation is created); VL_PASS_STEPS=1 keeps every step a single step.
 """
 import numpy as np
 import pytest
@@ -22,7 +22,7 @@ WIDTHS = {api.F64: ["0", "1", "2"], api.F32: ["0", "1", "2", "4"], api.F16: ["0"
 
 
 def make(monkeypatch, mode, v=None):
-    monkeypatch.setenv("VL_TWO_STEP", "0")  # single steps only (two-step passes: test_gpu_two_step.py)
+    monkeypatch.setenv("VL_PASS_STEPS", "1")  # single steps only (K-step passes: test_gpu_passes.py)
     monkeypatch.setenv("VL_STEP_MODE", mode)
     if v is None:
         monkeypatch.delenv("VL_BULK_V", raising=False)

@@ -44,8 +44,8 @@ class SimInfo(C.Structure):
                 ("t", C.c_int64), ("precision", C.c_int32), ("collision", C.c_int32),
                 ("omega_plus", C.c_double), ("omega_minus", C.c_double),
                 ("outlet_density_offset", C.c_double), ("device_bytes", C.c_int64), ("split", C.c_int32),
-                ("two_step", C.c_int32), ("two_step_nodes", C.c_int64), ("two_step_tasks", C.c_int64),
-                ("listed_nodes", C.c_int64), ("two_step_rows", C.c_int32), ("pad_", C.c_int32)]
+                ("pass_steps", C.c_int32), ("pass_nodes", C.c_int64), ("pass_tasks", C.c_int64),
+                ("listed_nodes", C.c_int64), ("pass_rows", C.c_int32), ("pad_", C.c_int32)]
 
 
 class Ctl(C.Structure):

@@ -68,17 +68,18 @@ __global__ __launch_bounds__(BULK_T) void k_bulk(const typename S::T* __restrict
     bulk_vec<S, M, V>((int)blockIdx.x, BULK_T, xblocks, A, B, flag, rows, g, c);
 }
 
-// Two steps of the covered interior in one pass: one task per wave, W waves per workgroup.
-template <class S, int M, int W>
-__global__ __launch_bounds__(W * TB_LANES) void k_two(const typename S::T* __restrict__ A,
-                                                     typename S::T* __restrict__ B, typename S::T* __restrict__ Cn,
-                                                     const int4* __restrict__ tasks, int ntasks, Geom g, Coll c) {
+// K steps of the covered interior in one pass: one task per wave, W waves per workgroup, V
+// columns per lane, D rows of loads in flight.
+template <class S, int M, int W, int V, int D, int K>
+__global__ __launch_bounds__(W * TB_LANES) void k_pass(const typename S::T* __restrict__ A,
+                                                      PassOut<typename S::T> po, const int4* __restrict__ tasks,
+                                                      int ntasks, Geom g, Coll c) {
   const int w = __builtin_amdgcn_readfirstlane((int)blockIdx.x * W + (int)(threadIdx.x / TB_LANES));
   if (w >= ntasks) return;
-  two_step_strip<S, M>(tasks[w], A, B, Cn, g, c);
+  multi_step_strip<S, M, V, D, K>(tasks[w], A, po, g, c);
 }
 
-// One single step of the interior nodes that no two-step task covers.
+// One single step of the interior nodes that no pass task covers.
 template <class S, int M>
 __global__ __launch_bounds__(256) void k_listed(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
                                                 const uint32_t* __restrict__ nodes, int n, Geom g, Coll c) {
@@ -216,55 +217,78 @@ static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool 
   return VL_OK;
 }
 
-// Two steps, t -> t+2, with A = buf[cur], B = the next buffer (time t+1), C = the one after
-// (time t+2):
-//   phase 1: k_two takes the covered interior from A straight to C (and writes its ring nodes'
-//            t+1 values to B); beside it, on the side stream, the boundary kernel and the
-//            uncovered interior do step t, A -> B;
-//   phase 2: after both, the boundary kernel and the uncovered interior do step t+1, B -> C.
-// Every node of C is written once, by exactly one of these kernels.
+template <class S, int M, int V, int D, int K>
+static void launch_pass(vl_sim* s, const typename S::T* A, const PassOut<typename S::T>& po) {
+  const unsigned grid = (unsigned)((s->ntasks + s->tb_waves - 1) / s->tb_waves);
+  if (s->tb_waves == 8)
+    hipLaunchKernelGGL((k_pass<S, M, 8, V, D, K>), dim3(grid), dim3(8 * TB_LANES), 0, s->stream, A, po, s->dtasks,
+                       s->ntasks, s->g, s->coll);
+  else
+    hipLaunchKernelGGL((k_pass<S, M, 4, V, D, K>), dim3(grid), dim3(4 * TB_LANES), 0, s->stream, A, po, s->dtasks,
+                       s->ntasks, s->g, s->coll);
+}
+
+template <class S, int M, int K>
+static void launch_pass_k(vl_sim* s, const typename S::T* A, const PassOut<typename S::T>& po) {
+  if constexpr (sizeof(typename S::T) == 8)
+    launch_pass<S, M, 1, 3, K>(s, A, po);
+  else if (s->tb_v == 2)
+    launch_pass<S, M, 2, 2, K>(s, A, po);
+  else
+    launch_pass<S, M, 1, 3, K>(s, A, po);
+}
+
+// K steps, t -> t+K, through the buffers b_j = buf[(cur + j) % (K + 1)] (b_j holds time t+j):
+//   phase 1: k_pass takes the covered interior from b_0 straight to b_K (and writes its ring
+//            nodes' values of time t+j to b_j); beside it, on the side stream, the boundary kernel
+//            and the uncovered interior do step t, b_0 -> b_1;
+//   phase j = 2..K: after the previous phase, the boundary kernel and the uncovered interior do
+//            step t+j-1, b_{j-1} -> b_j.
+// Every node of b_K is written once, by exactly one of these kernels.
 template <class S, int M>
-static vl_status launch_pair(vl_sim* s, const int64_t* kbase, int k_local, bool timed_ok) {
+static vl_status launch_group(vl_sim* s, const int64_t* kbase, int k_local, bool timed_ok) {
   using T = typename S::T;
-  const T* A = (const T*)s->buf[s->cur];
-  T* B = (T*)s->buf[(s->cur + 1) % 3];
-  T* Cn = (T*)s->buf[(s->cur + 2) % 3];
-  const StepArgs s0 = step_args(s, s->t, kbase, k_local);
-  const StepArgs s1 = step_args(s, s->t + 1, kbase, k_local + 1);
+  const int K = s->tb_k;
+  T* b[TB_MAX_K + 1];
+  for (int j = 0; j <= K; ++j) b[j] = (T*)s->buf[(s->cur + j) % (K + 1)];
   const bool timed = timed_ok && s->timing && s->ev_used + 2 <= (int64_t)s->ev.size();
   // phase 1
-  VL_TRY((fork_boundary<S, M>(s, A, B, s0)));
-  launch_listed<S, M>(s, s->side, A, B);
+  VL_TRY((fork_boundary<S, M>(s, b[0], b[1], step_args(s, s->t, kbase, k_local))));
+  launch_listed<S, M>(s, s->side, b[0], b[1]);
   VL_HIP(hipEventRecord(s->ev_join, s->side));
   if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
   if (s->ntasks > 0) {
-    const unsigned grid = (unsigned)((s->ntasks + s->tb_waves - 1) / s->tb_waves);
-    if (s->tb_waves == 8)
-      hipLaunchKernelGGL((k_two<S, M, 8>), dim3(grid), dim3(8 * TB_LANES), 0, s->stream, A, B, Cn, s->dtasks,
-                         s->ntasks, s->g, s->coll);
+    PassOut<T> po{};
+    for (int j = 1; j < K; ++j) po.ring[j - 1] = b[j];
+    po.out = b[K];
+    if (K == 4)
+      launch_pass_k<S, M, 4>(s, b[0], po);
+    else if (K == 3)
+      launch_pass_k<S, M, 3>(s, b[0], po);
     else
-      hipLaunchKernelGGL((k_two<S, M, 4>), dim3(grid), dim3(4 * TB_LANES), 0, s->stream, A, B, Cn, s->dtasks,
-                         s->ntasks, s->g, s->coll);
+      launch_pass_k<S, M, 2>(s, b[0], po);
   }
   if (timed) {
     VL_HIP(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
     s->ev_used += 2;
   }
   VL_HIP(hipStreamWaitEvent(s->stream, s->ev_join, 0));
-  // phase 2
-  VL_TRY((fork_boundary<S, M>(s, B, Cn, s1)));
-  VL_HIP(hipEventRecord(s->ev_join, s->side));
-  launch_listed<S, M>(s, s->stream, B, Cn);
-  VL_HIP(hipStreamWaitEvent(s->stream, s->ev_join, 0));
+  // phases 2..K
+  for (int j = 2; j <= K; ++j) {
+    VL_TRY((fork_boundary<S, M>(s, b[j - 1], b[j], step_args(s, s->t + j - 1, kbase, k_local + j - 1))));
+    VL_HIP(hipEventRecord(s->ev_join, s->side));
+    launch_listed<S, M>(s, s->stream, b[j - 1], b[j]);
+    VL_HIP(hipStreamWaitEvent(s->stream, s->ev_join, 0));
+  }
   VL_HIP(hipGetLastError());
-  s->cur = (s->cur + 2) % 3;
-  s->t += 2;
+  s->cur = (s->cur + K) % (K + 1);
+  s->t += K;
   return VL_OK;
 }
 
 template <class S, int M>
 static vl_status launch_steps(vl_sim* s, int n, const int64_t* kbase, int k_local, bool timed_ok) {
-  return n == 2 ? launch_pair<S, M>(s, kbase, k_local, timed_ok) : launch_step<S, M>(s, kbase, k_local, timed_ok);
+  return n > 1 ? launch_group<S, M>(s, kbase, k_local, timed_ok) : launch_step<S, M>(s, kbase, k_local, timed_ok);
 }
 
 template <class S>
@@ -276,7 +300,7 @@ static vl_status step_dispatch(vl_sim* s, int n, const int64_t* kbase, int k_loc
   }
 }
 
-// Launches the next `n` steps (n = 2 only with two-step passes) as steps k_local.. of a window.
+// Launches the next `n` steps (n = 1, or n = K with K-step passes) as steps k_local.. of a window.
 static vl_status some_steps(vl_sim* s, int n, const int64_t* kbase, int k_local, bool timed_ok) {
   switch (s->precision) {
     case VL_F32: return step_dispatch<StoreF32>(s, n, kbase, k_local, timed_ok);
@@ -285,8 +309,8 @@ static vl_status some_steps(vl_sim* s, int n, const int64_t* kbase, int k_local,
   }
 }
 
-// Steps advanced per launch group when at least two remain.
-static int group_steps(const vl_sim* s) { return s->two_step ? 2 : 1; }
+// Steps advanced per launch group when that many remain.
+static int group_steps(const vl_sim* s) { return s->tb_k; }
 
 // Outward velocities of the outlet-controller nodes from the current A (start of a run).
 static vl_status prime_outlet(vl_sim* s) {
@@ -416,32 +440,33 @@ static void free_sim(vl_sim* s) {
 
 static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
-// Two-step task list, from the node flags (0 interior, 1 boundary, 2 solid):
-//  * strips of up to TB_OUT output columns start at x = 2, 62, 122, ...;
-//  * a row is usable for a strip when every node of the strip's columns plus one on each side
+// Task list of K-step passes, from the node flags (0 interior, 1 boundary, 2 solid):
+//  * strips of up to tb_out(K, V) output columns (a multiple of V) start at x = x0,
+//    x0 + tb_out(K, V), ... (V columns per lane);
+//  * a row is usable for a strip when every node of the strip's columns plus K - 1 on each side
 //    is interior and its pulls stay inside the domain without wrapping (rows 1 .. ny-2,
 //    columns 1 .. nx-2);
-//  * a run of usable rows [ya, yb] gives output rows ya+1 .. yb-1 (the t+1 margin), cut into
-//    tasks of at most `rows` rows of near-equal size.
+//  * a run of usable rows [ya, yb] gives output rows ya+K-1 .. yb-K+1 (the margin of the levels
+//    below K), cut into tasks of at most `rows` rows of near-equal size.
 // Tasks are ordered by (first row, first column) so a workgroup's waves sweep adjacent strips.
 // Every interior node no task covers goes to the single-step list; a covered node is a ring node
 // when one of its eight neighbours is a fluid node that no task covers, and a task marks each of
 // its four edges that holds a ring node.
-static vl_status build_two_step(vl_sim* s, int rows) {
+static vl_status build_passes(vl_sim* s, int rows) {
   const int nx = s->g.nx, ny = s->g.ny;
   const std::vector<uint8_t>& fl = s->tab.flag;
   if (nx >= 65536 || ny >= 65536 || (int64_t)s->g.pitch * ny >= ((int64_t)1 << 31) || rows < 1)
-    return set_error(VL_EINVAL, "two-step passes need nx, ny < 65536, a plane below 2^31 nodes and rows >= 1");
+    return set_error(VL_EINVAL, "K-step passes need nx, ny < 65536, a plane below 2^31 nodes and rows >= 1");
   std::vector<int4> tasks;
   std::vector<uint8_t> covered((size_t)nx * ny, 0);
   std::vector<uint8_t> usable((size_t)ny);
-  int xs = s->tb_x0;
-  for (int x0 = xs; x0 < nx; x0 += TB_OUT) {
-    const int w = std::min(TB_OUT, nx - 2 - x0);
+  const int V = s->tb_v, K = s->tb_k, out = tb_out(K, V);
+  for (int x0 = s->tb_x0; x0 < nx; x0 += out) {
+    const int w = std::min(out, nx - K - x0) / V * V;
     if (w < 1) break;
     for (int y = 0; y < ny; ++y) {
       bool ok = y >= 1 && y <= ny - 2;
-      for (int x = x0 - 1; ok && x <= x0 + w; ++x) ok = fl[(size_t)y * nx + x] == 0;
+      for (int x = x0 - (K - 1); ok && x <= x0 + w + K - 2; ++x) ok = fl[(size_t)y * nx + x] == 0;
       usable[(size_t)y] = ok;
     }
     for (int y = 0; y < ny;) {
@@ -451,7 +476,7 @@ static vl_status build_two_step(vl_sim* s, int rows) {
       }
       int yb = y;
       while (yb + 1 < ny && usable[(size_t)yb + 1]) ++yb;
-      const int first = y + 1, len = yb - y - 1;
+      const int first = y + K - 1, len = yb - y - 2 * K + 3;
       if (len >= 1) {
         const int nseg = (len + rows - 1) / rows;
         for (int k = 0; k < nseg; ++k) {
@@ -620,27 +645,37 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
     if (v == 0 || v == 1 || v == 2 || (v == 4 && precision != VL_F64) || (v == 8 && precision == VL_F16))
       s->bulk_v = v;
   }
-  // Two-step passes on split-mode domains (VL_TWO_STEP=0|1 overrides; 1 also forces split mode);
-  // VL_TB_ROWS sets the output rows per task.
-  s->two_step = s->split;
-  if (const char* e = std::getenv("VL_TWO_STEP")) {
-    s->two_step = e[0] == '1';
-    s->split = s->split || s->two_step;
+  // K-step passes on split-mode domains (VL_PASS_STEPS=1..4 overrides; above 1 it also forces
+  // split mode).
+  // Steps per pass and rows per task, measured on MI355X at 8192 x 4096: FP32 and FP64 4 steps,
+  // 16 rows; FP16 3 steps, 12 rows, 2 columns per lane (its conversions make 4 steps slower).
+  int k_steps = s->split ? (precision == VL_F16 ? 3 : 4) : 1;
+  if (const char* e = std::getenv("VL_PASS_STEPS")) {
+    k_steps = std::min(std::max(std::atoi(e), 1), TB_MAX_K);
+    s->split = s->split || k_steps > 1;
   }
-  if (s->two_step) {
-    // Tuning (measured on MI355X, 8192 x 4096 FP32): 12 output rows per task, 8 waves per
-    // workgroup sweeping in step, strips starting at x = 32 so that each workgroup's 480 output
-    // columns cover whole 128-byte lines.  VL_TB_ROWS, VL_TB_WAVES (4 | 8), VL_TB_SYNC (0 | 1)
-    // and VL_TB_X0 (>= 2) override.
-    int rows = 12;
+  if (k_steps > 1) {
+    s->tb_k = k_steps;
+    // Layout (measured as above): 8 waves per workgroup sweeping in step, strips from x = 32.
+    // VL_TB_ROWS, VL_TB_WAVES (4 | 8), VL_TB_SYNC (0 | 1), VL_TB_X0 (>= the halo) and VL_TB_V
+    // (columns per lane, 1 | 2; FP32 / FP16 only) override.
+    int rows = precision == VL_F16 ? 12 : 16;
     if (const char* e = std::getenv("VL_TB_ROWS")) rows = std::atoi(e);
     if (const char* e = std::getenv("VL_TB_WAVES")) s->tb_waves = std::atoi(e) == 4 ? 4 : 8;
     if (const char* e = std::getenv("VL_TB_SYNC")) s->tb_sync = e[0] != '0';
-    if (const char* e = std::getenv("VL_TB_X0")) s->tb_x0 = std::max(2, std::atoi(e));
-    VL_TRY(build_two_step(s, rows));
-    s->nbuf = 3;
-    VL_HIP(hipMalloc(&s->buf[2], s->pop_bytes));
-    s->device_bytes += (int64_t)s->pop_bytes;
+    s->tb_v = precision == VL_F16 ? 2 : 1;
+    if (const char* e = std::getenv("VL_TB_V"))
+      if (precision != VL_F64) s->tb_v = std::atoi(e) == 2 ? 2 : 1;
+    const int halo = tb_halo(s->tb_k, s->tb_v);
+    s->tb_x0 = std::max(s->tb_x0, halo);
+    if (const char* e = std::getenv("VL_TB_X0")) s->tb_x0 = std::max(halo, std::atoi(e));
+    s->tb_x0 += (s->tb_x0 - halo) % s->tb_v;  // the strip's first column is a multiple of V
+    VL_TRY(build_passes(s, rows));
+    s->nbuf = s->tb_k + 1;
+    for (int j = 2; j < s->nbuf; ++j) {
+      VL_HIP(hipMalloc(&s->buf[j], s->pop_bytes));
+      s->device_bytes += (int64_t)s->pop_bytes;
+    }
   }
   if (s->split) {
     VL_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
@@ -706,6 +741,7 @@ static vl_status collect_times(vl_sim* s, int64_t steps) {
 // kernel arguments of a window do not change between replays (controls and history are
 // indexed through the device counter kbase), so no per-step host work remains.
 static constexpr int GRAPH_K = 32;
+static int window_steps(const vl_sim* s) { return GRAPH_K / s->tb_k * s->tb_k; }  // whole groups
 
 static vl_status get_window_graph(vl_sim* s, hipGraphExec_t* out) {
   const int key = s->cur * 2 + (int)(s->t & 1);
@@ -718,9 +754,9 @@ static vl_status get_window_graph(vl_sim* s, hipGraphExec_t* out) {
   hipGraph_t gph = nullptr;
   VL_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   vl_status st = VL_OK;
-  const int gs = group_steps(s);
-  for (int k = 0; k < GRAPH_K && st == VL_OK; k += gs) st = some_steps(s, gs, s->d_kbase, k, false);
-  if (st == VL_OK) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s->stream, s->d_kbase, (int64_t)GRAPH_K);
+  const int gs = group_steps(s), win = window_steps(s);
+  for (int k = 0; k < win && st == VL_OK; k += gs) st = some_steps(s, gs, s->d_kbase, k, false);
+  if (st == VL_OK) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s->stream, s->d_kbase, (int64_t)win);
   hipError_t e = hipStreamEndCapture(s->stream, &gph);
   s->graph_cur[key] = s->cur;
   s->cur = cur0;  // capture recorded the steps without running them
@@ -770,18 +806,19 @@ static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double*
       VL_HIP(hipEventRecord(s->ev_chunk[0], s->stream));
     }
     int64_t k = 0;
-    if (graphs && m >= GRAPH_K) {
+    const int win = window_steps(s);
+    if (graphs && m >= win) {
       VL_HIP(hipMemsetAsync(s->d_kbase, 0, sizeof(int64_t), s->stream));
-      for (; k + GRAPH_K <= m; k += GRAPH_K) {
+      for (; k + win <= m; k += win) {
         hipGraphExec_t ge;
         VL_TRY(get_window_graph(s, &ge));
         VL_HIP(hipGraphLaunch(ge, s->stream));
         s->cur = s->graph_cur[s->cur * 2 + (int)(s->t & 1)];
-        s->t += GRAPH_K;
+        s->t += win;
       }
     }
     while (k < m) {  // remainder, launched directly
-      const int n = (int)std::min<int64_t>(group_steps(s), m - k);
+      const int n = m - k >= group_steps(s) ? group_steps(s) : 1;
       VL_TRY(some_steps(s, n, s->d_kzero, (int)k, true));
       k += n;
     }
@@ -894,11 +931,11 @@ vl_status vl_sim_info_get(vl_sim* s, vl_sim_info* o) {
   o->outlet_density_offset = s->outlet_density_offset;
   o->device_bytes = s->device_bytes;
   o->split = s->split ? 1 : 0;
-  o->two_step = s->two_step ? 1 : 0;
-  o->two_step_nodes = s->tb_nodes;
-  o->two_step_tasks = s->ntasks;
+  o->pass_steps = s->tb_k;
+  o->pass_nodes = s->tb_nodes;
+  o->pass_tasks = s->ntasks;
   o->listed_nodes = s->nlisted;
-  o->two_step_rows = s->tb_rows;
+  o->pass_rows = s->tb_rows;
   return VL_OK;
 }
 

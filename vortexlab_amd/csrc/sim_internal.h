@@ -79,16 +79,17 @@ struct vl_sim {
   hipStream_t side = nullptr;        // stream of the boundary kernel in split mode
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipStream_t stream = nullptr;
-  void* buf[3] = {nullptr, nullptr, nullptr};  // population arrays; buf[cur] is A
-  int nbuf = 2;                                 // 3 with two-step passes (A -> B -> C)
+  void* buf[5] = {};  // population arrays; buf[cur] is A
+  int nbuf = 2;                                          // K + 1 with K-step passes
   int cur = 0;
-  // Two-step passes (temporal blocking of the bulk): a step pair runs one kernel that takes the
-  // covered interior nodes from t to t+2 beside two single steps of every other fluid node.
-  bool two_step = false;
+  // K-step passes (temporal blocking of the bulk): a group of K steps runs one kernel that takes
+  // the covered interior nodes from t to t+K beside K single steps of every other fluid node.
+  int tb_k = 1;                      // steps per pass (1: every step is a single step)
   int tb_rows = 0;                   // output rows per task (upper bound)
-  int tb_waves = 8;                  // waves (tasks) per workgroup of the two-step kernel
+  int tb_waves = 8;                  // waves (tasks) per workgroup of the pass kernel
   bool tb_sync = true;               // a workgroup's waves sweep in step where their rows agree
   int tb_x0 = 32;                    // first output column of the first strip
+  int tb_v = 1;                      // columns per lane of the pass kernel
   int ntasks = 0;                    // one task per wave
   int nlisted = 0;                   // interior nodes no task covers
   int64_t tb_nodes = 0;              // interior nodes the tasks cover
@@ -105,8 +106,8 @@ struct vl_sim {
   int64_t* d_kbase = nullptr;    // window counter for replayed step graphs
   int64_t* d_kzero = nullptr;    // a device zero (directly launched steps)
   bool use_graphs = true;
-  hipGraphExec_t graph[6] = {};  // by (current buffer, step parity)
-  int graph_cur[6] = {};           // current buffer after the window
+  hipGraphExec_t graph[10] = {};  // by (current buffer, step parity)
+  int graph_cur[10] = {};           // current buffer after the window
   int64_t hist_cap = 0;
   std::vector<double> last_force;  // Sim::force after the last step
   int64_t device_bytes = 0;

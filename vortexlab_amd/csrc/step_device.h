@@ -534,28 +534,35 @@ __device__ inline void bulk_lds(int tile, int xblocks, const typename S::T* __re
 }
 
 // ------------------------------------------------------------------------------------------
-// Two time steps in one pass (temporal blocking of the bulk)
+// K time steps in one pass (temporal blocking of the bulk)
 // ------------------------------------------------------------------------------------------
-// One wave sweeps a strip of 64 columns upward through a band of rows.  Lane l owns column
-// x0 - 2 + l.  At every row it loads that row's nine populations from A (time t), computes
-// time t+1 one row below, and time t+2 two rows below; the three rows each level needs stay in
-// registers, and the x - 1 / x + 1 neighbours come from the adjacent lanes.  Lanes 1..62 hold
-// valid t+1 values, lanes 2..61 valid t+2 values, so a wave writes up to 60 columns.  Every
-// node the wave computes at t+1 (its output plus a one-node margin) is interior, so no link
-// rule applies and the arithmetic per node is the single-step bulk's: t+1 values are rounded
-// through the storage type exactly as a store and reload would, and the result is bit-identical
-// to two single steps.  Nodes whose t+1 value a single-step neighbour will pull (the task's
-// "ring") are also written to B.
+// One wave sweeps a strip of 64 V columns upward through a band of rows; lane l owns the V
+// columns xs + V l .. xs + V l + V - 1, where xs = x0 - tb_halo(K, V).  At every row r it loads that row's
+// nine populations from A (time t, level 0; one V-vector per plane), and each level L = 1..K
+// computes time t+L at row r - L from the three most recent rows of level L - 1, which stay in
+// registers; the x - 1 / x + 1 neighbours come from the lane's own registers or, at the lane's
+// edge elements, from the adjacent lanes.  Level L holds valid values L columns in from each
+// side, so a wave writes up to tb_out(K, V) columns.  Every node the wave computes below level K
+// (its output plus a margin of K - 1) is interior, so no link rule applies and the arithmetic
+// per node is the single-step bulk's: values below level K are rounded through the storage type
+// exactly as a store and reload would, and the result is bit-identical to K single steps.
 constexpr int TB_LANES = 64;
-constexpr int TB_OUT = TB_LANES - 4;
-constexpr int TB_DEPTH = 3;  // rows of loads in flight per wave
+constexpr int TB_MAX_K = 4;  // steps per pass, at most
+// Columns a wave holds left of its first output column: K rounded up to a multiple of V, so that
+// every lane's V columns start at a multiple of V (aligned vector loads and stores), and output
+// columns a wave writes: the rest, less the K columns on the right, a multiple of V.
+__host__ __device__ constexpr int tb_halo(int k, int v) { return (k + v - 1) / v * v; }
+__host__ __device__ constexpr int tb_out(int k, int v) { return (TB_LANES * v - tb_halo(k, v) - k) / v * v; }
 
 // Task of one wave: x = first output column, y = first output row, z = output rows,
-// w = output columns (<= TB_OUT) | edge bits << 8.  Ring nodes (covered nodes whose t+1 value
-// a single-step neighbour pulls) lie on the task's edges, since every other output node has all
-// eight neighbours inside the task; edge bit 1 / 2 / 4 / 8 = the first row / last row / first
-// column / last column holds some, and the whole edge then writes its t+1 values to B (values a
-// single step would also produce, so the extra ones are harmless).
+// w = output columns (<= tb_out(K, V), a multiple of V) | edge bits << 8 | lockstep << 12.
+// Ring nodes (covered nodes whose intermediate values single-step neighbours pull) lie on the
+// task's edges, since every other output node has all eight neighbours inside the task; edge bit
+// 1 / 2 / 4 / 8 = the first row / last row / first column / last column holds some, and the whole
+// edge then writes its values of levels 1..K-1 to the intermediate arrays (values the single
+// steps would also produce, so the extra ones are harmless).  Lockstep: the workgroup's tasks
+// cover the same rows, and its waves sweep in step (one barrier a row), so that together they
+// read and write whole lines.
 template <class C>
 __device__ inline C from_left(C v) {  // lane l receives lane l - 1's value
   return __shfl_up(v, 1, TB_LANES);
@@ -563,6 +570,19 @@ __device__ inline C from_left(C v) {  // lane l receives lane l - 1's value
 template <class C>
 __device__ inline C from_right(C v) {  // lane l receives lane l + 1's value
   return __shfl_down(v, 1, TB_LANES);
+}
+// The x - 1 (left) / x + 1 (right) neighbour of each of a lane's V columns.
+template <class C, int V>
+__device__ inline void left_of(const C (&v)[V], C (&o)[V]) {
+  o[0] = from_left(v[V - 1]);
+#pragma unroll
+  for (int j = 1; j < V; ++j) o[j] = v[j - 1];
+}
+template <class C, int V>
+__device__ inline void right_of(const C (&v)[V], C (&o)[V]) {
+  o[V - 1] = from_right(v[0]);
+#pragma unroll
+  for (int j = 0; j + 1 < V; ++j) o[j] = v[j + 1];
 }
 
 // A copy of v in a register of its own.  The unshifted populations of a row are carried to the
@@ -573,131 +593,226 @@ __device__ inline float fresh(float v) {
   asm volatile("v_mov_b32 %0, %1" : "=&v"(r) : "v"(v));
   return r;
 }
-__device__ inline double fresh(double v) {
-  double r;
-  asm volatile("v_mov_b64 %0, %1" : "=&v"(r) : "v"(v));
-  return r;
+__device__ inline double fresh(double v) {  // two 32-bit moves (no register-pair alignment to meet)
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const float lo = fresh(__builtin_bit_cast(float, (uint32_t)b));
+  const float hi = fresh(__builtin_bit_cast(float, (uint32_t)(b >> 32)));
+  return __builtin_bit_cast(double, (uint64_t)__builtin_bit_cast(uint32_t, hi) << 32 | __builtin_bit_cast(uint32_t, lo));
 }
 
 // Element `e` of plane i of a population array, from a wave-uniform base (the same for every
-// lane).  The plane's base goes through a scalar move the optimiser cannot look into, so the
-// address stays (scalar base) + (32-bit lane offset) instead of nine 64-bit vector addresses.
+// lane) plus a 32-bit lane offset, as a global-memory pointer.
 template <class T>
 using gptr = __attribute__((address_space(1))) T*;  // global memory
 template <class T>
 __device__ inline gptr<T> plane_at(T* base, int64_t plane, int i, int e) {
-  T* pb = base + i * plane;
-  asm volatile("s_mov_b64 %0, %1" : "=s"(pb) : "s"(pb));
-  return (gptr<T>)((__attribute__((address_space(1))) char*)pb + (uint32_t)e * (uint32_t)sizeof(T));
+  return (gptr<T>)((__attribute__((address_space(1))) char*)(base + i * plane) + (uint32_t)e * (uint32_t)sizeof(T));
 }
 
-template <class S, int M>
-__device__ inline void two_step_strip(const int4 task, const typename S::T* __restrict__ A,
-                                      typename S::T* __restrict__ B, typename S::T* __restrict__ Cn,
-                                      const Geom& g, const Coll& c) {
+// V consecutive stored values, loaded and stored as one vector.
+template <class T, int V>
+struct Pack {
+  using type = T __attribute__((ext_vector_type(V)));
+};
+template <class T>
+struct Pack<T, 1> {
+  using type = T;
+};
+template <class T, int V>
+__device__ inline T elem(const typename Pack<T, V>::type& v, int j) {
+  if constexpr (V == 1)
+    return v;
+  else
+    return v[j];
+}
+
+// Output of a pass: level K goes to `out`, levels 1..K-1 of the ring nodes to `ring[L - 1]`.
+template <class T>
+struct PassOut {
+  T* ring[TB_MAX_K - 1];
+  T* out;
+};
+
+// The three most recent rows of one level, as the next level pulls them: row r-1's f0, f1, f3
+// (x-shifted where the pull needs it) and f2, f5, f6, and row r-2's f2, f5, f6, per column.
+template <class C, int V>
+struct Rows3 {
+  C p0[V], p1[V], p3[V], qa2[V], qa5[V], qa6[V], qb2[V], qb5[V], qb6[V];
+  // row r of this level (shifted) becomes row r-1
+  __device__ inline void push(const C (&f)[Q][V]) {
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      qb2[j] = qa2[j];
+      qb5[j] = qa5[j];
+      qb6[j] = qa6[j];
+      qa2[j] = f[2][j];
+      qa5[j] = f[5][j];
+      qa6[j] = f[6][j];
+      p0[j] = f[0][j];
+      p1[j] = f[1][j];
+      p3[j] = f[3][j];
+    }
+  }
+};
+
+template <class S, int M, int V, int D, int K>
+__device__ __attribute__((always_inline)) inline void multi_step_strip(const int4 task,
+                                                                       const typename S::T* __restrict__ A,
+                                                                       const PassOut<typename S::T> po,
+                                                                       const Geom& g, const Coll& c) {
   using T = typename S::T;
   using C = typename S::C;
+  using PT = typename Pack<T, V>::type;
+  static_assert(K >= 2 && K <= TB_MAX_K, "steps per pass");
   const int lane = (int)(threadIdx.x & (TB_LANES - 1));
   const int x0 = task.x, y0 = task.y, rows = task.z, w = task.w & 0xff;
   const int edges = (task.w >> 8) & 15;
-  // the workgroup's four tasks cover the same rows: its waves sweep in step, one barrier a row
   const bool lockstep = (task.w >> 12) & 1;
-  const int xr = x0 - 2 + lane;                    // this lane's column
-  const int xl = min(max(xr, 0), g.nx - 1);        // clamped for loads (lanes past the strip)
-  const bool out_lane = lane >= 2 && lane < 2 + w;
+  const int xs = x0 - tb_halo(K, V);                   // first column of the strip (a multiple of V)
+  const int xc = xs + V * lane;                        // this lane's first column
+  const int xl = min(xc, g.pitch - V);                 // clamped for loads (lanes past the row)
+  const bool out_lane = xc >= x0 && xc + V <= x0 + w;  // all V columns are output columns
   const int64_t P = g.plane;
   const int pitch = g.pitch;
   // Every lane reads the same row: the address of (xl, row) in plane i is the wave-uniform
-  // A + i*P + row*pitch + (x0 - 2), plus the lane's constant offset xl - (x0 - 2).
-  const int lane_off = xl - (x0 - 2);
-  const int64_t row0 = (int64_t)(y0 - 2) * pitch + (x0 - 2);  // uniform part of row y0 - 2
-  const int iters = rows + 4;  // rows y0 - 2 .. y0 + rows + 1 of time t
-  // TB_DEPTH rows are loaded ahead: row k sits in buf[k % TB_DEPTH], and consuming it frees its
-  // buffer for row k + TB_DEPTH (the rows past the last one reload the last one, so every load
-  // is in bounds and unconditional).
-  T buf[TB_DEPTH][Q];
+  // A + i*P + row*pitch + xs, plus the lane's constant offset xl - xs.
+  const int lane_off = xl - xs;
+  const int64_t row0 = (int64_t)(y0 - K) * pitch + xs;  // uniform part of row y0 - K
+  const int iters = rows + 2 * K;                       // rows y0 - K .. y0 + rows + K - 1 of time t
+  // D rows are loaded ahead: row k sits in buf[k % D], and consuming it frees its buffer for row
+  // k + D (the rows past the last one reload the last one, so every load is in bounds and
+  // unconditional).
+  PT buf[D][Q];
 #pragma unroll
-  for (int b = 0; b < TB_DEPTH; ++b)
+  for (int b = 0; b < D; ++b)
 #pragma unroll
-    for (int i = 0; i < Q; ++i) buf[b][i] = *plane_at(A + row0 + (int64_t)min(b, iters - 1) * pitch, P, i, lane_off);
-  // level 0 (time t): row r-1's f0, f1, f3 and f2, f5, f6; row r-2's f2, f5, f6
-  C p0 = C(0), p1 = C(0), p3 = C(0), qa2 = C(0), qa5 = C(0), qa6 = C(0), qb2 = C(0), qb5 = C(0), qb6 = C(0);
-  // level 1 (time t+1): the same for the rows of t+1 values
-  C gp0 = C(0), gp1 = C(0), gp3 = C(0), ga2 = C(0), ga5 = C(0), ga6 = C(0), gb2 = C(0), gb5 = C(0), gb6 = C(0);
-  // lanes of the first / last output column that write their t+1 values to B on every row
-  const bool ring_col = out_lane && (((edges & 4) && lane == 2) || ((edges & 8) && lane == 1 + w));
-  // Iteration k consumes row r = y0 - 2 + k: first the row that arrived, then the stores of the
-  // previous iteration's results and the load of row k + 2, then the compute.  The memory
-  // counter that the wait for a load uses also counts stores, and the compiler waits for all
-  // of them whenever the accesses in flight differ between paths; so the loads are
-  // unconditional and follow the stores, and each wait covers accesses issued at least one
-  // whole iteration earlier.  The loop runs in groups of TB_DEPTH iterations; the iterations
-  // past the last one store the last row (the first of them) and otherwise keep nothing.
-  auto iteration = [&](const int k, T (&cur)[Q]) __attribute__((always_inline)) {
+    for (int i = 0; i < Q; ++i)
+      buf[b][i] = *(gptr<const PT>)plane_at(A + row0 + (int64_t)min(b, iters - 1) * pitch, P, i, lane_off);
+  Rows3<C, V> st[K];  // levels 0..K-1, as their next level pulls them
+#pragma unroll
+  for (int L = 0; L < K; ++L)
+#pragma unroll
+    for (int j = 0; j < V; ++j)
+      st[L].p0[j] = st[L].p1[j] = st[L].p3[j] = st[L].qa2[j] = st[L].qa5[j] = st[L].qa6[j] = st[L].qb2[j] =
+          st[L].qb5[j] = st[L].qb6[j] = C(0);
+  // columns of the first / last output column that write their intermediate values on every row
+  bool ring_col[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j)
+    ring_col[j] = out_lane && (((edges & 4) && xc + j == x0) || ((edges & 8) && xc + j == x0 + w - 1));
+  // Iteration k consumes row r = y0 - K + k: first the row that arrived, then the load of row
+  // k + D, then the compute and its stores.  The memory counter that the wait for a load uses
+  // also counts stores, and the compiler waits for all of them whenever the accesses in flight
+  // differ between paths; so the loads are unconditional, and a wait for a row leaves the D - 1
+  // rows loaded after it in flight.  Level L runs from iteration 2L on (its three source rows
+  // exist); the loop runs in groups of D iterations, and the iterations past the last one keep
+  // nothing.
+  auto iteration = [&](const int k, PT (&cur)[Q]) __attribute__((always_inline)) {
     if (lockstep) __syncthreads();
-    const C f0 = fresh(S::cv(cur[0])), f2 = fresh(S::cv(cur[2])), f4 = fresh(S::cv(cur[4]));
-    const C f1 = from_left(S::cv(cur[1])), f5 = from_left(S::cv(cur[5])), f8 = from_left(S::cv(cur[8]));
-    const C f3 = from_right(S::cv(cur[3])), f6 = from_right(S::cv(cur[6])), f7 = from_right(S::cv(cur[7]));
+    C fr[Q][V];  // the newest row of the level being consumed, x-shifted as the pull needs it
+    {
+      C u1[V], u3[V], u5[V], u6[V], u7[V], u8[V];
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        fr[0][j] = fresh(S::cv(elem<T, V>(cur[0], j)));
+        fr[2][j] = fresh(S::cv(elem<T, V>(cur[2], j)));
+        fr[4][j] = fresh(S::cv(elem<T, V>(cur[4], j)));
+        u1[j] = S::cv(elem<T, V>(cur[1], j));
+        u3[j] = S::cv(elem<T, V>(cur[3], j));
+        u5[j] = S::cv(elem<T, V>(cur[5], j));
+        u6[j] = S::cv(elem<T, V>(cur[6], j));
+        u7[j] = S::cv(elem<T, V>(cur[7], j));
+        u8[j] = S::cv(elem<T, V>(cur[8], j));
+      }
+      left_of(u1, fr[1]);
+      left_of(u5, fr[5]);
+      left_of(u8, fr[8]);
+      right_of(u3, fr[3]);
+      right_of(u6, fr[6]);
+      right_of(u7, fr[7]);
+    }
     __builtin_amdgcn_sched_barrier(0);
     {
-      const T* rowp = A + row0 + (int64_t)min(k + TB_DEPTH, iters - 1) * pitch;
+      const T* rowp = A + row0 + (int64_t)min(k + D, iters - 1) * pitch;
 #pragma unroll
-      for (int i = 0; i < Q; ++i) cur[i] = *plane_at(rowp, P, i, lane_off);
+      for (int i = 0; i < Q; ++i) cur[i] = *(gptr<const PT>)plane_at(rowp, P, i, lane_off);
     }
     __builtin_amdgcn_sched_barrier(0);
-    const int s = y0 - 3 + k;
-    const bool ring_node = out_lane && s >= y0 && s < y0 + rows &&
-                           (ring_col || ((edges & 1) && s == y0) || ((edges & 2) && s == y0 + rows - 1));
-    if (k >= 2) {
-      // time t+1 at row s = r - 1
-      C e[Q] = {p0, p1, qb2, p3, f4, qb5, qb6, f7, f8};
-      collide<S, M>(e, c);
+    bool active = true;  // the level below produced a row this iteration
 #pragma unroll
-      for (int i = 0; i < Q; ++i) e[i] = S::round(e[i]);
-      if (ring_node) {
-        T* rowp = B + (int64_t)s * pitch + (x0 - 2);
+    for (int L = 1; L <= K; ++L) {
+      if (!active) break;
+      const bool run = k >= 2 * L && (L < K || k < iters);
+      C e[Q][V];
+      if (run) {
+        const int s = y0 - K + k - L;  // the row level L computes
+        const bool ring_row = L < K && s >= y0 && s < y0 + rows;
+        const bool ring_all =
+            ring_row && out_lane && (((edges & 1) && s == y0) || ((edges & 2) && s == y0 + rows - 1));
+        PT outv[Q];
 #pragma unroll
-        for (int i = 0; i < Q; ++i) *plane_at(rowp, P, i, lane) = S::enc(e[i]);
-      }
-      const C e1 = from_left(e[1]), e5 = from_left(e[5]), e8 = from_left(e[8]);
-      const C e3 = from_right(e[3]), e6 = from_right(e[6]), e7 = from_right(e[7]);
-      if (k >= 4 && k < iters) {
-        // time t+2 at row s - 1
-        C h[Q] = {gp0, gp1, gb2, gp3, e[4], gb5, gb6, e7, e8};
-        collide<S, M>(h, c);
-        if (out_lane) {
-          T* rowp = Cn + (int64_t)(s - 1) * pitch + (x0 - 2);
+        for (int j = 0; j < V; ++j) {
+          const Rows3<C, V>& b = st[L - 1];
+          C f[Q] = {b.p0[j], b.p1[j], b.qb2[j], b.p3[j], fr[4][j], b.qb5[j], b.qb6[j], fr[7][j], fr[8][j]};
+          collide<S, M>(f, c);
+          if (L < K) {
 #pragma unroll
-          for (int i = 0; i < Q; ++i) *plane_at(rowp, P, i, lane) = S::enc(h[i]);
+            for (int i = 0; i < Q; ++i) e[i][j] = S::round(f[i]);
+            if (ring_all || (ring_row && ring_col[j])) {
+              T* rowp = po.ring[L - 1] + (int64_t)s * pitch + xs;
+#pragma unroll
+              for (int i = 0; i < Q; ++i) *plane_at(rowp, P, i, V * lane + j) = S::enc(e[i][j]);
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+              if constexpr (V == 1)
+                outv[i] = S::enc(f[i]);
+              else
+                outv[i][j] = S::enc(f[i]);
+            }
+          }
+        }
+        if (L == K && out_lane) {
+          T* rowp = po.out + (int64_t)s * pitch + xs;
+#pragma unroll
+          for (int i = 0; i < Q; ++i) *(gptr<PT>)plane_at(rowp, P, i, V * lane) = outv[i];
         }
       }
-      gb2 = ga2;
-      gb5 = ga5;
-      gb6 = ga6;
-      ga2 = e[2];
-      ga5 = e5;
-      ga6 = e6;
-      gp0 = e[0];
-      gp1 = e1;
-      gp3 = e3;
+      st[L - 1].push(fr);
+      active = run && L < K;
+      if (active) {  // level L's row, x-shifted, is what level L + 1 consumes
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+          fr[0][j] = e[0][j];
+          fr[2][j] = e[2][j];
+          fr[4][j] = e[4][j];
+        }
+        C t1[V], t3[V], t5[V], t6[V], t7[V], t8[V];
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+          t1[j] = e[1][j];
+          t3[j] = e[3][j];
+          t5[j] = e[5][j];
+          t6[j] = e[6][j];
+          t7[j] = e[7][j];
+          t8[j] = e[8][j];
+        }
+        left_of(t1, fr[1]);
+        left_of(t5, fr[5]);
+        left_of(t8, fr[8]);
+        right_of(t3, fr[3]);
+        right_of(t6, fr[6]);
+        right_of(t7, fr[7]);
+      }
     }
-    qb2 = qa2;
-    qb5 = qa5;
-    qb6 = qa6;
-    qa2 = f2;
-    qa5 = f5;
-    qa6 = f6;
-    p0 = f0;
-    p1 = f1;
-    p3 = f3;
   };
 #pragma unroll 1
-  for (int k = 0; k < iters; k += TB_DEPTH) {
+  for (int k = 0; k < iters; k += D) {
 #pragma unroll
-    for (int b = 0; b < TB_DEPTH; ++b) iteration(k + b, buf[b]);
+    for (int b = 0; b < D; ++b) iteration(k + b, buf[b]);
   }
 }
-
 
 // The single-step update of an interior node from a list (packed y << 16 | x), for the
 // interior nodes that no two-step task covers.
