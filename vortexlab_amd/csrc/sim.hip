@@ -79,11 +79,19 @@ __global__ __launch_bounds__(W * TB_LANES) void k_pass(const typename S::T* __re
   multi_step_strip<S, M, V, D, K>(tasks[w], A, po, g, c);
 }
 
-// One single step of the interior nodes that no pass task covers.
+// One single step of the nodes a K-step pass does not cover, in one launch: workgroups
+// [0, nlb) are the boundary part of the step (as in k_boundary), the rest take the listed
+// interior nodes.
 template <class S, int M>
-__global__ __launch_bounds__(256) void k_listed(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
-                                                const uint32_t* __restrict__ nodes, int n, Geom g, Coll c) {
-  listed_node<S, M>((int)blockIdx.x * 256 + (int)threadIdx.x, n, nodes, A, B, g, c);
+__global__ __launch_bounds__(STEP_T) void k_phase(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
+                                                  const uint32_t* __restrict__ nodes, int n, Geom g, Coll c,
+                                                  DevTables t, StepArgs sa) {
+  __shared__ double red[STEP_T];
+  const int blk = (int)blockIdx.x;
+  if (blk < sa.nlb)
+    boundary_block<S, M>(blk, A, B, g, c, t, sa, red);
+  else
+    listed_node<S, M>((blk - sa.nlb) * STEP_T + (int)threadIdx.x, n, nodes, A, B, g, c);
 }
 
 // Advances the window counter at the end of a replayed window of steps.
@@ -152,11 +160,13 @@ static vl_status fork_boundary(vl_sim* s, const typename S::T* A, typename S::T*
   return VL_OK;
 }
 
+// One single step of every node a K-step pass does not cover (boundary and listed interior).
 template <class S, int M>
-static void launch_listed(vl_sim* s, hipStream_t st, const typename S::T* A, typename S::T* B) {
-  if (s->nlisted > 0)
-    hipLaunchKernelGGL((k_listed<S, M>), dim3((unsigned)((s->nlisted + 255) / 256)), dim3(256), 0, st, A, B,
-                       s->dlisted, s->nlisted, s->g, s->coll);
+static void launch_phase(vl_sim* s, hipStream_t st, const typename S::T* A, typename S::T* B, const StepArgs& sa) {
+  const int nblk = sa.nlb + (s->nlisted + STEP_T - 1) / STEP_T;
+  if (nblk > 0)
+    hipLaunchKernelGGL((k_phase<S, M>), dim3((unsigned)nblk), dim3(STEP_T), 0, st, A, B, s->dlisted, s->nlisted,
+                       s->g, s->coll, s->dt, sa);
 }
 
 // Launch one step.  Fused mode: one kernel.  Split mode: the boundary kernel on the side stream
@@ -253,8 +263,9 @@ static vl_status launch_group(vl_sim* s, const int64_t* kbase, int k_local, bool
   for (int j = 0; j <= K; ++j) b[j] = (T*)s->buf[(s->cur + j) % (K + 1)];
   const bool timed = timed_ok && s->timing && s->ev_used + 2 <= (int64_t)s->ev.size();
   // phase 1
-  VL_TRY((fork_boundary<S, M>(s, b[0], b[1], step_args(s, s->t, kbase, k_local))));
-  launch_listed<S, M>(s, s->side, b[0], b[1]);
+  VL_HIP(hipEventRecord(s->ev_fork, s->stream));
+  VL_HIP(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+  launch_phase<S, M>(s, s->side, b[0], b[1], step_args(s, s->t, kbase, k_local));
   VL_HIP(hipEventRecord(s->ev_join, s->side));
   if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
   if (s->ntasks > 0) {
@@ -274,12 +285,8 @@ static vl_status launch_group(vl_sim* s, const int64_t* kbase, int k_local, bool
   }
   VL_HIP(hipStreamWaitEvent(s->stream, s->ev_join, 0));
   // phases 2..K
-  for (int j = 2; j <= K; ++j) {
-    VL_TRY((fork_boundary<S, M>(s, b[j - 1], b[j], step_args(s, s->t + j - 1, kbase, k_local + j - 1))));
-    VL_HIP(hipEventRecord(s->ev_join, s->side));
-    launch_listed<S, M>(s, s->stream, b[j - 1], b[j]);
-    VL_HIP(hipStreamWaitEvent(s->stream, s->ev_join, 0));
-  }
+  for (int j = 2; j <= K; ++j)
+    launch_phase<S, M>(s, s->stream, b[j - 1], b[j], step_args(s, s->t + j - 1, kbase, k_local + j - 1));
   VL_HIP(hipGetLastError());
   s->cur = (s->cur + K) % (K + 1);
   s->t += K;

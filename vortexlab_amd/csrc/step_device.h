@@ -212,6 +212,28 @@ __device__ inline void reduce_forces(const double* __restrict__ fc, const int32_
   }
 }
 
+// The fields of one link (the per-link columns of the boundary tables).
+struct LinkFields {
+  int dir, kind, vgroup, lflags, lnx, lny, fslot;
+  double q, cu6, rho, uwx, uwy;
+  __device__ static inline LinkFields load(const DevTables& t, int l) {
+    LinkFields L;
+    L.dir = t.dir[l];
+    L.kind = t.kind[l];
+    L.vgroup = t.vgroup[l];
+    L.lflags = t.lflags[l];
+    L.lnx = t.lnx[l];
+    L.lny = t.lny[l];
+    L.fslot = t.fslot[l];
+    L.q = t.q[l];
+    L.cu6 = t.cu6[l];
+    L.rho = t.rho[l];
+    L.uwx = t.uwx[l];
+    L.uwy = t.uwy[l];
+    return L;
+  }
+};
+
 // One boundary node (src/sim.rs:456-507): pull, replace the populations that would come out
 // of non-fluid cells by the link rules in link order, collide, write B; each force link
 // stores its momentum-exchange term.
@@ -232,21 +254,28 @@ __device__ inline void boundary_node(const typename S::T* __restrict__ A, typena
   // same value per link).
   bool have_u = false;
   double ux = 0.0, uy = 0.0;
-  const int l1 = t.bfirst[b + 1];
-  for (int l = t.bfirst[b]; l < l1; ++l) {
-    const int j = t.dir[l];
+  // Every field of a link is loaded together, unconditionally, and the next link's fields while
+  // this one is processed: one memory round trip per node's link list instead of several per
+  // link.  Indices past the table end are clamped (those loads are never used).
+  const int l0 = t.bfirst[b], l1 = t.bfirst[b + 1];
+  const int lmax = max(t.nl - 1, 0);
+  LinkFields nxt = LinkFields::load(t, min(l0, lmax));
+  for (int l = l0; l < l1; ++l) {
+    const LinkFields L = nxt;
+    nxt = LinkFields::load(t, min(l + 1, lmax));
+    const int j = L.dir;
     const int i = opp(j);
     const C out = own[j];
-    const double scale = ctl.vscale[t.vgroup[l]];
+    const double scale = ctl.vscale[L.vgroup];
     C back;
-    if (t.kind[l] == LINK_WALL) {
-      const double q = t.q[l];
+    if (L.kind == LINK_WALL) {
+      const double q = L.q;
       if constexpr (S::exact) {
-        const double corr = t.cu6[l] * scale;
+        const double corr = L.cu6 * scale;
         if (q >= 0.5) {
           const double k = 0.5 / q;
           back = k * (out - corr) + (1.0 - k) * own[i];
-        } else if (t.lflags[l] & 2) {
+        } else if (L.lflags & 2) {
           const int bxn = wrapi(x - cx(j), g.nx), byn = wrapi(y - cy(j), g.ny);
           back = 2.0 * q * out + (1.0 - 2.0 * q) * A[j * g.plane + (int64_t)byn * g.pitch + bxn] - corr;
         } else {
@@ -255,12 +284,12 @@ __device__ inline void boundary_node(const typename S::T* __restrict__ A, typena
       } else {
         // The wall rule is affine with weights summing to one, so it holds unchanged for
         // deviations (the w_i terms cancel).
-        const float corr = (float)(t.cu6[l] * scale);
+        const float corr = (float)(L.cu6 * scale);
         const float qf = (float)q;
         if (q >= 0.5) {
           const float k = (float)(0.5 / q);
           back = k * (out - corr) + (1.0f - k) * own[i];
-        } else if (t.lflags[l] & 2) {
+        } else if (L.lflags & 2) {
           const int bxn = wrapi(x - cx(j), g.nx), byn = wrapi(y - cy(j), g.ny);
           back = 2.0f * qf * out + (1.0f - 2.0f * qf) * S::ld(A + j * g.plane + (int64_t)byn * g.pitch + bxn) -
                  corr;
@@ -285,21 +314,21 @@ __device__ inline void boundary_node(const typename S::T* __restrict__ A, typena
         }
       }
       const double cu = 3.0 * ((double)cx(j) * ux + (double)cy(j) * uy);
-      const double off = (t.lflags[l] & 1) ? ((double)t.lnx[l] * ux + (double)t.lny[l] * uy) / sqrt_cs2 : drho;
+      const double off = (L.lflags & 1) ? ((double)L.lnx * ux + (double)L.lny * uy) / sqrt_cs2 : drho;
       if constexpr (S::exact) {
-        const double rho_w = t.rho[l] + off;
+        const double rho_w = L.rho + off;
         back = -out + 2.0 * wgt(j) * rho_w * (1.0 + 0.5 * cu * cu - 1.5 * (ux * ux + uy * uy));
       } else {
         // back - w = -d_out + 2 w (drho_w + rho_w (cu^2/2 - 1.5 u^2))
-        const double drw = (t.rho[l] - 1.0) + off;
+        const double drw = (L.rho - 1.0) + off;
         const double rho_w = 1.0 + drw;
         back = (C)(-(double)out + 2.0 * wgt(j) * (drw + rho_w * (0.5 * cu * cu - 1.5 * (ux * ux + uy * uy))));
       }
     }
     f[i] = back;
-    const int slot = t.fslot[l];
+    const int slot = L.fslot;
     if (slot >= 0) {
-      const double wx = t.uwx[l] * scale, wy = t.uwy[l] * scale;
+      const double wx = L.uwx * scale, wy = L.uwy * scale;
       double o = (double)out, bk = (double)back;
       if constexpr (!S::exact) {
         o += wgt(j);
