@@ -104,3 +104,31 @@ def test_pass_equals_single_step_reduced_precision(gpu, monkeypatch, k, prec, ro
     assert np.array_equal(ch.sim.read(), ref_pops), "populations differ"
     assert np.array_equal(coef, ref_coef), "forces differ"
     ch.close()
+
+
+@pytest.mark.parametrize("workload,prec,steps", [("notched_8192x4096", api.F32, 42),
+                                                 ("circle_16384x8192", api.F16, 23)])
+def test_pass_full_size_equals_single_step(gpu, monkeypatch, workload, prec, steps):
+    """The benchmark domains at full size (8192 x 4096 FP32 with the notched square, 16384 x 8192
+    FP16 with the D = 512 circle), in their default passes (4 and 3 steps): the state after a
+    step count that mixes passes with single steps has the single-step path's hash and forces."""
+    import bench
+    print(f"[passes] full size {workload} prec={prec} steps={steps}", flush=True)
+    builder, d = bench.WORKLOADS[workload][:2]
+    spec = builder(api, 1)
+    single_step(monkeypatch)
+    ref = api.Channel(gpu, spec, prec)
+    assert ref.sim.info().pass_steps == 1
+    ref_coef = ref.run(steps, d)
+    ref_hash = ref.sim.state_hash()
+    ref.close()
+    for name in ("VL_PASS_STEPS", "VL_STEP_MODE", "VL_TB_ROWS", "VL_TB_WAVES", "VL_TB_SYNC", "VL_TB_X0", "VL_TB_V"):
+        monkeypatch.delenv(name, raising=False)
+    ch = api.Channel(gpu, spec, prec)
+    info = ch.sim.info()
+    assert info.pass_steps == (3 if prec == api.F16 else 4)
+    assert info.pass_nodes > 0.98 * info.fluid_nodes
+    coef = ch.run(steps, d)
+    assert ch.sim.state_hash() == ref_hash
+    assert np.array_equal(coef, ref_coef)
+    ch.close()
