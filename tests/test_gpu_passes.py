@@ -84,12 +84,13 @@ def test_pass_periodic_box_equals_oracle(gpu, orc, monkeypatch, k, model, steps)
     assert np.array_equal(g.read(), o.read())
 
 
-@pytest.mark.parametrize("k", KS)
-@pytest.mark.parametrize("prec", [api.F32, api.F16])
-@pytest.mark.parametrize("rows,v", [(64, 1), (3, 1), (12, 2)])
+@pytest.mark.parametrize("k,prec,rows,v", [(k, p, r, v) for k in KS for p in (api.F32, api.F16)
+                                            for r, v in ((64, 1), (3, 1), (12, 2))]
+                         + [(5, api.F32, 64, 1), (5, api.F32, 3, 1), (5, api.F32, 20, 1)])
 def test_pass_equals_single_step_reduced_precision(gpu, monkeypatch, k, prec, rows, v):
     """A tunnel with radiating sides, a ramping inflow and a square body: K-step passes give the
-    single-step bits, populations and forces, over a step count that is not a multiple of K."""
+    single-step bits, populations and forces, over a step count that is not a multiple of K
+    (K = 5 is FP32 only, with one column per lane)."""
     print(f"[passes] K={k} prec={prec} rows={rows} V={v}", flush=True)
     spec = tunnel_spec(d=16, re=100.0)
     single_step(monkeypatch)
@@ -106,11 +107,11 @@ def test_pass_equals_single_step_reduced_precision(gpu, monkeypatch, k, prec, ro
     ch.close()
 
 
-@pytest.mark.parametrize("workload,prec,steps", [("notched_8192x4096", api.F32, 42),
+@pytest.mark.parametrize("workload,prec,steps", [("notched_8192x4096", api.F32, 43),
                                                  ("circle_16384x8192", api.F16, 23)])
 def test_pass_full_size_equals_single_step(gpu, monkeypatch, workload, prec, steps):
     """The benchmark domains at full size (8192 x 4096 FP32 with the notched square, 16384 x 8192
-    FP16 with the D = 512 circle), in their default passes (4 and 3 steps): the state after a
+    FP16 with the D = 512 circle), in their default passes (5 and 3 steps): the state after a
     step count that mixes passes with single steps has the single-step path's hash and forces."""
     import bench
     print(f"[passes] full size {workload} prec={prec} steps={steps}", flush=True)
@@ -126,7 +127,7 @@ def test_pass_full_size_equals_single_step(gpu, monkeypatch, workload, prec, ste
         monkeypatch.delenv(name, raising=False)
     ch = api.Channel(gpu, spec, prec)
     info = ch.sim.info()
-    assert info.pass_steps == (3 if prec == api.F16 else 4)
+    assert info.pass_steps == (3 if prec == api.F16 else 5)
     assert info.pass_nodes > 0.98 * info.fluid_nodes
     coef = ch.run(steps, d)
     assert ch.sim.state_hash() == ref_hash

@@ -148,6 +148,16 @@ static StepArgs step_args(vl_sim* s, int64_t t, const int64_t* kbase, int k_loca
   return sa;
 }
 
+// Joins the side stream of overlapped groups back into the main stream (before anything that
+// reads or writes nodes the side stream may still be stepping).
+static vl_status join_overlap(vl_sim* s) {
+  if (!s->ov_open) return VL_OK;
+  VL_HIP(hipEventRecord(s->ev_join, s->side));
+  VL_HIP(hipStreamWaitEvent(s->stream, s->ev_join, 0));
+  s->ov_open = false;
+  return VL_OK;
+}
+
 // Split mode: the boundary kernel of one step on the side stream, forked from the main stream.
 // The caller joins it (ev_join) before anything that reads what it wrote.
 template <class S, int M>
@@ -174,6 +184,7 @@ static void launch_phase(vl_sim* s, hipStream_t st, const typename S::T* A, type
 template <class S, int M>
 static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool timed_ok) {
   using T = typename S::T;
+  VL_TRY(join_overlap(s));
   const T* A = (const T*)s->buf[s->cur];
   T* B = (T*)s->buf[(s->cur + 1) % s->nbuf];
   const Geom& g = s->g;
@@ -227,66 +238,99 @@ static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool 
   return VL_OK;
 }
 
+// The pass kernel over the tasks [tasks, tasks + n) on stream `st`.
 template <class S, int M, int V, int D, int K>
-static void launch_pass(vl_sim* s, const typename S::T* A, const PassOut<typename S::T>& po) {
-  const unsigned grid = (unsigned)((s->ntasks + s->tb_waves - 1) / s->tb_waves);
+static void launch_pass(vl_sim* s, hipStream_t st, const int4* tasks, int n, const typename S::T* A,
+                        const PassOut<typename S::T>& po) {
+  const unsigned grid = (unsigned)((n + s->tb_waves - 1) / s->tb_waves);
   if (s->tb_waves == 8)
-    hipLaunchKernelGGL((k_pass<S, M, 8, V, D, K>), dim3(grid), dim3(8 * TB_LANES), 0, s->stream, A, po, s->dtasks,
-                       s->ntasks, s->g, s->coll);
+    hipLaunchKernelGGL((k_pass<S, M, 8, V, D, K>), dim3(grid), dim3(8 * TB_LANES), 0, st, A, po, tasks, n, s->g,
+                       s->coll);
   else
-    hipLaunchKernelGGL((k_pass<S, M, 4, V, D, K>), dim3(grid), dim3(4 * TB_LANES), 0, s->stream, A, po, s->dtasks,
-                       s->ntasks, s->g, s->coll);
+    hipLaunchKernelGGL((k_pass<S, M, 4, V, D, K>), dim3(grid), dim3(4 * TB_LANES), 0, st, A, po, tasks, n, s->g,
+                       s->coll);
 }
 
-template <class S, int M, int K>
-static void launch_pass_k(vl_sim* s, const typename S::T* A, const PassOut<typename S::T>& po) {
-  if constexpr (sizeof(typename S::T) == 8)
-    launch_pass<S, M, 1, 3, K>(s, A, po);
-  else if (s->tb_v == 2)
-    launch_pass<S, M, 2, 2, K>(s, A, po);
-  else
-    launch_pass<S, M, 1, 3, K>(s, A, po);
+template <class S, int M>
+static void launch_pass_tasks(vl_sim* s, hipStream_t st, const int4* tasks, int n, const typename S::T* A,
+                              const PassOut<typename S::T>& po) {
+  if (n <= 0) return;
+  const int K = s->tb_k;
+  const bool wide = sizeof(typename S::T) < 8 && s->tb_v == 2;  // V = 2 (D = 2) or V = 1 (D = 3)
+  if (K == 5) {
+    if constexpr (sizeof(typename S::T) == 4)  // FP32 only (FP64 and FP16 are slower above 4 / 3)
+      launch_pass<S, M, 1, 3, 5>(s, st, tasks, n, A, po);
+  } else if (K == 4) {
+    if (wide)
+      launch_pass<S, M, 2, 2, 4>(s, st, tasks, n, A, po);
+    else
+      launch_pass<S, M, 1, 3, 4>(s, st, tasks, n, A, po);
+  } else if (K == 3) {
+    if (wide)
+      launch_pass<S, M, 2, 2, 3>(s, st, tasks, n, A, po);
+    else
+      launch_pass<S, M, 1, 3, 3>(s, st, tasks, n, A, po);
+  } else {
+    if (wide)
+      launch_pass<S, M, 2, 2, 2>(s, st, tasks, n, A, po);
+    else
+      launch_pass<S, M, 1, 3, 2>(s, st, tasks, n, A, po);
+  }
 }
 
-// K steps, t -> t+K, through the buffers b_j = buf[(cur + j) % (K + 1)] (b_j holds time t+j):
-//   phase 1: k_pass takes the covered interior from b_0 straight to b_K (and writes its ring
-//            nodes' values of time t+j to b_j); beside it, on the side stream, the boundary kernel
-//            and the uncovered interior do step t, b_0 -> b_1;
-//   phase j = 2..K: after the previous phase, the boundary kernel and the uncovered interior do
-//            step t+j-1, b_{j-1} -> b_j.
-// Every node of b_K is written once, by exactly one of these kernels.
+
+// K steps, t -> t+K, through the buffers b_j = buf[(cur + j) % (K + 1)] (b_j holds time t+j).
+// The pass kernel takes the covered nodes from b_0 straight to b_K (and writes its ring nodes'
+// values of time t+j to b_j); the single-step phases take every other fluid node through
+// b_0 -> b_1 -> ... -> b_K, one launch per step.  Every node of b_K is written once.
+//
+// Overlapped (the default): the far tasks' pass runs on the main stream; the near tasks' pass
+// and the K phases run on the side stream.
+//   * near(g) waits for far(g-1) (its region holds far outputs) and, in stream order, for the
+//     phases of g-1 (it holds uncovered nodes);
+//   * far(g) waits for near(g-1) only: no far region holds an uncovered node, so far(g) runs
+//     while the phases of g-1 finish.
+// The far outputs are disjoint from the uncovered nodes and from the near outputs, and the
+// phases read and write only uncovered nodes and ring nodes (near outputs), so nothing a far
+// pass writes is touched by a phase still running; near(g+1) waits for far(g) before it writes
+// the ring nodes that far(g) may read.  Events alternate by group parity.
 template <class S, int M>
 static vl_status launch_group(vl_sim* s, const int64_t* kbase, int k_local, bool timed_ok) {
   using T = typename S::T;
   const int K = s->tb_k;
   T* b[TB_MAX_K + 1];
   for (int j = 0; j <= K; ++j) b[j] = (T*)s->buf[(s->cur + j) % (K + 1)];
+  PassOut<T> po{};
+  for (int j = 1; j < K; ++j) po.ring[j - 1] = b[j];
+  po.out = b[K];
   const bool timed = timed_ok && s->timing && s->ev_used + 2 <= (int64_t)s->ev.size();
-  // phase 1
-  VL_HIP(hipEventRecord(s->ev_fork, s->stream));
-  VL_HIP(hipStreamWaitEvent(s->side, s->ev_fork, 0));
-  launch_phase<S, M>(s, s->side, b[0], b[1], step_args(s, s->t, kbase, k_local));
-  VL_HIP(hipEventRecord(s->ev_join, s->side));
-  if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
-  if (s->ntasks > 0) {
-    PassOut<T> po{};
-    for (int j = 1; j < K; ++j) po.ring[j - 1] = b[j];
-    po.out = b[K];
-    if (K == 4)
-      launch_pass_k<S, M, 4>(s, b[0], po);
-    else if (K == 3)
-      launch_pass_k<S, M, 3>(s, b[0], po);
-    else
-      launch_pass_k<S, M, 2>(s, b[0], po);
+  const int nfar = s->tb_overlap ? s->ntasks_far : 0, nnear = s->ntasks - nfar;
+  bool first = false;
+  if (!s->ov_open) {  // fork the side stream from everything before this group
+    VL_HIP(hipEventRecord(s->ev_fork, s->stream));
+    VL_HIP(hipStreamWaitEvent(s->side, s->ev_fork, 0));
+    s->ov_open = true;
+    first = true;
   }
+  const int p = s->ov_parity;
+  // side stream: near pass, phase 1, (event), phases 2..K
+  if (!first) VL_HIP(hipStreamWaitEvent(s->side, s->ev_far[p ^ 1], 0));
+  launch_pass_tasks<S, M>(s, s->side, s->dtasks + nfar, nnear, b[0], po);
+  launch_phase<S, M>(s, s->side, b[0], b[1], step_args(s, s->t, kbase, k_local));
+  VL_HIP(hipEventRecord(s->ev_near[p], s->side));
+  for (int j = 2; j <= K; ++j)
+    launch_phase<S, M>(s, s->side, b[j - 1], b[j], step_args(s, s->t + j - 1, kbase, k_local + j - 1));
+  // main stream: far pass
+  if (!first) VL_HIP(hipStreamWaitEvent(s->stream, s->ev_near[p ^ 1], 0));
+  if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
+  launch_pass_tasks<S, M>(s, s->stream, s->dtasks, nfar, b[0], po);
   if (timed) {
     VL_HIP(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
     s->ev_used += 2;
   }
-  VL_HIP(hipStreamWaitEvent(s->stream, s->ev_join, 0));
-  // phases 2..K
-  for (int j = 2; j <= K; ++j)
-    launch_phase<S, M>(s, s->stream, b[j - 1], b[j], step_args(s, s->t + j - 1, kbase, k_local + j - 1));
+  VL_HIP(hipEventRecord(s->ev_far[p], s->stream));
+  s->ov_parity ^= 1;
+  if (!s->tb_overlap) VL_TRY(join_overlap(s));  // not overlapped: the group ends complete
   VL_HIP(hipGetLastError());
   s->cur = (s->cur + K) % (K + 1);
   s->t += K;
@@ -440,6 +484,8 @@ static void free_sim(vl_sim* s) {
   if (s->side) (void)hipStreamSynchronize(s->side);
   if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
   if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+  for (hipEvent_t e : {s->ev_far[0], s->ev_far[1], s->ev_near[0], s->ev_near[1]})
+    if (e) (void)hipEventDestroy(e);
   if (s->side) (void)hipStreamDestroy(s->side);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -467,10 +513,36 @@ static vl_status build_passes(vl_sim* s, int rows) {
   std::vector<int4> tasks;
   std::vector<uint8_t> covered((size_t)nx * ny, 0);
   std::vector<uint8_t> usable((size_t)ny);
-  const int V = s->tb_v, K = s->tb_k, out = tb_out(K, V);
-  for (int x0 = s->tb_x0; x0 < nx; x0 += out) {
-    const int w = std::min(out, nx - K - x0) / V * V;
-    if (w < 1) break;
+  const int V = s->tb_v, K = s->tb_k, halo = tb_halo(K, V), out = tb_out(K, V);
+  // Columns interior in at least half the rows: the strips' footprints stay within
+  // [xa, xb], so that a column of boundary nodes at an inlet or outlet does not disqualify a
+  // whole strip.  The strips are the aligned grid from tb_x0, a narrower first strip ending at
+  // tb_x0, and a last strip clipped at xb.
+  int xa = 1, xb = nx - 2;
+  {
+    std::vector<int> interior((size_t)nx, 0);
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) interior[(size_t)x] += fl[(size_t)y * nx + x] == 0;
+    while (xa < nx - 1 && 2 * interior[(size_t)xa] < ny) ++xa;
+    while (xb > 0 && 2 * interior[(size_t)xb] < ny) --xb;
+    xa = std::max(xa, 1);
+    xb = std::min(xb, nx - 2);
+  }
+  std::vector<std::pair<int, int>> strips;  // (first output column, output columns)
+  {
+    int x_first = std::max(halo, xa + K - 1);
+    x_first += (V - (x_first - halo) % V) % V;  // the strip's first column is a multiple of V
+    if (x_first < s->tb_x0) {
+      const int w = std::min({out, s->tb_x0 - x_first, xb - (K - 1) - x_first + 1}) / V * V;
+      if (w >= 1) strips.emplace_back(x_first, w);
+    }
+    for (int x0 = std::max(s->tb_x0, x_first); x0 < nx; x0 += out) {
+      const int w = std::min({out, nx - K - x0, xb - (K - 1) - x0 + 1}) / V * V;
+      if (w < 1) break;
+      strips.emplace_back(x0, w);
+    }
+  }
+  for (const auto& [x0, w] : strips) {
     for (int y = 0; y < ny; ++y) {
       bool ok = y >= 1 && y <= ny - 2;
       for (int x = x0 - (K - 1); ok && x <= x0 + w + K - 2; ++x) ok = fl[(size_t)y * nx + x] == 0;
@@ -524,24 +596,58 @@ static vl_status build_passes(vl_sim* s, int rows) {
       for (int xx = xa; xx <= xb; ++xx) {
         if (!ring[(size_t)yy * nx + xx]) continue;
         if (yy != ya && yy != yb && xx != xa && xx != xb)
-          return set_error(VL_EINVAL, "internal: a ring node inside a two-step task");
+          return set_error(VL_EINVAL, "internal: a ring node inside a pass task");
         edges |= (yy == ya ? 1 : 0) | (yy == yb ? 2 : 0) | (xx == xa ? 4 : 0) | (xx == xb ? 8 : 0);
       }
     tk.w |= edges << 8;
   }
+  // Far and near tasks.  A task is near when the region its outputs depend on (K nodes around
+  // them) holds a fluid node that no task covers: its pass must wait for the single steps of
+  // those nodes.  Far tasks depend on covered nodes only, so the next group's far pass can run
+  // while the single steps of this group's uncovered nodes go on.  Far tasks come first in the
+  // list; ring nodes are all in near tasks.
+  std::vector<int64_t> unc((size_t)(nx + 1) * (ny + 1), 0);  // 2-D prefix count of uncovered fluid
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) {
+      const size_t k = (size_t)y * nx + x;
+      unc[(size_t)(y + 1) * (nx + 1) + x + 1] = unc[(size_t)y * (nx + 1) + x + 1] +
+                                                unc[(size_t)(y + 1) * (nx + 1) + x] -
+                                                unc[(size_t)y * (nx + 1) + x] + (fl[k] != 2 && !covered[k]);
+    }
+  auto uncovered_in = [&](int xa, int ya, int xb, int yb) {  // inclusive, clipped to the domain
+    xa = std::max(xa, 0), ya = std::max(ya, 0), xb = std::min(xb, nx - 1), yb = std::min(yb, ny - 1);
+    return unc[(size_t)(yb + 1) * (nx + 1) + xb + 1] - unc[(size_t)ya * (nx + 1) + xb + 1] -
+           unc[(size_t)(yb + 1) * (nx + 1) + xa] + unc[(size_t)ya * (nx + 1) + xa];
+  };
+  std::stable_partition(tasks.begin(), tasks.end(), [&](const int4& tk) {
+    const int w = tk.w & 0xff;
+    return uncovered_in(tk.x - K, tk.y - K, tk.x + w - 1 + K, tk.y + tk.z - 1 + K) == 0;
+  });
+  int nfar = 0;
+  while (nfar < (int)tasks.size()) {
+    const int4& tk = tasks[(size_t)nfar];
+    const int w = tk.w & 0xff;
+    if (uncovered_in(tk.x - K, tk.y - K, tk.x + w - 1 + K, tk.y + tk.z - 1 + K) != 0) break;
+    if ((tk.w >> 8) & 15) return set_error(VL_EINVAL, "internal: a ring node in a far task");
+    ++nfar;
+  }
   // Lockstep bit: set on the tasks of a workgroup when they cover the same rows, so that its
-  // waves execute the same number of barriers.
+  // waves execute the same number of barriers.  Far and near tasks are launched separately,
+  // each set in workgroups of its own.
   if (s->tb_sync) {
     const size_t per = (size_t)s->tb_waves;
-    for (size_t b = 0; b + per <= tasks.size(); b += per) {
-      bool same = true;
-      for (size_t i = 1; i < per; ++i) same = same && tasks[b + i].y == tasks[b].y && tasks[b + i].z == tasks[b].z;
-      if (same)
-        for (size_t i = 0; i < per; ++i) tasks[b + i].w |= 1 << 12;
-    }
+    for (const auto [first, last] : {std::pair<size_t, size_t>(0, (size_t)nfar), {(size_t)nfar, tasks.size()}})
+      for (size_t b = first; b + per <= last; b += per) {
+        bool same = true;
+        for (size_t i = 1; i < per; ++i)
+          same = same && tasks[b + i].y == tasks[b].y && tasks[b + i].z == tasks[b].z;
+        if (same)
+          for (size_t i = 0; i < per; ++i) tasks[b + i].w |= 1 << 12;
+      }
   }
   s->tb_rows = rows;
   s->ntasks = (int)tasks.size();
+  s->ntasks_far = nfar;
   s->nlisted = (int)listed.size();
   s->tb_nodes = ncov;
   VL_TRY(upload(s, s->dtasks, tasks));
@@ -652,27 +758,33 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
     if (v == 0 || v == 1 || v == 2 || (v == 4 && precision != VL_F64) || (v == 8 && precision == VL_F16))
       s->bulk_v = v;
   }
-  // K-step passes on split-mode domains (VL_PASS_STEPS=1..4 overrides; above 1 it also forces
-  // split mode).
-  // Steps per pass and rows per task, measured on MI355X at 8192 x 4096: FP32 and FP64 4 steps,
-  // 16 rows; FP16 3 steps, 12 rows, 2 columns per lane (its conversions make 4 steps slower).
-  int k_steps = s->split ? (precision == VL_F16 ? 3 : 4) : 1;
+  // K-step passes on split-mode domains (VL_PASS_STEPS=1..4, 5 for FP32, overrides; above 1 it
+  // also forces split mode).
+  // Steps per pass and rows per task, measured on MI355X at 8192 x 4096: FP32 5 steps, 20 rows;
+  // FP64 4 steps, 16 rows (5 would need more registers than its occupancy allows); FP16 3
+  // steps, 12 rows, 2 columns per lane (its conversions make 4 steps slower).
+  int k_steps = s->split ? (precision == VL_F16 ? 3 : (precision == VL_F32 ? 5 : 4)) : 1;
   if (const char* e = std::getenv("VL_PASS_STEPS")) {
-    k_steps = std::min(std::max(std::atoi(e), 1), TB_MAX_K);
+    k_steps = std::min(std::max(std::atoi(e), 1), precision == VL_F32 ? TB_MAX_K : 4);
     s->split = s->split || k_steps > 1;
   }
   if (k_steps > 1) {
     s->tb_k = k_steps;
-    // Layout (measured as above): 8 waves per workgroup sweeping in step, strips from x = 32.
+    // Layout (measured as above): 8 waves per workgroup sweeping in step (FP64: 4), strips on a
+    // grid from x = 32.
     // VL_TB_ROWS, VL_TB_WAVES (4 | 8), VL_TB_SYNC (0 | 1), VL_TB_X0 (>= the halo) and VL_TB_V
     // (columns per lane, 1 | 2; FP32 / FP16 only) override.
-    int rows = precision == VL_F16 ? 12 : 16;
+    int rows = precision == VL_F16 ? 12 : (precision == VL_F32 ? 20 : 16);
     if (const char* e = std::getenv("VL_TB_ROWS")) rows = std::atoi(e);
+    // FP64 needs about 170 VGPRs: 4-wave workgroups fit 3 to a CU, 8-wave ones only 1
+    s->tb_waves = precision == VL_F64 ? 4 : 8;
     if (const char* e = std::getenv("VL_TB_WAVES")) s->tb_waves = std::atoi(e) == 4 ? 4 : 8;
     if (const char* e = std::getenv("VL_TB_SYNC")) s->tb_sync = e[0] != '0';
+    if (const char* e = std::getenv("VL_TB_OVERLAP")) s->tb_overlap = e[0] != '0';
     s->tb_v = precision == VL_F16 ? 2 : 1;
     if (const char* e = std::getenv("VL_TB_V"))
       if (precision != VL_F64) s->tb_v = std::atoi(e) == 2 ? 2 : 1;
+    if (s->tb_k == 5) s->tb_v = 1;  // the 5-step pass is built with one column per lane only
     const int halo = tb_halo(s->tb_k, s->tb_v);
     s->tb_x0 = std::max(s->tb_x0, halo);
     if (const char* e = std::getenv("VL_TB_X0")) s->tb_x0 = std::max(halo, std::atoi(e));
@@ -688,6 +800,10 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
     VL_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
     VL_HIP(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
     VL_HIP(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+    for (int i = 0; i < 2; ++i) {
+      VL_HIP(hipEventCreateWithFlags(&s->ev_far[i], hipEventDisableTiming));
+      VL_HIP(hipEventCreateWithFlags(&s->ev_near[i], hipEventDisableTiming));
+    }
   }
   if (!outlet_nodes_match(tb, t.npb))
     return set_error(VL_EINVAL, "internal: outlet-controller nodes do not match the pressure-boundary nodes");
@@ -763,6 +879,7 @@ static vl_status get_window_graph(vl_sim* s, hipGraphExec_t* out) {
   vl_status st = VL_OK;
   const int gs = group_steps(s), win = window_steps(s);
   for (int k = 0; k < win && st == VL_OK; k += gs) st = some_steps(s, gs, s->d_kbase, k, false);
+  if (st == VL_OK) st = join_overlap(s);
   if (st == VL_OK) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s->stream, s->d_kbase, (int64_t)win);
   hipError_t e = hipStreamEndCapture(s->stream, &gph);
   s->graph_cur[key] = s->cur;
@@ -829,6 +946,7 @@ static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double*
       VL_TRY(some_steps(s, n, s->d_kzero, (int)k, true));
       k += n;
     }
+    VL_TRY(join_overlap(s));
     if (ng > 0) VL_TRY(finish_forces(s, s->dhist + (m - 1) * ng * 2));
     if (s->timing) VL_HIP(hipEventRecord(s->ev_chunk[1], s->stream));
     if (ng > 0) {

@@ -79,7 +79,7 @@ struct vl_sim {
   hipStream_t side = nullptr;        // stream of the boundary kernel in split mode
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipStream_t stream = nullptr;
-  void* buf[5] = {};  // population arrays; buf[cur] is A
+  void* buf[6] = {};  // population arrays; buf[cur] is A
   int nbuf = 2;                                          // K + 1 with K-step passes
   int cur = 0;
   // K-step passes (temporal blocking of the bulk): a group of K steps runs one kernel that takes
@@ -90,7 +90,14 @@ struct vl_sim {
   bool tb_sync = true;               // a workgroup's waves sweep in step where their rows agree
   int tb_x0 = 32;                    // first output column of the first strip
   int tb_v = 1;                      // columns per lane of the pass kernel
-  int ntasks = 0;                    // one task per wave
+  int ntasks = 0;                    // one task per wave: far tasks first, then near ones
+  int ntasks_far = 0;
+  // Overlap of a group's single steps with the next group's far pass (side stream): whether the
+  // side stream has been forked since the last join, and the parity of the group's events.
+  bool tb_overlap = true;
+  bool ov_open = false;
+  int ov_parity = 0;
+  hipEvent_t ev_far[2] = {nullptr, nullptr}, ev_near[2] = {nullptr, nullptr};
   int nlisted = 0;                   // interior nodes no task covers
   int64_t tb_nodes = 0;              // interior nodes the tasks cover
   int4* dtasks = nullptr;
@@ -106,8 +113,8 @@ struct vl_sim {
   int64_t* d_kbase = nullptr;    // window counter for replayed step graphs
   int64_t* d_kzero = nullptr;    // a device zero (directly launched steps)
   bool use_graphs = true;
-  hipGraphExec_t graph[10] = {};  // by (current buffer, step parity)
-  int graph_cur[10] = {};           // current buffer after the window
+  hipGraphExec_t graph[12] = {};  // by (current buffer, step parity)
+  int graph_cur[12] = {};           // current buffer after the window
   int64_t hist_cap = 0;
   std::vector<double> last_force;  // Sim::force after the last step
   int64_t device_bytes = 0;

@@ -159,7 +159,13 @@ template <class S>
 __device__ inline double outlet_offset(const double* __restrict__ pv, int np, double target, double sqrt_cs2,
                                 double* red) {
   double sum = 0.0;  // serial running sum (exact) or this thread's strided partial (tree)
-  for (int base = 0; base < np; base += STEP_T) {
+  if constexpr (!S::exact) {
+    // this thread's strided partial: the loads are independent, so eight are issued at a time
+    // (the addition order stays slot k, k + T, k + 2T, ...)
+#pragma unroll 8
+    for (int k = (int)threadIdx.x; k < np; k += STEP_T) sum += pv[k];
+  }
+  for (int base = 0; S::exact && base < np; base += STEP_T) {
     const int k = base + (int)threadIdx.x;
     const double v = k < np ? pv[k] : 0.0;
     if constexpr (S::exact) {
@@ -170,8 +176,6 @@ __device__ inline double outlet_offset(const double* __restrict__ pv, int np, do
         for (int i = 0; i < m; ++i) sum += red[i];
       }
       __syncthreads();
-    } else {
-      sum += v;
     }
   }
   if constexpr (!S::exact) {
@@ -576,7 +580,7 @@ __device__ inline void bulk_lds(int tile, int xblocks, const typename S::T* __re
 // per node is the single-step bulk's: values below level K are rounded through the storage type
 // exactly as a store and reload would, and the result is bit-identical to K single steps.
 constexpr int TB_LANES = 64;
-constexpr int TB_MAX_K = 4;  // steps per pass, at most
+constexpr int TB_MAX_K = 5;  // steps per pass, at most
 // Columns a wave holds left of its first output column: K rounded up to a multiple of V, so that
 // every lane's V columns start at a multiple of V (aligned vector loads and stores), and output
 // columns a wave writes: the rest, less the K columns on the right, a multiple of V.
