@@ -109,7 +109,8 @@ def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
                step_ms=total_ms / max(tsteps, 1), coef=np.concatenate([coef_w, coef]), d=d, dt=spec.u / d,
                split=bool(info.split), steps_total=int(info.t), nx=int(info.nx), ny=int(info.ny), spec=spec,
                state_hash=state_hash, pass_steps=int(info.pass_steps), pass_nodes=int(info.pass_nodes),
-               pass_tasks=int(info.pass_tasks), listed=int(info.listed_nodes), pass_rows=int(info.pass_rows))
+               pass_tasks=int(info.pass_tasks), listed=int(info.listed_nodes), pass_rows=int(info.pass_rows),
+               pass_far_nodes=int(info.pass_far_nodes), pass_far_tasks=int(info.pass_far_tasks))
     kname = (f"{res['pass_steps']}-step pass kernel" if res["pass_steps"] > 1
              else ("bulk kernel" if res["split"] else "step kernel"))
     log(f"[bench] {PREC_NAME[precision]}: {steps} steps in {secs:.2f} s = {res['glups']:.2f} GLUPS, "
@@ -124,11 +125,14 @@ def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
 def dominant_kernel(r):
     """(name, units, algorithmic bytes per launch) of the kernel that dominates a step.
     k_pass takes its covered interior nodes K steps per launch and moves each node's nine
-    populations in and out of HBM once per launch; k_bulk moves them once per step for every
-    interior node; fused, k_step does so for every fluid node."""
+    populations in and out of HBM once per launch.  The timed launch is the far tasks' pass
+    (the near tasks' pass runs beside it on another stream), so its units are the far tasks'
+    nodes.  k_bulk moves them once per step for every interior node; fused, k_step does so for
+    every fluid node."""
     p, b = PREC_NAME[r["precision"]], BYTES_PER_NODE[r["precision"]]
     if r["pass_steps"] > 1:
-        return f"k_pass<{p},K={r['pass_steps']}>", r["pass_nodes"], b * r["pass_nodes"]
+        n = r["pass_far_nodes"] if r["pass_far_tasks"] > 0 else r["pass_nodes"]
+        return f"k_pass<{p},K={r['pass_steps']}>", n, b * n
     if r["split"]:
         return f"k_bulk<{p}>", r["fluid"] - r["bnodes"], b * (r["fluid"] - r["bnodes"])
     return f"k_step<{p}>", r["fluid"], b * r["fluid"]
@@ -378,7 +382,8 @@ def main():
         # peak; above 1 when a pass carries more than one step per HBM round trip
         "glups_frac_hbm": round(main_run["glups"] * BYTES_PER_NODE[prec] / HBM_PEAK_GBS, 4),
         "pass": {"steps": main_run["pass_steps"], "covered_nodes": main_run["pass_nodes"],
-                 "listed_nodes": main_run["listed"], "tasks": main_run["pass_tasks"],
+                 "far_nodes": main_run["pass_far_nodes"], "listed_nodes": main_run["listed"],
+                 "tasks": main_run["pass_tasks"], "far_tasks": main_run["pass_far_tasks"],
                  "rows_per_task": main_run["pass_rows"]},
         "cpu_baseline": cpu,
     }

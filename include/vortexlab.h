@@ -140,6 +140,8 @@ typedef struct {
   int64_t listed_nodes;    /* interior nodes stepped one step at a time beside it */
   int32_t pass_rows;       /* output rows per task, at most */
   int32_t pad_;
+  int64_t pass_far_nodes;  /* covered nodes of the far tasks (their pass overlaps the single steps) */
+  int64_t pass_far_tasks;
 } vl_sim_info;
 
 /* Per-step host controls (what Channel::step sets before Sim::step, cases/channel.rs:136-147).

@@ -45,7 +45,8 @@ class SimInfo(C.Structure):
                 ("omega_plus", C.c_double), ("omega_minus", C.c_double),
                 ("outlet_density_offset", C.c_double), ("device_bytes", C.c_int64), ("split", C.c_int32),
                 ("pass_steps", C.c_int32), ("pass_nodes", C.c_int64), ("pass_tasks", C.c_int64),
-                ("listed_nodes", C.c_int64), ("pass_rows", C.c_int32), ("pad_", C.c_int32)]
+                ("listed_nodes", C.c_int64), ("pass_rows", C.c_int32), ("pad_", C.c_int32),
+                ("pass_far_nodes", C.c_int64), ("pass_far_tasks", C.c_int64)]
 
 
 class Ctl(C.Structure):

@@ -624,13 +624,16 @@ static vl_status build_passes(vl_sim* s, int rows) {
     return uncovered_in(tk.x - K, tk.y - K, tk.x + w - 1 + K, tk.y + tk.z - 1 + K) == 0;
   });
   int nfar = 0;
+  int64_t far_nodes = 0;
   while (nfar < (int)tasks.size()) {
     const int4& tk = tasks[(size_t)nfar];
     const int w = tk.w & 0xff;
     if (uncovered_in(tk.x - K, tk.y - K, tk.x + w - 1 + K, tk.y + tk.z - 1 + K) != 0) break;
     if ((tk.w >> 8) & 15) return set_error(VL_EINVAL, "internal: a ring node in a far task");
+    far_nodes += (int64_t)w * tk.z;
     ++nfar;
   }
+  s->tb_far_nodes = far_nodes;
   // Lockstep bit: set on the tasks of a workgroup when they cover the same rows, so that its
   // waves execute the same number of barriers.  Far and near tasks are launched separately,
   // each set in workgroups of its own.
@@ -1061,6 +1064,8 @@ vl_status vl_sim_info_get(vl_sim* s, vl_sim_info* o) {
   o->pass_tasks = s->ntasks;
   o->listed_nodes = s->nlisted;
   o->pass_rows = s->tb_rows;
+  o->pass_far_nodes = s->tb_far_nodes;
+  o->pass_far_tasks = s->tb_overlap ? s->ntasks_far : 0;
   return VL_OK;
 }
 

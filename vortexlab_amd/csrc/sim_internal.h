@@ -92,6 +92,7 @@ struct vl_sim {
   int tb_v = 1;                      // columns per lane of the pass kernel
   int ntasks = 0;                    // one task per wave: far tasks first, then near ones
   int ntasks_far = 0;
+  int64_t tb_far_nodes = 0;          // covered nodes of the far tasks
   // Overlap of a group's single steps with the next group's far pass (side stream): whether the
   // side stream has been forked since the last join, and the parity of the group's events.
   bool tb_overlap = true;
