@@ -765,7 +765,7 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   // also forces split mode).
   // Steps per pass and rows per task, measured on MI355X at 8192 x 4096: FP32 5 steps, 20 rows;
   // FP64 4 steps, 16 rows (5 would need more registers than its occupancy allows); FP16 3
-  // steps, 12 rows, 2 columns per lane (its conversions make 4 steps slower).
+  // steps, 24 rows, 2 columns per lane (its conversions make 4 steps slower).
   int k_steps = s->split ? (precision == VL_F16 ? 3 : (precision == VL_F32 ? 5 : 4)) : 1;
   if (const char* e = std::getenv("VL_PASS_STEPS")) {
     k_steps = std::min(std::max(std::atoi(e), 1), precision == VL_F32 ? TB_MAX_K : 4);
@@ -777,7 +777,7 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
     // grid from x = 32.
     // VL_TB_ROWS, VL_TB_WAVES (4 | 8), VL_TB_SYNC (0 | 1), VL_TB_X0 (>= the halo) and VL_TB_V
     // (columns per lane, 1 | 2; FP32 / FP16 only) override.
-    int rows = precision == VL_F16 ? 12 : (precision == VL_F32 ? 20 : 16);
+    int rows = precision == VL_F16 ? 24 : (precision == VL_F32 ? 20 : 16);
     if (const char* e = std::getenv("VL_TB_ROWS")) rows = std::atoi(e);
     // FP64 needs about 170 VGPRs: 4-wave workgroups fit 3 to a CU, 8-wave ones only 1
     s->tb_waves = precision == VL_F64 ? 4 : 8;
