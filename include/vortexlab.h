@@ -245,6 +245,11 @@ typedef struct {
   double autotune_ms[3][4]; /* time of a trial step 0 (boundary kernel beside the bulk) per
                                (tile shape 64x4 / 128x2 / 256x1, bulk grid: one workgroup per
                                tile / 4 / 8 / 16 persistent per CU) */
+  /* K-step passes (pass_steps > 1): a case that runs a whole group of K steps advances its
+   * covered interior (pass_nodes over all cases, in pass_tasks strips of up to pass_rows rows)
+   * K steps per pass and its other interior nodes (listed_nodes) one step at a time. */
+  int32_t pass_steps, pass_rows, pass_tasks, listed_tiles;
+  int64_t pass_nodes, listed_nodes;
 } vl_batch_info;
 vl_status vl_batch_create(int precision, vl_batch** out);
 /* Adds a case (the channel of vl_channel_create; `body` consumed) that runs `steps` steps;

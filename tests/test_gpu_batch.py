@@ -140,3 +140,49 @@ def test_partial_runs_resume(gpu):
         b.run(piece)
     for k in range(len(cases)):
         assert np.array_equal(whole.result(k)[0], b.result(k)[0])
+
+
+def wide_cases(seed, n=3):
+    """Cases wide enough for eight strips to share a row band (workgroups in lockstep)."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for k in range(n):
+        nx = int(rng.integers(520, 700))
+        ny = int(rng.integers(60, 120))
+        d = float(rng.uniform(10.0, 18.0))
+        coll = api.BGK if k % 2 == 0 else api.TRT
+        sub = np.random.default_rng(200 + k)
+        c = workloads.tank_case(nx, ny, d, re=80.0, u=0.05, collision=coll,
+                                outline_fn=lambda cx, cy: workloads.star_polygon(sub, d, cx, cy), units=1.5,
+                                ramp_units=0.5, kick_units=0.25)
+        cases.append(c)
+    return cases
+
+
+@pytest.mark.parametrize("prec,k,rows", [(api.F32, 1, 20), (api.F32, 2, 20), (api.F32, 3, 7), (api.F32, 4, 3),
+                                         (api.F32, 5, 20), (api.F16, 2, 24), (api.F16, 4, 5), (api.F64, 3, 16)])
+def test_batch_passes_equal_single_case_runs(gpu, monkeypatch, prec, k, rows):
+    """K-step passes in the batch (VL_BATCH_PASS_STEPS=K; 1 = single steps only): every case,
+    whether it runs its last steps in groups of K or alone in single steps, ends with the
+    coefficient history and populations of the same case run alone."""
+    monkeypatch.setenv("VL_BATCH_PASS_STEPS", str(k))
+    monkeypatch.setenv("VL_BATCH_TB_ROWS", str(rows))
+    cases = wide_cases(seed=21 + k)
+    print(f"[batch] passes K={k} rows={rows} prec={prec} steps={[c.steps for c in cases]}", flush=True)
+    b = run_batch(gpu, cases, prec)
+    info = b.info()
+    assert info.pass_steps == k
+    if k > 1:
+        assert info.pass_tasks > 0 and info.pass_nodes > 0.5 * info.fluid_nodes
+    else:
+        assert info.pass_tasks == 0
+    monkeypatch.delenv("VL_BATCH_PASS_STEPS")
+    monkeypatch.delenv("VL_BATCH_TB_ROWS")
+    for j, c in enumerate(cases):
+        coef, done, status = b.result(j)
+        assert status == 0 and done == c.steps
+        ch = api.Channel(gpu, c.spec, prec)
+        ref = ch.run(c.steps, c.d)
+        assert np.array_equal(coef, ref), f"case {j}: coefficient history differs"
+        assert np.array_equal(b.read_case(j, c.spec.nx, c.spec.ny), ch.sim.read()), f"case {j}: populations differ"
+        ch.close()

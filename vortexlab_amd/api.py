@@ -77,7 +77,9 @@ class BatchInfoDesc(C.Structure):
                 ("boundary_tiles", C.c_int32), ("bulk_tiles", C.c_int32), ("tile_w", C.c_int32),
                 ("tile_h", C.c_int32), ("grid_bulk", C.c_int32), ("grid_boundary", C.c_int32),
                 ("last_run_seconds", C.c_double), ("device_bytes", C.c_int64),
-                ("autotune_ms", (C.c_double * 4) * 3)]
+                ("autotune_ms", (C.c_double * 4) * 3),
+                ("pass_steps", C.c_int32), ("pass_rows", C.c_int32), ("pass_tasks", C.c_int32),
+                ("listed_tiles", C.c_int32), ("pass_nodes", C.c_int64), ("listed_nodes", C.c_int64)]
 
 
 class VortexError(RuntimeError):

@@ -22,6 +22,8 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "step_device.h"
@@ -125,7 +127,10 @@ __global__ __launch_bounds__(BT) void k_batch_boundary(const typename S::T* __re
   }
 }
 
-// Bulk tiles: TW x TH interior nodes of one case per pass of a workgroup (TW * TH = BT).
+// Bulk tiles: TW x TH interior nodes of one case per pass of a workgroup (TW * TH = BT).  The
+// tiles run are [first, act[t]), where first = 0 in single steps; in step `jpos` of a group of
+// K steps that starts at t - jpos, first = first_at[t - jpos] skips the tiles of the cases that
+// run the whole group in K-step passes.
 template <class S>
 __global__ __launch_bounds__(BT) void k_batch_bulk(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
                                                    const uint8_t* __restrict__ flags,
@@ -133,12 +138,14 @@ __global__ __launch_bounds__(BT) void k_batch_bulk(const typename S::T* __restri
                                                    const int32_t* __restrict__ tcase, const int32_t* __restrict__ ty0,
                                                    const int32_t* __restrict__ tx0, const int32_t* __restrict__ act,
                                                    const int64_t* tbase, int t_local, int tw_log2,
-                                                   const int32_t* __restrict__ status) {
+                                                   const int32_t* __restrict__ status,
+                                                   const int32_t* __restrict__ first_at, int jpos) {
   const int64_t t = *tbase + t_local;
   const int nact = act[t];
+  const int first = first_at ? first_at[t - jpos] : 0;
   const int tx = (int)threadIdx.x & ((1 << tw_log2) - 1);
   const int ty = (int)threadIdx.x >> tw_log2;
-  for (int i = blockIdx.x; i < nact; i += gridDim.x) {
+  for (int i = first + (int)blockIdx.x; i < nact; i += gridDim.x) {
     const int ci = tcase[i];
     if (status[ci]) continue;
     const BatchCase& cs = cases[ci];
@@ -156,6 +163,74 @@ __global__ __launch_bounds__(BT) void k_batch_bulk(const typename S::T* __restri
       collide<S, MODEL_TRT>(f, cs.c);
 #pragma unroll
     for (int q = 0; q < Q; ++q) S::st_stream(b + q * cs.g.plane + idx, f[q]);
+  }
+}
+
+// K steps (t .. t+K) of the covered interior of the cases that run the whole group: one task
+// per wave, a strip of one case (task_case), on the case's arrays at its offset in each pool.
+// The cases are in descending step order, so those tasks are a prefix of ntask_at[t] tasks.
+template <class S, int W, int V, int D, int K>
+__global__ __launch_bounds__(W * TB_LANES) void k_batch_pass(const typename S::T* __restrict__ A,
+                                                            PassOut<typename S::T> po, const int4* __restrict__ tasks,
+                                                            const int32_t* __restrict__ task_case,
+                                                            const int32_t* __restrict__ ntask_at, const int64_t* tbase,
+                                                            int t_local, const BatchCase* __restrict__ cases,
+                                                            const int32_t* __restrict__ status) {
+  using T = typename S::T;
+  const int64_t t = *tbase + t_local;
+  const int w = __builtin_amdgcn_readfirstlane((int)blockIdx.x * W + (int)(threadIdx.x / TB_LANES));
+  if (w >= ntask_at[t]) return;
+  const int4 task = tasks[w];
+  const int ci = __builtin_amdgcn_readfirstlane(task_case[w]);
+  // A workgroup in lockstep holds the tasks of one case only (the prefix ends between cases, so
+  // all its waves got here): one read of the case's status serves them all, so they all run and
+  // meet the same barriers, or all return.
+  int skip;
+  if ((task.w >> 12) & 1) {
+    __shared__ int sk;
+    if (threadIdx.x == 0) sk = status[ci];
+    __syncthreads();
+    skip = sk;
+  } else {
+    skip = status[ci];
+  }
+  if (skip) return;
+  const BatchCase& cs = cases[ci];
+  const int64_t off = cs.pop_off;
+  PassOut<T> pc;
+#pragma unroll
+  for (int L = 0; L < TB_MAX_K - 1; ++L) pc.ring[L] = L < K - 1 ? po.ring[L] + off : nullptr;
+  pc.out = po.out + off;
+  if (cs.c.model == MODEL_BGK)
+    multi_step_strip<S, MODEL_BGK, V, D, K>(task, A + off, pc, cs.g, cs.c);
+  else
+    multi_step_strip<S, MODEL_TRT, V, D, K>(task, A + off, pc, cs.g, cs.c);
+}
+
+// One single step of the interior nodes that no task covers, for the cases in K-step passes:
+// listed tile i holds count[i] <= BT nodes of case lcase[i] (packed y << 16 | x from
+// nodes + first[i]); the tiles of the cases that run the group starting at t - jpos are a prefix
+// of nlist_at[t - jpos] tiles.
+template <class S>
+__global__ __launch_bounds__(BT) void k_batch_listed(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
+                                                     const BatchCase* __restrict__ cases,
+                                                     const int32_t* __restrict__ lcase, const int32_t* __restrict__ lfirst,
+                                                     const int32_t* __restrict__ lcount,
+                                                     const uint32_t* __restrict__ nodes,
+                                                     const int32_t* __restrict__ nlist_at, const int64_t* tbase,
+                                                     int t_local, int jpos, const int32_t* __restrict__ status) {
+  const int64_t t = *tbase + t_local;
+  const int n = nlist_at[t - jpos];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int ci = lcase[i];
+    if (status[ci]) continue;
+    const BatchCase& cs = cases[ci];
+    const typename S::T* a = A + cs.pop_off;
+    typename S::T* b = B + cs.pop_off;
+    if (cs.c.model == MODEL_BGK)
+      listed_node<S, MODEL_BGK>((int)threadIdx.x, lcount[i], nodes + lfirst[i], a, b, cs.g, cs.c);
+    else
+      listed_node<S, MODEL_TRT>((int)threadIdx.x, lcount[i], nodes + lfirst[i], a, b, cs.g, cs.c);
   }
 }
 
@@ -223,7 +298,22 @@ struct vl_batch {
   // device
   hipStream_t stream = nullptr, side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  void* pool[2] = {nullptr, nullptr};
+  // population pools: 2, or K + 1 with K-step passes (pool j + 1 follows pool j in time)
+  void* pool[TB_MAX_K + 1] = {};
+  int nbuf = 2;
+  // K-step passes (pass_k > 1): the covered interior of a case that runs a whole group of K
+  // steps advances in one pass; its other interior nodes are listed single steps.  Tasks and
+  // listed tiles are in the cases' descending step order.
+  int pass_k = 1, pass_rows = 0, pass_waves = 8;
+  int npass_tasks = 0, nlisted_tiles = 0;
+  int64_t pass_nodes = 0, listed_nodes = 0;
+  int4* d_ptask = nullptr;
+  int32_t *d_ptcase = nullptr, *d_pn_at = nullptr, *d_kfirst_at = nullptr;
+  int32_t *d_lcase = nullptr, *d_lfirst = nullptr, *d_lcount = nullptr, *d_ln_at = nullptr;
+  uint32_t* d_lnodes = nullptr;
+  // per group start t: tasks and listed tiles of the cases with steps >= t + K, and the bulk
+  // tiles of those cases (the bulk kernel skips them in a group)
+  std::vector<int32_t> h_pn_at, h_ln_at, h_kfirst_at;
   uint8_t* dflags = nullptr;
   BatchCase* dcases = nullptr;
   std::vector<void*> allocs;  // table pools etc.
@@ -244,7 +334,8 @@ struct vl_batch {
   std::vector<int32_t> ladder;  // grid sizes used when bulk_mult == 0
   int cur = 0;
   int64_t t = 0;
-  std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (A/B parity, bulk grid) -> window
+  // (first pool, bulk grid, pass grid, listed grid) -> window
+  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
   bool use_graphs = true;
   double autotune_ms[3][4] = {};
   double last_run_seconds = 0.0;
@@ -338,7 +429,7 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, int 
                             const vl_batch::StepEvents* ev) {
   using T = typename S::T;
   const T* A = (const T*)b->pool[b->cur];
-  T* B = (T*)b->pool[b->cur ^ 1];
+  T* B = (T*)b->pool[(b->cur + 1) % b->nbuf];
   // serial: the boundary kernel first, then the bulk, both on the main stream
   hipStream_t bs = b->serial ? b->stream : b->side;
   if (!b->serial) {
@@ -353,14 +444,102 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, int 
   if (!b->serial) VL_HIP(hipEventRecord(b->ev_join, b->side));
   if (ev) VL_HIP(hipEventRecord(ev->ks, b->stream));
   hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)grid_bulk), dim3(BT), 0, b->stream, A, B, b->dflags,
-                     b->dcases, b->d_kcase, b->d_ky0, b->d_kx0, b->d_kact, tbase, t_local, b->tw_log2, b->d_status);
+                     b->dcases, b->d_kcase, b->d_ky0, b->d_kx0, b->d_kact, tbase, t_local, b->tw_log2, b->d_status,
+                     (const int32_t*)nullptr, 0);
   if (ev) VL_HIP(hipEventRecord(ev->ke, b->stream));
   if (!b->serial) VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
   if (ev) VL_HIP(hipEventRecord(ev->ej, b->stream));
   VL_HIP(hipGetLastError());
-  b->cur ^= 1;
+  b->cur = (b->cur + 1) % b->nbuf;
   b->t += 1;
   return VL_OK;
+}
+
+// Grids of a group of K steps (workgroups): the pass, the listed tiles, the bulk tiles.
+struct GroupGrids {
+  int pass, listed, bulk;
+};
+
+template <class S, int W, int V, int D, int K>
+void launch_batch_pass(vl_batch* b, hipStream_t st, const typename S::T* A, PassOut<typename S::T> po,
+                       const int64_t* tbase, int t_local, int grid) {
+  hipLaunchKernelGGL((k_batch_pass<S, W, V, D, K>), dim3((unsigned)grid), dim3(W * TB_LANES), 0, st, A, po,
+                     b->d_ptask, b->d_ptcase, b->d_pn_at, tbase, t_local, b->dcases, b->d_status);
+}
+
+// The pass kernel of the batch's K and precision: FP32 one column per lane (K = 2..5), FP16
+// two (K = 2..4), FP64 one, in 4-wave workgroups (K = 2..4); 8 waves per workgroup otherwise.
+template <class S>
+vl_status launch_pass_k(vl_batch* b, hipStream_t st, const typename S::T* A, PassOut<typename S::T> po,
+                        const int64_t* tbase, int t_local, int grid) {
+  const int K = b->pass_k;
+  if constexpr (std::is_same<S, StoreF16>::value) {
+    if (K == 2) launch_batch_pass<S, 8, 2, 2, 2>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 3) launch_batch_pass<S, 8, 2, 2, 3>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 4) launch_batch_pass<S, 8, 2, 2, 4>(b, st, A, po, tbase, t_local, grid);
+    else return set_error(VL_EINVAL, "FP16 batch passes take 2..4 steps");
+  } else if constexpr (std::is_same<S, StoreF64>::value) {
+    if (K == 2) launch_batch_pass<S, 4, 1, 3, 2>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 3) launch_batch_pass<S, 4, 1, 3, 3>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 4) launch_batch_pass<S, 4, 1, 3, 4>(b, st, A, po, tbase, t_local, grid);
+    else return set_error(VL_EINVAL, "FP64 batch passes take 2..4 steps");
+  } else {
+    if (K == 2) launch_batch_pass<S, 8, 1, 3, 2>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 3) launch_batch_pass<S, 8, 1, 3, 3>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 4) launch_batch_pass<S, 8, 1, 3, 4>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 5) launch_batch_pass<S, 8, 1, 3, 5>(b, st, A, po, tbase, t_local, grid);
+    else return set_error(VL_EINVAL, "FP32 batch passes take 2..5 steps");
+  }
+  return VL_OK;
+}
+
+// A group of K steps from t = *tbase + t_local, pools cur .. cur + K.  Step j reads pool cur + j
+// and writes pool cur + j + 1: the boundary kernel on the side stream; on the main stream, in
+// step 0 only, the pass of the covered interior (t -> t + K into pool cur + K, ring nodes' levels
+// into pools cur + 1 .. cur + K - 1), then the listed nodes and the bulk tiles of the cases that
+// end inside the group.  Step j + 1 starts after step j has joined, so its boundary and listed
+// nodes find their covered neighbours' level j written.
+template <class S>
+vl_status launch_batch_group(vl_batch* b, const int64_t* tbase, int t_local, const GroupGrids& gg) {
+  using T = typename S::T;
+  const int K = b->pass_k, nb = b->nbuf;
+  auto P = [&](int j) { return (T*)b->pool[(b->cur + j) % nb]; };
+  PassOut<T> po{};
+  for (int L = 1; L < K; ++L) po.ring[L - 1] = P(L);
+  po.out = P(K);
+  hipStream_t bs = b->serial ? b->stream : b->side;
+  for (int j = 0; j < K; ++j) {
+    const T* A = P(j);
+    T* B = P(j + 1);
+    if (!b->serial) {
+      VL_HIP(hipEventRecord(b->ev_fork, b->stream));
+      VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+    }
+    hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, bs, A, B, b->dcases,
+                       b->d_bcase, b->d_bb0, b->d_bact, tbase, t_local + j, b->d_kick, b->d_hist, b->d_status,
+                       b->d_done, SQRT_CS2_B);
+    if (!b->serial) VL_HIP(hipEventRecord(b->ev_join, b->side));
+    if (j == 0) VL_TRYB(launch_pass_k<S>(b, b->stream, A, po, tbase, t_local, gg.pass));
+    hipLaunchKernelGGL(k_batch_listed<S>, dim3((unsigned)gg.listed), dim3(BT), 0, b->stream, A, B, b->dcases,
+                       b->d_lcase, b->d_lfirst, b->d_lcount, b->d_lnodes, b->d_ln_at, tbase, t_local + j, j,
+                       b->d_status);
+    hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)gg.bulk), dim3(BT), 0, b->stream, A, B, b->dflags,
+                       b->dcases, b->d_kcase, b->d_ky0, b->d_kx0, b->d_kact, tbase, t_local + j, b->tw_log2,
+                       b->d_status, (const int32_t*)b->d_kfirst_at, j);
+    if (!b->serial) VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+    VL_HIP(hipGetLastError());
+  }
+  b->cur = (b->cur + K) % nb;
+  b->t += K;
+  return VL_OK;
+}
+
+vl_status batch_group(vl_batch* b, const int64_t* tbase, int t_local, const GroupGrids& gg) {
+  switch (b->precision) {
+    case VL_F64: return launch_batch_group<StoreF64>(b, tbase, t_local, gg);
+    case VL_F16: return launch_batch_group<StoreF16>(b, tbase, t_local, gg);
+    default: return launch_batch_group<StoreF32>(b, tbase, t_local, gg);
+  }
 }
 
 vl_status batch_step(vl_batch* b, const int64_t* tbase, int t_local, int grid_bulk,
@@ -394,6 +573,34 @@ int bulk_grid_at(const vl_batch* b, int64_t t) {
   return it == b->ladder.end() ? b->ladder.back() : *it;
 }
 
+// The smallest value of the grid ladder (1, then about 9 % steps) that is >= need, capped at all.
+int ladder_ceil(int64_t need, int64_t all) {
+  all = std::max<int64_t>(all, 1);
+  int64_t v = 1;
+  while (v < need && v < all) v = std::max(v + 1, (v * 109 + 99) / 100);
+  return (int)std::min(v, all);
+}
+
+// Grids of the groups of K steps starting at t, t + K, ..., t + (groups - 1) K.  The pass must
+// cover every task it may run (its count only shrinks with t, so the count at t covers the
+// span); the listed and bulk kernels loop over their tiles, so their grids only set the speed.
+GroupGrids group_grids_at(const vl_batch* b, int64_t t, int groups) {
+  const int K = b->pass_k, W = b->pass_waves;
+  const int64_t last = (int64_t)b->h_pn_at.size() - 1;
+  auto at = [&](const std::vector<int32_t>& v, int64_t i) { return v[(size_t)std::min(std::max<int64_t>(i, 0), last)]; };
+  GroupGrids gg;
+  gg.pass = ladder_ceil((at(b->h_pn_at, t) + W - 1) / W, (b->npass_tasks + W - 1) / W);
+  gg.listed = ladder_ceil(at(b->h_ln_at, t), b->nlisted_tiles);
+  int64_t bulk = 1;
+  for (int g = 0; g < groups; ++g)
+    for (int j = 0; j < K; ++j) {
+      const int64_t tg = t + (int64_t)g * K;
+      bulk = std::max<int64_t>(bulk, (int64_t)at(b->h_kact, tg + j) - at(b->h_kfirst_at, tg));
+    }
+  gg.bulk = ladder_ceil(bulk, b->nbulk_tiles);
+  return gg;
+}
+
 // Adds the kernel and step times of the first n steps' events to the running totals.
 vl_status read_timing(vl_batch* b, int n) {
   VL_HIP(hipStreamSynchronize(b->stream));
@@ -421,7 +628,7 @@ vl_status launch_trial_step(vl_batch* b, int grid, int tw_log2, const int32_t* k
                             const int32_t* kx, const int32_t* kact) {
   using T = typename S::T;
   const T* A = (const T*)b->pool[b->cur];
-  T* B = (T*)b->pool[b->cur ^ 1];
+  T* B = (T*)b->pool[(b->cur + 1) % b->nbuf];
   const int64_t* t0 = b->d_tbase + 1;  // a zero
   hipStream_t bs = b->serial ? b->stream : b->side;
   if (!b->serial) {
@@ -432,7 +639,7 @@ vl_status launch_trial_step(vl_batch* b, int grid, int tw_log2, const int32_t* k
                      b->d_bb0, b->d_bact, t0, 0, b->d_kick, b->d_hist, b->d_status, b->d_done, SQRT_CS2_B);
   if (!b->serial) VL_HIP(hipEventRecord(b->ev_join, b->side));
   hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)grid), dim3(BT), 0, b->stream, A, B, b->dflags, b->dcases, kc,
-                     ky, kx, kact, t0, 0, tw_log2, b->d_status);
+                     ky, kx, kact, t0, 0, tw_log2, b->d_status, (const int32_t*)nullptr, 0);
   if (!b->serial) VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
   VL_HIP(hipGetLastError());
   return VL_OK;
@@ -454,8 +661,8 @@ void free_batch(vl_batch* b) {
   for (auto& p : b->pending)
     if (p.body) vl_shape_free(p.body);
   for (void* p : b->allocs) (void)hipFree(p);
-  if (b->pool[0]) (void)hipFree(b->pool[0]);
-  if (b->pool[1]) (void)hipFree(b->pool[1]);
+  for (void* p : b->pool)
+    if (p) (void)hipFree(p);
   for (auto& kv : b->graphs)
     if (kv.second) (void)hipGraphExecDestroy(kv.second);
   for (auto& e : b->tev)
@@ -537,13 +744,44 @@ vl_status build(vl_batch* b) {
     b->max_steps = std::max(b->max_steps, p.steps);
     (void)inlet;
   }
+  // ---- K-step passes: the plan of every case (its strips' tasks and listed nodes) ----
+  // Defaults: FP32 5 steps per pass (the 512-case sweep on MI355X: 93 GLUPS against 87 at 4 and
+  // 67 in single steps), FP64 4, FP16 3; tasks of 20 (FP32), 16 (FP64) or 24 (FP16) rows.
+  // VL_BATCH_PASS_STEPS (1 = single steps only) and VL_BATCH_TB_ROWS override.
+  b->pass_k = b->precision == VL_F16 ? 3 : (b->precision == VL_F32 ? 5 : 4);
+  if (const char* e = std::getenv("VL_BATCH_PASS_STEPS"))
+    b->pass_k = std::min(std::max(std::atoi(e), 1), b->precision == VL_F32 ? TB_MAX_K : 4);
+  b->pass_rows = b->precision == VL_F16 ? 24 : (b->precision == VL_F32 ? 20 : 16);
+  if (const char* e = std::getenv("VL_BATCH_TB_ROWS")) b->pass_rows = std::max(1, std::atoi(e));
+  b->pass_waves = b->precision == VL_F64 ? 4 : 8;
+  std::vector<PassPlan> plans;
+  if (b->pass_k > 1) {
+    const int K = b->pass_k, V = b->precision == VL_F16 ? 2 : 1, halo = tb_halo(K, V);
+    int x0_grid = std::max(32, halo);
+    x0_grid += (x0_grid - halo) % V;  // the strip's first column is a multiple of V
+    plans.assign(nc, PassPlan{});
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nth; ++w)
+      pool.emplace_back([&, w] {
+        for (size_t i = w; i < nc; i += nth) {
+          const Tables& tab = b->tabs[i];
+          st[i] = plan_passes(tab.flag, (int)tab.nx, (int)tab.ny, b->hcases[i].g.pitch, K, V, b->pass_rows, x0_grid,
+                              plans[i]);
+          if (st[i] != VL_OK) err[i] = vl_last_error();
+        }
+      });
+    for (auto& th : pool) th.join();
+    for (size_t i = 0; i < nc; ++i)
+      if (st[i] != VL_OK) return set_error(st[i], "case " + std::to_string(i) + ": " + err[i]);
+  }
+  b->nbuf = b->pass_k > 1 ? b->pass_k + 1 : 2;
   VL_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   VL_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
   VL_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
   VL_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
-  VL_HIP(hipMalloc(&b->pool[0], (size_t)pop_elems * b->elem));
-  VL_HIP(hipMalloc(&b->pool[1], (size_t)pop_elems * b->elem));
-  b->device_bytes += 2 * pop_elems * (int64_t)b->elem;
+  for (int j = 0; j < b->nbuf; ++j) VL_HIP(hipMalloc(&b->pool[j], (size_t)pop_elems * b->elem));
+  b->device_bytes += b->nbuf * pop_elems * (int64_t)b->elem;
   // flags (pitched; padding columns are "not fluid")
   {
     std::vector<uint8_t> fl((size_t)flag_bytes, 2);
@@ -697,22 +935,26 @@ vl_status build(vl_batch* b) {
   VL_HIP(hipMemset(b->d_hist, 0, (size_t)std::max<int64_t>(hist_len, 2) * sizeof(double)));
   if (kick_tab.empty()) kick_tab.push_back(0.0);
   VL_TRYB(dupload(b, b->d_kick, kick_tab));
-  // ---- initial state ----
-  for (size_t i = 0; i < nc; ++i) {
-    switch (b->precision) {
-      case VL_F64:
-        hipLaunchKernelGGL(k_batch_init<StoreF64>, dim3(1024), dim3(BT), 0, b->stream, (double*)b->pool[0],
-                           (double*)b->pool[1], b->dflags, b->dcases, (int)i);
-        break;
-      case VL_F16:
-        hipLaunchKernelGGL(k_batch_init<StoreF16>, dim3(1024), dim3(BT), 0, b->stream, (_Float16*)b->pool[0],
-                           (_Float16*)b->pool[1], b->dflags, b->dcases, (int)i);
-        break;
-      default:
-        hipLaunchKernelGGL(k_batch_init<StoreF32>, dim3(1024), dim3(BT), 0, b->stream, (float*)b->pool[0],
-                           (float*)b->pool[1], b->dflags, b->dcases, (int)i);
+  // ---- initial state, in every pool (two at a time) ----
+  for (int j = 0; j < b->nbuf; j += 2) {
+    void* pa = b->pool[j];
+    void* pb = b->pool[std::min(j + 1, b->nbuf - 1)];
+    for (size_t i = 0; i < nc; ++i) {
+      switch (b->precision) {
+        case VL_F64:
+          hipLaunchKernelGGL(k_batch_init<StoreF64>, dim3(1024), dim3(BT), 0, b->stream, (double*)pa, (double*)pb,
+                             b->dflags, b->dcases, (int)i);
+          break;
+        case VL_F16:
+          hipLaunchKernelGGL(k_batch_init<StoreF16>, dim3(1024), dim3(BT), 0, b->stream, (_Float16*)pa,
+                             (_Float16*)pb, b->dflags, b->dcases, (int)i);
+          break;
+        default:
+          hipLaunchKernelGGL(k_batch_init<StoreF32>, dim3(1024), dim3(BT), 0, b->stream, (float*)pa, (float*)pb,
+                             b->dflags, b->dcases, (int)i);
+      }
+      VL_HIP(hipGetLastError());
     }
-    VL_HIP(hipGetLastError());
   }
   // pressure-node velocities of the initial state: the boundary update writes them from then on
   for (size_t i = 0; i < nc; ++i) {
@@ -804,6 +1046,65 @@ vl_status build(vl_batch* b) {
   VL_TRYB(dupload(b, b->d_kx0, kx));
   VL_TRYB(dupload(b, b->d_kact, kact));
   b->h_kact = std::move(kact);
+  // ---- pass tasks and listed tiles, cases in descending step order ----
+  {
+    std::vector<int4> ptask;
+    std::vector<int32_t> ptcase, lcase, lfirst, lcount;
+    std::vector<uint32_t> lnodes;
+    std::vector<int32_t> pend(nc, 0), lend(nc, 0);
+    if (b->pass_k > 1)
+      for (int ci : order) {
+        const PassPlan& pl = plans[(size_t)ci];
+        for (const int4& tk : pl.tasks) {
+          ptask.push_back(tk);
+          ptcase.push_back(ci);
+        }
+        for (size_t k = 0; k < pl.listed.size(); k += BT) {
+          lcase.push_back(ci);
+          lfirst.push_back((int32_t)(lnodes.size() + k));
+          lcount.push_back((int32_t)std::min<size_t>(BT, pl.listed.size() - k));
+        }
+        lnodes.insert(lnodes.end(), pl.listed.begin(), pl.listed.end());
+        pend[(size_t)ci] = (int32_t)ptask.size();
+        lend[(size_t)ci] = (int32_t)lcase.size();
+        b->pass_nodes += pl.covered;
+        b->listed_nodes += (int64_t)pl.listed.size();
+      }
+    // Lockstep: the waves of a workgroup sweep in step when its tasks belong to one case and
+    // cover the same rows (so they meet the same barriers, and a retired case stops them all).
+    const size_t W = (size_t)b->pass_waves;
+    for (size_t g = 0; g + W <= ptask.size(); g += W) {
+      bool same = true;
+      for (size_t i = 1; i < W; ++i)
+        same = same && ptcase[g + i] == ptcase[g] && ptask[g + i].y == ptask[g].y && ptask[g + i].z == ptask[g].z;
+      if (same)
+        for (size_t i = 0; i < W; ++i) ptask[g + i].w |= 1 << 12;
+    }
+    // per group start t: the tasks, listed tiles and bulk tiles of the cases with steps >= t + K
+    const size_t nt = (size_t)b->max_steps + 2;
+    b->h_pn_at.assign(nt, 0);
+    b->h_ln_at.assign(nt, 0);
+    b->h_kfirst_at.assign(nt, 0);
+    if (b->pass_k > 1)
+      for (size_t tt = 0; tt < nt; ++tt)
+        for (int ci : order) {
+          if (b->hcases[(size_t)ci].steps < (int64_t)tt + b->pass_k) break;
+          b->h_pn_at[tt] = pend[(size_t)ci];
+          b->h_ln_at[tt] = lend[(size_t)ci];
+          b->h_kfirst_at[tt] = kend[(size_t)ci];
+        }
+    b->npass_tasks = (int)ptask.size();
+    b->nlisted_tiles = (int)lcase.size();
+    VL_TRYB(dupload(b, b->d_ptask, ptask));
+    VL_TRYB(dupload(b, b->d_ptcase, ptcase));
+    VL_TRYB(dupload(b, b->d_lcase, lcase));
+    VL_TRYB(dupload(b, b->d_lfirst, lfirst));
+    VL_TRYB(dupload(b, b->d_lcount, lcount));
+    VL_TRYB(dupload(b, b->d_lnodes, lnodes));
+    VL_TRYB(dupload(b, b->d_pn_at, b->h_pn_at));
+    VL_TRYB(dupload(b, b->d_ln_at, b->h_ln_at));
+    VL_TRYB(dupload(b, b->d_kfirst_at, b->h_kfirst_at));
+  }
   b->use_graphs = true;
   if (const char* e = std::getenv("VL_GRAPHS")) b->use_graphs = e[0] != '0';
 
@@ -811,9 +1112,15 @@ vl_status build(vl_batch* b) {
   return VL_OK;
 }
 
+// Steps per replayed window: GRAPH_W single steps, or the whole groups of K steps that fit.
+int window_steps(const vl_batch* b) { return b->pass_k > 1 ? GRAPH_W / b->pass_k * b->pass_k : GRAPH_W; }
+
 vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
-  const int grid = bulk_grid_at(b, b->t);
-  const auto key = std::make_pair(b->cur & 1, grid);
+  const int wsteps = window_steps(b);
+  const int K = b->pass_k;
+  GroupGrids gg{0, 0, 0};
+  const int grid = K > 1 ? (gg = group_grids_at(b, b->t, wsteps / K)).bulk : bulk_grid_at(b, b->t);
+  const auto key = std::make_tuple(b->cur, grid, gg.pass, gg.listed);
   auto found = b->graphs.find(key);
   if (found != b->graphs.end()) {
     *out = found->second;
@@ -824,8 +1131,11 @@ vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
   hipGraph_t g = nullptr;
   VL_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   vl_status st = VL_OK;
-  for (int k = 0; k < GRAPH_W && st == VL_OK; ++k) st = batch_step(b, b->d_tbase, k, grid);
-  if (st == VL_OK) hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, b->stream, b->d_tbase, (int64_t)GRAPH_W);
+  if (K > 1)
+    for (int k = 0; k < wsteps && st == VL_OK; k += K) st = batch_group(b, b->d_tbase, k, gg);
+  else
+    for (int k = 0; k < wsteps && st == VL_OK; ++k) st = batch_step(b, b->d_tbase, k, grid);
+  if (st == VL_OK) hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, b->stream, b->d_tbase, (int64_t)wsteps);
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   b->cur = cur0;
   b->t = t0;
@@ -906,23 +1216,30 @@ vl_status vl_batch_run(vl_batch* b, int64_t max_steps) {
     }
     k = todo;
   }
-  if (b->use_graphs && todo - k >= GRAPH_W) {
+  const int wsteps = window_steps(b);
+  if (b->use_graphs && todo - k >= wsteps) {
     // the window counter holds the global step of the window's first step
     VL_HIP(hipMemcpyAsync(b->d_tbase, &b->t, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
     VL_HIP(hipStreamSynchronize(b->stream));
-    for (; k + GRAPH_W <= todo; k += GRAPH_W) {
+    for (; k + wsteps <= todo; k += wsteps) {
       hipGraphExec_t ge;
       VL_TRYB(window_graph(b, &ge));
       VL_HIP(hipGraphLaunch(ge, b->stream));
-      b->t += GRAPH_W;
+      b->t += wsteps;
+      b->cur = (b->cur + wsteps) % b->nbuf;
     }
   }
   if (k < todo) {
-    // remaining steps launched directly; their base is the global step in a device scalar
+    // remaining steps launched directly (whole groups of K steps first); their base is the
+    // global step in a device scalar
     VL_HIP(hipStreamSynchronize(b->stream));
     VL_HIP(hipMemcpyAsync(b->d_tbase, &b->t, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
     VL_HIP(hipStreamSynchronize(b->stream));
-    for (int kk = 0; k < todo; ++k, ++kk) VL_TRYB(batch_step(b, b->d_tbase, kk, bulk_grid_at(b, b->t)));
+    int kk = 0;
+    if (b->pass_k > 1)
+      for (; k + b->pass_k <= todo; k += b->pass_k, kk += b->pass_k)
+        VL_TRYB(batch_group(b, b->d_tbase, kk, group_grids_at(b, b->t, 1)));
+    for (; k < todo; ++k, ++kk) VL_TRYB(batch_step(b, b->d_tbase, kk, bulk_grid_at(b, b->t)));
   }
   VL_HIP(hipStreamSynchronize(b->stream));
   b->last_run_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -975,6 +1292,12 @@ vl_status vl_batch_info_get(vl_batch* b, vl_batch_info* o) {
   o->device_bytes = b->device_bytes;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 4; ++j) o->autotune_ms[i][j] = b->autotune_ms[i][j];
+  o->pass_steps = b->pass_k;
+  o->pass_rows = b->pass_k > 1 ? b->pass_rows : 0;
+  o->pass_tasks = b->npass_tasks;
+  o->listed_tiles = b->nlisted_tiles;
+  o->pass_nodes = b->pass_nodes;
+  o->listed_nodes = b->listed_nodes;
   return VL_OK;
 }
 
@@ -1009,10 +1332,12 @@ vl_status vl_batch_read_case(vl_batch* b, int id, double* f) {
   VL_HIP(hipStreamSynchronize(b->stream));
   const BatchCase& c = b->hcases[id];
   // A case stops updating after its own step count while the batch goes on, so its latest
-  // populations sit in the pool of its own update parity (every case starts in pool 0).
+  // populations sit in the pool its last update wrote: step k writes pool (k + 1) mod nbuf, and
+  // every case starts in pool 0.  (A run ends between groups of K steps, where every node of a
+  // case in passes is up to date.)
   const int64_t updates = std::min<int64_t>(b->t, c.steps);
   std::vector<char> host((size_t)(Q * c.g.plane) * b->elem);
-  VL_HIP(hipMemcpy(host.data(), (char*)b->pool[updates & 1] + (size_t)c.pop_off * b->elem, host.size(),
+  VL_HIP(hipMemcpy(host.data(), (char*)b->pool[updates % b->nbuf] + (size_t)c.pop_off * b->elem, host.size(),
                    hipMemcpyDeviceToHost));
   const int64_t nx = c.g.nx, ny = c.g.ny, n = nx * ny;
   for (int i = 0; i < Q; ++i)

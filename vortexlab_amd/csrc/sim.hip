@@ -504,16 +504,18 @@ static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 // Tasks are ordered by (first row, first column) so a workgroup's waves sweep adjacent strips.
 // Every interior node no task covers goes to the single-step list; a covered node is a ring node
 // when one of its eight neighbours is a fluid node that no task covers, and a task marks each of
-// its four edges that holds a ring node.
-static vl_status build_passes(vl_sim* s, int rows) {
-  const int nx = s->g.nx, ny = s->g.ny;
-  const std::vector<uint8_t>& fl = s->tab.flag;
-  if (nx >= 65536 || ny >= 65536 || (int64_t)s->g.pitch * ny >= ((int64_t)1 << 31) || rows < 1)
+// its four edges that holds a ring node.  Far tasks (below) come first; the caller sets the
+// lockstep bits, which depend on how it groups tasks into workgroups.
+vl_status plan_passes(const std::vector<uint8_t>& fl, int nx, int ny, int64_t pitch, int K, int V, int rows,
+                      int x0_grid, PassPlan& plan) {
+  if (nx >= 65536 || ny >= 65536 || pitch * ny >= ((int64_t)1 << 31) || rows < 1 || K < 2 || K > TB_MAX_K ||
+      (V != 1 && V != 2))
     return set_error(VL_EINVAL, "K-step passes need nx, ny < 65536, a plane below 2^31 nodes and rows >= 1");
-  std::vector<int4> tasks;
+  std::vector<int4>& tasks = plan.tasks;
+  tasks.clear();
   std::vector<uint8_t> covered((size_t)nx * ny, 0);
   std::vector<uint8_t> usable((size_t)ny);
-  const int V = s->tb_v, K = s->tb_k, halo = tb_halo(K, V), out = tb_out(K, V);
+  const int halo = tb_halo(K, V), out = tb_out(K, V);
   // Columns interior in at least half the rows: the strips' footprints stay within
   // [xa, xb], so that a column of boundary nodes at an inlet or outlet does not disqualify a
   // whole strip.  The strips are the aligned grid from tb_x0, a narrower first strip ending at
@@ -532,11 +534,11 @@ static vl_status build_passes(vl_sim* s, int rows) {
   {
     int x_first = std::max(halo, xa + K - 1);
     x_first += (V - (x_first - halo) % V) % V;  // the strip's first column is a multiple of V
-    if (x_first < s->tb_x0) {
-      const int w = std::min({out, s->tb_x0 - x_first, xb - (K - 1) - x_first + 1}) / V * V;
+    if (x_first < x0_grid) {
+      const int w = std::min({out, x0_grid - x_first, xb - (K - 1) - x_first + 1}) / V * V;
       if (w >= 1) strips.emplace_back(x_first, w);
     }
-    for (int x0 = std::max(s->tb_x0, x_first); x0 < nx; x0 += out) {
+    for (int x0 = std::max(x0_grid, x_first); x0 < nx; x0 += out) {
       const int w = std::min({out, nx - K - x0, xb - (K - 1) - x0 + 1}) / V * V;
       if (w < 1) break;
       strips.emplace_back(x0, w);
@@ -569,7 +571,8 @@ static vl_status build_passes(vl_sim* s, int rows) {
   }
   std::sort(tasks.begin(), tasks.end(),
             [](const int4& a, const int4& b) { return a.y != b.y ? a.y < b.y : a.x < b.x; });
-  std::vector<uint32_t> listed;
+  std::vector<uint32_t>& listed = plan.listed;
+  listed.clear();
   std::vector<uint8_t> ring((size_t)nx * ny, 0);
   int64_t ncov = 0;
   for (int y = 0; y < ny; ++y)
@@ -633,7 +636,18 @@ static vl_status build_passes(vl_sim* s, int rows) {
     far_nodes += (int64_t)w * tk.z;
     ++nfar;
   }
-  s->tb_far_nodes = far_nodes;
+  plan.nfar = nfar;
+  plan.far_nodes = far_nodes;
+  plan.covered = ncov;
+  return VL_OK;
+}
+
+static vl_status build_passes(vl_sim* s, int rows) {
+  PassPlan plan;
+  VL_TRY(plan_passes(s->tab.flag, s->g.nx, s->g.ny, s->g.pitch, s->tb_k, s->tb_v, rows, s->tb_x0, plan));
+  std::vector<int4>& tasks = plan.tasks;
+  const int nfar = plan.nfar;
+  s->tb_far_nodes = plan.far_nodes;
   // Lockstep bit: set on the tasks of a workgroup when they cover the same rows, so that its
   // waves execute the same number of barriers.  Far and near tasks are launched separately,
   // each set in workgroups of its own.
@@ -651,10 +665,10 @@ static vl_status build_passes(vl_sim* s, int rows) {
   s->tb_rows = rows;
   s->ntasks = (int)tasks.size();
   s->ntasks_far = nfar;
-  s->nlisted = (int)listed.size();
-  s->tb_nodes = ncov;
+  s->nlisted = (int)plan.listed.size();
+  s->tb_nodes = plan.covered;
   VL_TRY(upload(s, s->dtasks, tasks));
-  VL_TRY(upload(s, s->dlisted, listed));
+  VL_TRY(upload(s, s->dlisted, plan.listed));
   return VL_OK;
 }
 

@@ -57,6 +57,20 @@ struct DevTables {
   double* pvel = nullptr;      // 2 steps x np outward velocities of the outlet-controller nodes
 };
 
+// K-step pass plan of one domain (plan_passes in sim.hip): the strips' tasks (first output
+// column x, first output row y, rows z, output columns | ring-edge bits << 8 in w), far tasks
+// first; the interior nodes that no task covers (y << 16 | x); the covered node counts.
+struct PassPlan {
+  std::vector<int4> tasks;
+  std::vector<uint32_t> listed;
+  int nfar = 0;
+  int64_t covered = 0, far_nodes = 0;
+};
+// flags: nx * ny bytes, 0 interior, 1 boundary, 2 solid; K steps per pass, V columns per lane,
+// tasks of at most `rows` output rows, strips aligned from column x0_grid.
+vl_status plan_passes(const std::vector<uint8_t>& flags, int nx, int ny, int64_t pitch, int K, int V, int rows,
+                      int x0_grid, PassPlan& plan);
+
 }  // namespace vl
 
 struct vl_sim {

@@ -146,7 +146,9 @@ def main(argv=None):
            "glups": round(info.lattice_updates / t_run / 1e9, 2), "diverged": int(sum(1 for s in status if s)),
            "tile": f"{info.tile_w}x{info.tile_h}", "grid_bulk": int(info.grid_bulk),
            "grid_boundary": int(info.grid_boundary), "boundary_tiles": int(info.boundary_tiles),
-           "bulk_tiles": int(info.bulk_tiles)}
+           "bulk_tiles": int(info.bulk_tiles), "pass_steps": int(info.pass_steps),
+           "pass_tasks": int(info.pass_tasks), "pass_nodes": int(info.pass_nodes),
+           "listed_nodes": int(info.listed_nodes), "device_gb": round(info.device_bytes / 1e9, 2)}
     b.close()
     print(json.dumps(out), file=sys.stdout, flush=True)
 
