@@ -159,21 +159,18 @@ def wide_cases(seed, n=3):
     return cases
 
 
-@pytest.mark.parametrize("prec,k,rows,overlap", [(api.F32, 1, 20, 0), (api.F32, 2, 20, 0), (api.F32, 3, 7, 1),
-                                                 (api.F32, 4, 3, 0), (api.F32, 4, 20, 1), (api.F32, 5, 20, 0),
-                                                 (api.F32, 5, 20, 1), (api.F16, 2, 24, 1), (api.F16, 4, 5, 0),
-                                                 (api.F64, 3, 16, 0)])
-def test_batch_passes_equal_single_case_runs(gpu, monkeypatch, prec, k, rows, overlap):
-    """K-step passes in the batch (VL_BATCH_PASS_STEPS=K; 1 = single steps only), overlapped
-    across groups or not (VL_BATCH_OVERLAP): every case, whether it runs its last steps in
-    groups of K or alone in single steps, ends with the coefficient history and populations of
-    the same case run alone."""
+@pytest.mark.parametrize("prec,k,rows", [(api.F32, 1, 20), (api.F32, 2, 20), (api.F32, 3, 7), (api.F32, 4, 3),
+                                         (api.F32, 4, 20), (api.F32, 5, 20), (api.F16, 2, 24), (api.F16, 4, 5),
+                                         (api.F64, 3, 16)])
+def test_batch_passes_equal_single_case_runs(gpu, monkeypatch, prec, k, rows):
+    """K-step passes in the batch (VL_BATCH_PASS_STEPS=K; 1 = single steps only), BGK and TRT
+    cases in passes of their own: every case, whether it runs its last steps in groups of K or
+    alone in single steps, ends with the coefficient history and populations of the same case
+    run alone."""
     monkeypatch.setenv("VL_BATCH_PASS_STEPS", str(k))
     monkeypatch.setenv("VL_BATCH_TB_ROWS", str(rows))
-    monkeypatch.setenv("VL_BATCH_OVERLAP", str(overlap))
     cases = wide_cases(seed=21 + k)
-    print(f"[batch] passes K={k} rows={rows} overlap={overlap} prec={prec} steps={[c.steps for c in cases]}",
-          flush=True)
+    print(f"[batch] passes K={k} rows={rows} prec={prec} steps={[c.steps for c in cases]}", flush=True)
     b = run_batch(gpu, cases, prec)
     info = b.info()
     assert info.pass_steps == k
@@ -181,7 +178,7 @@ def test_batch_passes_equal_single_case_runs(gpu, monkeypatch, prec, k, rows, ov
         assert info.pass_tasks > 0 and info.pass_nodes > 0.5 * info.fluid_nodes
     else:
         assert info.pass_tasks == 0
-    for name in ("VL_BATCH_PASS_STEPS", "VL_BATCH_TB_ROWS", "VL_BATCH_OVERLAP"):
+    for name in ("VL_BATCH_PASS_STEPS", "VL_BATCH_TB_ROWS"):
         monkeypatch.delenv(name)
     for j, c in enumerate(cases):
         coef, done, status = b.result(j)

@@ -169,6 +169,9 @@ __global__ __launch_bounds__(BT) void k_batch_bulk(const typename S::T* __restri
 // K steps (t .. t+K) of the covered interior of the cases that run the whole group: one task
 // per wave, a strip of one case (task_case), on the case's arrays at its offset in each pool.
 // The cases are in descending step order, so those tasks are a prefix of ntask_at[t] tasks.
+// The wave-uniform branch on the case's model inlines the strip once per collide; it needs
+// more registers (3 waves per SIMD for FP32 at K = 5) than a branch inside each collide (4),
+// and still ran the 512-case sweep 5 % faster.
 template <class S, int W, int V, int D, int K>
 __global__ __launch_bounds__(W * TB_LANES) void k_batch_pass(const typename S::T* __restrict__ A,
                                                             PassOut<typename S::T> po, const int4* __restrict__ tasks,
@@ -296,8 +299,8 @@ struct vl_batch {
   std::vector<Tables> tabs;
   int64_t total_nodes = 0, total_fluid = 0, total_lu = 0, max_steps = 0;
   // device
-  hipStream_t stream = nullptr, side = nullptr, aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_aux[2] = {};
+  hipStream_t stream = nullptr, side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // population pools: 2, or K + 1 with K-step passes (pool j + 1 follows pool j in time)
   void* pool[TB_MAX_K + 1] = {};
   int nbuf = 2;
@@ -307,29 +310,22 @@ struct vl_batch {
   int pass_k = 1, pass_rows = 0, pass_waves = 8;
   int nlisted_tiles = 0;
   int64_t pass_nodes = 0, listed_nodes = 0;
-  // Far tasks depend on covered nodes only; near tasks have an uncovered node within K of
-  // their outputs and hold every ring node; `all` holds both, each case's tasks together (the
-  // passes of groups that do not overlap).  Per group start t, n_at[t] = the tasks of the cases
-  // with steps >= t + K (a prefix).
+  // Pass tasks, each case's together (BGK and TRT cases alike: the pass kernel branches on
+  // the case's model).  Per group start t, n_at[t] = the tasks of the cases with steps >= t + K
+  // (a prefix).
   struct TaskList {
     int4* task = nullptr;
     int32_t* tcase = nullptr;
     int32_t* n_at = nullptr;
     int n = 0;
     std::vector<int32_t> h_n_at;
-  } far, near, all;
+  } tasks;
   int32_t *d_kfirst_at = nullptr;
   int32_t *d_lcase = nullptr, *d_lfirst = nullptr, *d_lcount = nullptr, *d_ln_at = nullptr;
   uint32_t* d_lnodes = nullptr;
   // per group start t: listed tiles of the cases with steps >= t + K, and the bulk tiles of
   // those cases (the bulk kernel skips them in a group)
   std::vector<int32_t> h_ln_at, h_kfirst_at;
-  // Overlap of groups (pass_overlap): the far pass of group g + 1 runs on the main stream while
-  // the side stream finishes group g's near pass and single-step phases.  ov_open: the side
-  // stream has forked and not joined back; ov_parity picks the event pair of the group.
-  bool pass_overlap = false, ov_open = false;
-  int ov_parity = 0;
-  hipEvent_t ev_far[2] = {}, ev_near[2] = {};
   uint8_t* dflags = nullptr;
   BatchCase* dcases = nullptr;
   std::vector<void*> allocs;  // table pools etc.
@@ -350,8 +346,8 @@ struct vl_batch {
   std::vector<int32_t> ladder;  // grid sizes used when bulk_mult == 0
   int cur = 0;
   int64_t t = 0;
-  // (first pool, bulk grid, far-pass grid, near-pass grid, listed grid) -> window
-  std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
+  // (first pool, bulk grid, pass grid, listed grid) -> window
+  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
   bool use_graphs = true;
   double autotune_ms[3][4] = {};
   double last_run_seconds = 0.0;
@@ -471,40 +467,40 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, int 
   return VL_OK;
 }
 
-// Grids of a group of K steps (workgroups): the far and near passes, the listed tiles, the
-// bulk tiles.
+// Grids of a group of K steps (workgroups): the pass, the listed tiles, the bulk tiles.
 struct GroupGrids {
-  int far, near, all, listed, bulk;
+  int pass, listed, bulk;
 };
 
 template <class S, int W, int V, int D, int K>
-void launch_batch_pass(vl_batch* b, hipStream_t st, const vl_batch::TaskList& tl, const typename S::T* A,
-                       PassOut<typename S::T> po, const int64_t* tbase, int t_local, int grid) {
-  hipLaunchKernelGGL((k_batch_pass<S, W, V, D, K>), dim3((unsigned)grid), dim3(W * TB_LANES), 0, st, A, po, tl.task,
-                     tl.tcase, tl.n_at, tbase, t_local, b->dcases, b->d_status);
+void launch_batch_pass(vl_batch* b, hipStream_t st, const typename S::T* A, PassOut<typename S::T> po,
+                       const int64_t* tbase, int t_local, int grid) {
+  const vl_batch::TaskList& tl = b->tasks;
+  hipLaunchKernelGGL((k_batch_pass<S, W, V, D, K>), dim3((unsigned)grid), dim3(W * TB_LANES), 0, st, A, po,
+                     tl.task, tl.tcase, tl.n_at, tbase, t_local, b->dcases, b->d_status);
 }
 
 // The pass kernel of the batch's K and precision: FP32 one column per lane (K = 2..5), FP16
 // two (K = 2..4), FP64 one, in 4-wave workgroups (K = 2..4); 8 waves per workgroup otherwise.
 template <class S>
-vl_status launch_pass_k(vl_batch* b, hipStream_t st, const vl_batch::TaskList& tl, const typename S::T* A,
-                        PassOut<typename S::T> po, const int64_t* tbase, int t_local, int grid) {
+vl_status launch_pass_k(vl_batch* b, hipStream_t st, const typename S::T* A, PassOut<typename S::T> po,
+                        const int64_t* tbase, int t_local, int grid) {
   const int K = b->pass_k;
   if constexpr (std::is_same<S, StoreF16>::value) {
-    if (K == 2) launch_batch_pass<S, 8, 2, 2, 2>(b, st, tl, A, po, tbase, t_local, grid);
-    else if (K == 3) launch_batch_pass<S, 8, 2, 2, 3>(b, st, tl, A, po, tbase, t_local, grid);
-    else if (K == 4) launch_batch_pass<S, 8, 2, 2, 4>(b, st, tl, A, po, tbase, t_local, grid);
+    if (K == 2) launch_batch_pass<S, 8, 2, 2, 2>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 3) launch_batch_pass<S, 8, 2, 2, 3>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 4) launch_batch_pass<S, 8, 2, 2, 4>(b, st, A, po, tbase, t_local, grid);
     else return set_error(VL_EINVAL, "FP16 batch passes take 2..4 steps");
   } else if constexpr (std::is_same<S, StoreF64>::value) {
-    if (K == 2) launch_batch_pass<S, 4, 1, 3, 2>(b, st, tl, A, po, tbase, t_local, grid);
-    else if (K == 3) launch_batch_pass<S, 4, 1, 3, 3>(b, st, tl, A, po, tbase, t_local, grid);
-    else if (K == 4) launch_batch_pass<S, 4, 1, 3, 4>(b, st, tl, A, po, tbase, t_local, grid);
+    if (K == 2) launch_batch_pass<S, 4, 1, 3, 2>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 3) launch_batch_pass<S, 4, 1, 3, 3>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 4) launch_batch_pass<S, 4, 1, 3, 4>(b, st, A, po, tbase, t_local, grid);
     else return set_error(VL_EINVAL, "FP64 batch passes take 2..4 steps");
   } else {
-    if (K == 2) launch_batch_pass<S, 8, 1, 3, 2>(b, st, tl, A, po, tbase, t_local, grid);
-    else if (K == 3) launch_batch_pass<S, 8, 1, 3, 3>(b, st, tl, A, po, tbase, t_local, grid);
-    else if (K == 4) launch_batch_pass<S, 8, 1, 3, 4>(b, st, tl, A, po, tbase, t_local, grid);
-    else if (K == 5) launch_batch_pass<S, 8, 1, 3, 5>(b, st, tl, A, po, tbase, t_local, grid);
+    if (K == 2) launch_batch_pass<S, 8, 1, 3, 2>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 3) launch_batch_pass<S, 8, 1, 3, 3>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 4) launch_batch_pass<S, 8, 1, 3, 4>(b, st, A, po, tbase, t_local, grid);
+    else if (K == 5) launch_batch_pass<S, 8, 1, 3, 5>(b, st, A, po, tbase, t_local, grid);
     else return set_error(VL_EINVAL, "FP32 batch passes take 2..5 steps");
   }
   return VL_OK;
@@ -526,29 +522,20 @@ void launch_group_phase(vl_batch* b, hipStream_t bs, hipStream_t ms, const typen
                      (const int32_t*)b->d_kfirst_at, j);
 }
 
-// Joins the side stream of overlapped groups back into the main stream.
-vl_status join_groups(vl_batch* b) {
-  if (!b->ov_open) return VL_OK;
-  VL_HIP(hipEventRecord(b->ev_join, b->side));
-  VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
-  b->ov_open = false;
-  return VL_OK;
-}
-
 // A group of K steps from t = *tbase + t_local, pools cur .. cur + K.  The passes take the
-// covered interior from t to t + K (pool cur to pool cur + K; the near tasks also write their
-// ring nodes' levels 1 .. K-1 into pools cur + 1 .. cur + K - 1); step j of the single-step part
-// reads pool cur + j and writes pool cur + j + 1, after the near pass, so that its boundary and
-// listed nodes find their covered neighbours' level j written.
-//  * Overlapped (VL_BATCH_OVERLAP=1): the far pass on the main stream; the near pass and the K phases
-//    in order on the side stream.  The far pass of group g + 1 waits only for the near pass of
-//    group g (its inputs are covered nodes, and it writes none that the phases of group g read),
-//    and the near pass of group g + 1 waits for the far pass of group g, whose inputs it
-//    overwrites.  join_groups ends the overlap.
-//  * Otherwise (the default: on the 512-case sweep the overlapped phases slowed the far pass
-//    more than they hid): step 0 runs one pass of all tasks, each case's together, beside its
-//    single-step part, and each later step its boundary kernel beside the listed and bulk
-//    kernels; every step joins before the next.
+// covered interior from t to t + K (pool cur to pool cur + K; tasks with ring nodes also write
+// their levels 1 .. K-1 into pools cur + 1 .. cur + K - 1); step j of the single-step part reads
+// pool cur + j and writes pool cur + j + 1, after the passes, so that its boundary and listed
+// nodes find their covered neighbours' level j written.
+//  * Step 0: the pass (main stream) beside step 0's single-step part (side stream).  The pass
+//    reads pool cur and writes covered nodes only; step 0's boundary and listed nodes read pool
+//    cur and write pool cur + 1.
+//  * Steps 1 .. K-1: the boundary kernel on the side stream beside the listed and bulk kernels.
+// Every step joins before the next.  (Measured alternatives, all slower on the 512-case sweep:
+// overlapping group g's single steps with group g + 1's passes of the tasks that depend on
+// covered nodes only, as the single-domain path does, since the batch's single-step kernels
+// slow the passes more than they hide; and one pass kernel per collision model, side by side
+// or one after the other, since each case's tasks then no longer run together.)
 template <class S>
 vl_status launch_batch_group(vl_batch* b, const int64_t* tbase, int t_local, const GroupGrids& gg) {
   using T = typename S::T;
@@ -557,48 +544,17 @@ vl_status launch_batch_group(vl_batch* b, const int64_t* tbase, int t_local, con
   PassOut<T> po{};
   for (int L = 1; L < K; ++L) po.ring[L - 1] = P(L);
   po.out = P(K);
-  if (b->pass_overlap && !b->serial) {
-    bool first = false;
-    if (!b->ov_open) {
-      VL_HIP(hipEventRecord(b->ev_fork, b->stream));
-      VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
-      b->ov_open = true;
-      first = true;
-    }
-    const int p = b->ov_parity;
-    if (!first) VL_HIP(hipStreamWaitEvent(b->side, b->ev_far[p ^ 1], 0));
-    VL_TRYB(launch_pass_k<S>(b, b->side, b->near, P(0), po, tbase, t_local, gg.near));
-    VL_HIP(hipEventRecord(b->ev_near[p], b->side));
-    // each step: the boundary kernel on the side stream beside the listed and bulk kernels on
-    // the third stream, joined back into the side stream before the next step
-    for (int j = 0; j < K; ++j) {
-      VL_HIP(hipEventRecord(b->ev_aux[0], b->side));
-      VL_HIP(hipStreamWaitEvent(b->aux, b->ev_aux[0], 0));
-      launch_group_phase<S>(b, b->side, b->aux, P(j), P(j + 1), tbase, t_local, j, gg);
-      VL_HIP(hipEventRecord(b->ev_aux[1], b->aux));
-      VL_HIP(hipStreamWaitEvent(b->side, b->ev_aux[1], 0));
-    }
-    if (!first) VL_HIP(hipStreamWaitEvent(b->stream, b->ev_near[p ^ 1], 0));
-    VL_TRYB(launch_pass_k<S>(b, b->stream, b->far, P(0), po, tbase, t_local, gg.far));
-    VL_HIP(hipEventRecord(b->ev_far[p], b->stream));
-    VL_HIP(hipGetLastError());
-    b->ov_parity ^= 1;
-  } else if (b->serial) {
-    VL_TRYB(launch_pass_k<S>(b, b->stream, b->all, P(0), po, tbase, t_local, gg.all));
+  if (b->serial) {
+    VL_TRYB((launch_pass_k<S>(b, b->stream, P(0), po, tbase, t_local, gg.pass)));
     for (int j = 0; j < K; ++j) launch_group_phase<S>(b, b->stream, b->stream, P(j), P(j + 1), tbase, t_local, j, gg);
     VL_HIP(hipGetLastError());
   } else {
-    // step 0: the pass of every task (main stream) beside the single-step part (side stream):
-    // the pass reads pool cur and writes covered nodes only, step 0's boundary and listed nodes
-    // read pool cur and write pool cur + 1
     VL_HIP(hipEventRecord(b->ev_fork, b->stream));
     VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
-    VL_TRYB(launch_pass_k<S>(b, b->stream, b->all, P(0), po, tbase, t_local, gg.all));
+    VL_TRYB((launch_pass_k<S>(b, b->stream, P(0), po, tbase, t_local, gg.pass)));
     launch_group_phase<S>(b, b->side, b->side, P(0), P(1), tbase, t_local, 0, gg);
     VL_HIP(hipEventRecord(b->ev_join, b->side));
     VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
-    // steps 1 .. K-1: the boundary kernel on the side stream beside the listed and bulk kernels,
-    // joined before the next step
     for (int j = 1; j < K; ++j) {
       VL_HIP(hipEventRecord(b->ev_fork, b->stream));
       VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
@@ -668,9 +624,7 @@ GroupGrids group_grids_at(const vl_batch* b, int64_t t, int groups) {
   const int64_t last = (int64_t)b->h_ln_at.size() - 1;
   auto at = [&](const std::vector<int32_t>& v, int64_t i) { return v[(size_t)std::min(std::max<int64_t>(i, 0), last)]; };
   GroupGrids gg;
-  gg.far = ladder_ceil((at(b->far.h_n_at, t) + W - 1) / W, (b->far.n + W - 1) / W);
-  gg.near = ladder_ceil((at(b->near.h_n_at, t) + W - 1) / W, (b->near.n + W - 1) / W);
-  gg.all = ladder_ceil((at(b->all.h_n_at, t) + W - 1) / W, (b->all.n + W - 1) / W);
+  gg.pass = ladder_ceil((at(b->tasks.h_n_at, t) + W - 1) / W, (b->tasks.n + W - 1) / W);
   gg.listed = ladder_ceil(at(b->h_ln_at, t), b->nlisted_tiles);
   int64_t bulk = 1;
   for (int g = 0; g < groups; ++g)
@@ -739,7 +693,6 @@ void free_batch(vl_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->side) (void)hipStreamSynchronize(b->side);
-  if (b->aux) (void)hipStreamSynchronize(b->aux);
   for (auto& p : b->pending)
     if (p.body) vl_shape_free(p.body);
   for (void* p : b->allocs) (void)hipFree(p);
@@ -752,13 +705,7 @@ void free_batch(vl_batch* b) {
       if (h) (void)hipEventDestroy(h);
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->ev_join) (void)hipEventDestroy(b->ev_join);
-  for (int i = 0; i < 2; ++i) {
-    if (b->ev_far[i]) (void)hipEventDestroy(b->ev_far[i]);
-    if (b->ev_near[i]) (void)hipEventDestroy(b->ev_near[i]);
-    if (b->ev_aux[i]) (void)hipEventDestroy(b->ev_aux[i]);
-  }
   if (b->side) (void)hipStreamDestroy(b->side);
-  if (b->aux) (void)hipStreamDestroy(b->aux);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -834,9 +781,10 @@ vl_status build(vl_batch* b) {
   }
   // ---- K-step passes: the plan of every case (its strips' tasks and listed nodes) ----
   // Defaults: FP32 5 steps per pass (the 512-case sweep on MI355X: 93 GLUPS against 87 at 4 and
-  // 67 in single steps), FP64 4, FP16 3; tasks of 20 (FP32), 16 (FP64) or 24 (FP16) rows.
-  // VL_BATCH_PASS_STEPS (1 = single steps only) and VL_BATCH_TB_ROWS override.
-  b->pass_k = b->precision == VL_F16 ? 3 : (b->precision == VL_F32 ? 5 : 4);
+  // 67 in single steps), FP64 3 (the 4-step FP64 pass of both collides needs all 256 registers),
+  // FP16 3; tasks of 20 (FP32), 16 (FP64) or 24 (FP16) rows.  VL_BATCH_PASS_STEPS (1 = single
+  // steps only) and VL_BATCH_TB_ROWS override.
+  b->pass_k = b->precision == VL_F32 ? 5 : 3;
   if (const char* e = std::getenv("VL_BATCH_PASS_STEPS"))
     b->pass_k = std::min(std::max(std::atoi(e), 1), b->precision == VL_F32 ? TB_MAX_K : 4);
   b->pass_rows = b->precision == VL_F16 ? 24 : (b->precision == VL_F32 ? 20 : 16);
@@ -1136,21 +1084,16 @@ vl_status build(vl_batch* b) {
   b->h_kact = std::move(kact);
   // ---- pass tasks (far and near lists) and listed tiles, cases in descending step order ----
   {
-    // lists 0 far, 1 near, 2 all
-    std::vector<int4> task[3];
-    std::vector<int32_t> tcase[3], lcase, lfirst, lcount;
+    std::vector<int4> task;
+    std::vector<int32_t> tcase, lcase, lfirst, lcount;
     std::vector<uint32_t> lnodes;
-    std::vector<int32_t> tend[3], lend(nc, 0);
-    for (auto& v : tend) v.assign(nc, 0);
+    std::vector<int32_t> tend(nc, 0), lend(nc, 0);
     if (b->pass_k > 1)
       for (int ci : order) {
         const PassPlan& pl = plans[(size_t)ci];
-        for (size_t i = 0; i < pl.tasks.size(); ++i) {
-          const int lst = (int)i < pl.nfar ? 0 : 1;
-          task[lst].push_back(pl.tasks[i]);
-          tcase[lst].push_back(ci);
-          task[2].push_back(pl.tasks[i]);
-          tcase[2].push_back(ci);
+        for (const int4& tk : pl.tasks) {
+          task.push_back(tk);
+          tcase.push_back(ci);
         }
         for (size_t k = 0; k < pl.listed.size(); k += BT) {
           lcase.push_back(ci);
@@ -1158,7 +1101,7 @@ vl_status build(vl_batch* b) {
           lcount.push_back((int32_t)std::min<size_t>(BT, pl.listed.size() - k));
         }
         lnodes.insert(lnodes.end(), pl.listed.begin(), pl.listed.end());
-        for (int lst = 0; lst < 3; ++lst) tend[lst][(size_t)ci] = (int32_t)task[lst].size();
+        tend[(size_t)ci] = (int32_t)task.size();
         lend[(size_t)ci] = (int32_t)lcase.size();
         b->pass_nodes += pl.covered;
         b->listed_nodes += (int64_t)pl.listed.size();
@@ -1166,39 +1109,31 @@ vl_status build(vl_batch* b) {
     // Lockstep: the waves of a workgroup sweep in step when its tasks belong to one case and
     // cover the same rows (so they meet the same barriers, and a retired case stops them all).
     const size_t W = (size_t)b->pass_waves;
-    for (int lst = 0; lst < 3; ++lst)
-      for (size_t g = 0; g + W <= task[lst].size(); g += W) {
-        bool same = true;
-        for (size_t i = 1; i < W; ++i)
-          same = same && tcase[lst][g + i] == tcase[lst][g] && task[lst][g + i].y == task[lst][g].y &&
-                 task[lst][g + i].z == task[lst][g].z;
-        if (same)
-          for (size_t i = 0; i < W; ++i) task[lst][g + i].w |= 1 << 12;
-      }
+    for (size_t g = 0; g + W <= task.size(); g += W) {
+      bool same = true;
+      for (size_t i = 1; i < W; ++i)
+        same = same && tcase[g + i] == tcase[g] && task[g + i].y == task[g].y && task[g + i].z == task[g].z;
+      if (same)
+        for (size_t i = 0; i < W; ++i) task[g + i].w |= 1 << 12;
+    }
     // per group start t: the tasks, listed tiles and bulk tiles of the cases with steps >= t + K
     const size_t nt = (size_t)b->max_steps + 2;
-    b->far.h_n_at.assign(nt, 0);
-    b->near.h_n_at.assign(nt, 0);
-    b->all.h_n_at.assign(nt, 0);
+    vl_batch::TaskList& tl = b->tasks;
+    tl.h_n_at.assign(nt, 0);
     b->h_ln_at.assign(nt, 0);
     b->h_kfirst_at.assign(nt, 0);
     if (b->pass_k > 1)
       for (size_t tt = 0; tt < nt; ++tt)
         for (int ci : order) {
           if (b->hcases[(size_t)ci].steps < (int64_t)tt + b->pass_k) break;
-          b->far.h_n_at[tt] = tend[0][(size_t)ci];
-          b->near.h_n_at[tt] = tend[1][(size_t)ci];
-          b->all.h_n_at[tt] = tend[2][(size_t)ci];
+          tl.h_n_at[tt] = tend[(size_t)ci];
           b->h_ln_at[tt] = lend[(size_t)ci];
           b->h_kfirst_at[tt] = kend[(size_t)ci];
         }
-    vl_batch::TaskList* lists[3] = {&b->far, &b->near, &b->all};
-    for (int lst = 0; lst < 3; ++lst) {
-      lists[lst]->n = (int)task[lst].size();
-      VL_TRYB(dupload(b, lists[lst]->task, task[lst]));
-      VL_TRYB(dupload(b, lists[lst]->tcase, tcase[lst]));
-      VL_TRYB(dupload(b, lists[lst]->n_at, lists[lst]->h_n_at));
-    }
+    tl.n = (int)task.size();
+    VL_TRYB(dupload(b, tl.task, task));
+    VL_TRYB(dupload(b, tl.tcase, tcase));
+    VL_TRYB(dupload(b, tl.n_at, tl.h_n_at));
     b->nlisted_tiles = (int)lcase.size();
     VL_TRYB(dupload(b, b->d_lcase, lcase));
     VL_TRYB(dupload(b, b->d_lfirst, lfirst));
@@ -1206,15 +1141,6 @@ vl_status build(vl_batch* b) {
     VL_TRYB(dupload(b, b->d_lnodes, lnodes));
     VL_TRYB(dupload(b, b->d_ln_at, b->h_ln_at));
     VL_TRYB(dupload(b, b->d_kfirst_at, b->h_kfirst_at));
-    if (b->pass_k > 1) {
-      if (const char* e = std::getenv("VL_BATCH_OVERLAP")) b->pass_overlap = e[0] != '0';
-      for (int i = 0; i < 2; ++i) {
-        VL_HIP(hipEventCreateWithFlags(&b->ev_far[i], hipEventDisableTiming));
-        VL_HIP(hipEventCreateWithFlags(&b->ev_near[i], hipEventDisableTiming));
-        VL_HIP(hipEventCreateWithFlags(&b->ev_aux[i], hipEventDisableTiming));
-      }
-      VL_HIP(hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking));
-    }
   }
   b->use_graphs = true;
   if (const char* e = std::getenv("VL_GRAPHS")) b->use_graphs = e[0] != '0';
@@ -1229,27 +1155,23 @@ int window_steps(const vl_batch* b) { return b->pass_k > 1 ? GRAPH_W / b->pass_k
 vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
   const int wsteps = window_steps(b);
   const int K = b->pass_k;
-  GroupGrids gg{0, 0, 0, 0, 0};
+  GroupGrids gg{0, 0, 0};
   const int grid = K > 1 ? (gg = group_grids_at(b, b->t, wsteps / K)).bulk : bulk_grid_at(b, b->t);
-  const auto key = std::make_tuple(b->cur, grid, b->pass_overlap ? gg.far : gg.all, gg.near, gg.listed);
+  const auto key = std::make_tuple(b->cur, grid, gg.pass, gg.listed);
   auto found = b->graphs.find(key);
   if (found != b->graphs.end()) {
     *out = found->second;
     return VL_OK;
   }
-  VL_TRYB(join_groups(b));  // a window starts with the side stream joined, and ends so
   const int cur0 = b->cur;
   const int64_t t0 = b->t;
   hipGraph_t g = nullptr;
   VL_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   vl_status st = VL_OK;
-  if (K > 1) {
+  if (K > 1)
     for (int k = 0; k < wsteps && st == VL_OK; k += K) st = batch_group(b, b->d_tbase, k, gg);
-    if (st == VL_OK) st = join_groups(b);
-  } else {
+  else
     for (int k = 0; k < wsteps && st == VL_OK; ++k) st = batch_step(b, b->d_tbase, k, grid);
-  }
-  b->ov_open = false;
   if (st == VL_OK) hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, b->stream, b->d_tbase, (int64_t)wsteps);
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   b->cur = cur0;
@@ -1354,7 +1276,6 @@ vl_status vl_batch_run(vl_batch* b, int64_t max_steps) {
     if (b->pass_k > 1)
       for (; k + b->pass_k <= todo; k += b->pass_k, kk += b->pass_k)
         VL_TRYB(batch_group(b, b->d_tbase, kk, group_grids_at(b, b->t, 1)));
-    VL_TRYB(join_groups(b));
     for (; k < todo; ++k, ++kk) VL_TRYB(batch_step(b, b->d_tbase, kk, bulk_grid_at(b, b->t)));
   }
   VL_HIP(hipStreamSynchronize(b->stream));
@@ -1410,7 +1331,7 @@ vl_status vl_batch_info_get(vl_batch* b, vl_batch_info* o) {
     for (int j = 0; j < 4; ++j) o->autotune_ms[i][j] = b->autotune_ms[i][j];
   o->pass_steps = b->pass_k;
   o->pass_rows = b->pass_k > 1 ? b->pass_rows : 0;
-  o->pass_tasks = b->all.n;
+  o->pass_tasks = b->tasks.n;
   o->listed_tiles = b->nlisted_tiles;
   o->pass_nodes = b->pass_nodes;
   o->listed_nodes = b->listed_nodes;
