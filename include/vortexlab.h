@@ -232,9 +232,10 @@ vl_status vl_channel_coefficients(vl_channel* c, double d, double* cd, double* c
 /* ------------------------------------------------------------------------------------
  * Batched cases (no reference counterpart: the reference runs one case at a time,
  * src/cases/tunnel.rs:152-192).  Each case is a Channel (spec + body outline) run for its own
- * number of steps; all cases step together on one GPU through two persistent kernels over a
- * packed tile table, in replayed step-window graphs.  A case's results are bit-identical to
- * the same case run alone with vl_channel_run in the same precision.
+ * number of steps; all cases step together on one GPU over a packed tile table: the covered
+ * interior of every case in K-step passes, its boundary and remaining interior nodes in single
+ * steps, in replayed step-window graphs.  A case's results are bit-identical to the same case
+ * run alone with vl_channel_run in the same precision.
  * ---------------------------------------------------------------------------------- */
 typedef struct vl_batch vl_batch;
 typedef struct {
@@ -263,9 +264,9 @@ vl_status vl_batch_run(vl_batch* b, int64_t max_steps);
 vl_status vl_batch_info_get(vl_batch* b, vl_batch_info* out);
 /* Kernel timing of the batch's own stepping (no reference counterpart: the reference's
  * `bench` times whole runs only, src/main.rs:448).  While on, vl_batch_run launches every step
- * directly with HIP events around the boundary kernel (side stream), the bulk kernel (main
- * stream) and the whole step; vl_batch_get_timing returns the sums in ms since the last
- * vl_batch_set_timing call. */
+ * directly, as a single step (no K-step passes), with HIP events around the boundary kernel
+ * (side stream), the bulk kernel (main stream) and the whole step; vl_batch_get_timing returns
+ * the sums in ms since the last vl_batch_set_timing call. */
 vl_status vl_batch_set_timing(vl_batch* b, int on);
 vl_status vl_batch_get_timing(vl_batch* b, double* boundary_ms, double* bulk_ms, double* step_ms, int64_t* steps);
 /* (cd, cl) per step of case `id` (2 * steps doubles), the steps completed, and its status
