@@ -188,3 +188,35 @@ def test_batch_passes_equal_single_case_runs(gpu, monkeypatch, prec, k, rows):
         assert np.array_equal(coef, ref), f"case {j}: coefficient history differs"
         assert np.array_equal(b.read_case(j, c.spec.nx, c.spec.ny), ch.sim.read()), f"case {j}: populations differ"
         ch.close()
+
+
+def test_batch_passes_edge_cases(gpu, monkeypatch):
+    """5-step passes with a case too small for any pass task (all its interior listed), cases of
+    3 and 5 steps (fewer than, and exactly, one group) and a case of a few groups plus a
+    remainder: each ends with the bits of the same case run alone."""
+    monkeypatch.setenv("VL_BATCH_PASS_STEPS", "5")
+    star = lambda seed, d: (lambda cx, cy: workloads.star_polygon(np.random.default_rng(seed), d, cx, cy))
+    tiny = workloads.tank_case(16, 10, 3.0, re=40.0, u=0.05, collision=api.BGK, outline_fn=star(1, 3.0),
+                               units=1.0, ramp_units=0.25, kick_units=0.1)
+    runs = [(tiny, tiny.steps)]
+    for k, steps in enumerate((3, 5, 203)):
+        c = workloads.tank_case(300, 80, 10.0, re=60.0, u=0.05, collision=api.TRT if k % 2 else api.BGK,
+                                outline_fn=star(10 + k, 10.0), units=1.0, ramp_units=0.25, kick_units=0.1)
+        runs.append((c, steps))
+    b = api.Batch(gpu, api.F32)
+    for c, steps in runs:
+        b.add(c.spec, steps, c.d)
+    b.build()
+    info = b.info()
+    print(f"[batch] edge cases: pass tasks {info.pass_tasks}, listed {info.listed_nodes}", flush=True)
+    assert info.pass_steps == 5 and info.pass_tasks > 0
+    b.run()
+    monkeypatch.delenv("VL_BATCH_PASS_STEPS")
+    for j, (c, steps) in enumerate(runs):
+        coef, done, status = b.result(j)
+        assert status == 0 and done == steps
+        ch = api.Channel(gpu, c.spec, api.F32)
+        ref = ch.run(steps, c.d)
+        assert np.array_equal(coef, ref), f"case {j}: coefficient history differs"
+        assert np.array_equal(b.read_case(j, c.spec.nx, c.spec.ny), ch.sim.read()), f"case {j}: populations differ"
+        ch.close()
