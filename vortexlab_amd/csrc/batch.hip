@@ -1086,7 +1086,7 @@ vl_status build(vl_batch* b) {
   VL_TRYB(dupload(b, b->d_kx0, kx));
   VL_TRYB(dupload(b, b->d_kact, kact));
   b->h_kact = std::move(kact);
-  // ---- pass tasks (far and near lists) and listed tiles, cases in descending step order ----
+  // ---- pass tasks and listed tiles, cases in descending step order ----
   {
     std::vector<int4> task;
     std::vector<int32_t> tcase, lcase, lfirst, lcount;
