@@ -592,8 +592,9 @@ __host__ __device__ constexpr int tb_out(int k, int v) { return (TB_LANES * v - 
 // Ring nodes (covered nodes whose intermediate values single-step neighbours pull) lie on the
 // task's edges, since every other output node has all eight neighbours inside the task; edge bit
 // 1 / 2 / 4 / 8 = the first row / last row / first column / last column holds some, and the whole
-// edge then writes its values of levels 1..K-1 to the intermediate arrays (values the single
-// steps would also produce, so the extra ones are harmless).  Lockstep: the workgroup's tasks
+// edge then writes its values of levels 1..K-1 to the intermediate arrays: the three populations
+// that point out of the task across that edge, the ones its uncovered neighbours pull (values
+// the single steps would also produce, so the extra ones are harmless).  Lockstep: the workgroup's tasks
 // cover the same rows, and its waves sweep in step (one barrier a row), so that together they
 // read and write whole lines.
 template <class C>
@@ -728,11 +729,13 @@ __device__ __attribute__((always_inline)) inline void multi_step_strip(const int
     for (int j = 0; j < V; ++j)
       st[L].p0[j] = st[L].p1[j] = st[L].p3[j] = st[L].qa2[j] = st[L].qa5[j] = st[L].qa6[j] = st[L].qb2[j] =
           st[L].qb5[j] = st[L].qb6[j] = C(0);
-  // columns of the first / last output column that write their intermediate values on every row
-  bool ring_col[V];
+  // the first / last output column, when it writes its intermediate values on every row
+  bool ring_l[V], ring_r[V];
 #pragma unroll
-  for (int j = 0; j < V; ++j)
-    ring_col[j] = out_lane && (((edges & 4) && xc + j == x0) || ((edges & 8) && xc + j == x0 + w - 1));
+  for (int j = 0; j < V; ++j) {
+    ring_l[j] = out_lane && (edges & 4) && xc + j == x0;
+    ring_r[j] = out_lane && (edges & 8) && xc + j == x0 + w - 1;
+  }
   // Iteration k consumes row r = y0 - K + k: first the row that arrived, then the load of row
   // k + D, then the compute and its stores.  The memory counter that the wait for a load uses
   // also counts stores, and the compiler waits for all of them whenever the accesses in flight
@@ -779,9 +782,14 @@ __device__ __attribute__((always_inline)) inline void multi_step_strip(const int
       C e[Q][V];
       if (run) {
         const int s = y0 - K + k - L;  // the row level L computes
+        // Ring nodes' intermediate values are read only by the single steps of their uncovered
+        // neighbours, which are interior nodes (covered nodes lie K - 1 nodes inside an
+        // all-interior footprint), and each pulls the populations that point at it: row y0
+        // writes those with cy = -1 (its uncovered neighbours lie below), the last row cy = +1,
+        // the first column cx = -1, the last column cx = +1.
         const bool ring_row = L < K && s >= y0 && s < y0 + rows;
-        const bool ring_all =
-            ring_row && out_lane && (((edges & 1) && s == y0) || ((edges & 2) && s == y0 + rows - 1));
+        const bool ring_lo = ring_row && out_lane && (edges & 1) && s == y0;
+        const bool ring_hi = ring_row && out_lane && (edges & 2) && s == y0 + rows - 1;
         PT outv[Q];
 #pragma unroll
         for (int j = 0; j < V; ++j) {
@@ -791,10 +799,13 @@ __device__ __attribute__((always_inline)) inline void multi_step_strip(const int
           if (L < K) {
 #pragma unroll
             for (int i = 0; i < Q; ++i) e[i][j] = S::round(f[i]);
-            if (ring_all || (ring_row && ring_col[j])) {
+            const bool rl = ring_row && ring_l[j], rr = ring_row && ring_r[j];
+            if (ring_lo || ring_hi || rl || rr) {
               T* rowp = po.ring[L - 1] + (int64_t)s * pitch + xs;
 #pragma unroll
-              for (int i = 0; i < Q; ++i) *plane_at(rowp, P, i, V * lane + j) = S::enc(e[i][j]);
+              for (int i = 1; i < Q; ++i)
+                if ((ring_lo && cy(i) < 0) || (ring_hi && cy(i) > 0) || (rl && cx(i) < 0) || (rr && cx(i) > 0))
+                  *plane_at(rowp, P, i, V * lane + j) = S::enc(e[i][j]);
             }
           } else {
 #pragma unroll
