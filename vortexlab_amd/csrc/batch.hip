@@ -1097,7 +1097,11 @@ vl_status build(vl_batch* b) {
     std::vector<int32_t> tend(nc, 0), lend(nc, 0);
     if (b->pass_k > 1)
       for (int ci : order) {
-        const PassPlan& pl = plans[(size_t)ci];
+        PassPlan& pl = plans[(size_t)ci];
+        // one launch takes far and near tasks alike: (first row, first column) order, so that
+        // a workgroup's waves sweep adjacent strips of the same rows
+        std::sort(pl.tasks.begin(), pl.tasks.end(),
+                  [](const int4& a, const int4& c) { return a.y != c.y ? a.y < c.y : a.x < c.x; });
         for (const int4& tk : pl.tasks) {
           task.push_back(tk);
           tcase.push_back(ci);
