@@ -190,6 +190,16 @@ vl_status vl_sim_set_timing(vl_sim* s, int on);
 vl_status vl_device_copy_bandwidth(size_t bytes, int iters, double* gbs);
 vl_status vl_sim_get_timing(vl_sim* s, double* bulk_ms, int64_t* bulk_launches, double* total_ms,
                             int64_t* steps);
+/* The K-step pass plan of one domain, host only, no device needed (no reference counterpart:
+ * the planner behind the K-step passes that vl_sim_create and vl_batch_build make, exposed for
+ * tests and diagnostics).  flags: nx * ny bytes, 0 interior, 1 boundary, 2 solid; k steps per
+ * pass (2..5), v columns per lane (1, 2), tasks of at most `rows` output rows, strips from column
+ * x0 (>= the halo).  Writes up to max_tasks tasks, 4 int32 each (first output column, first row,
+ * rows, output columns | ring-edge bits << 8: first row, last row, first column, last column),
+ * far tasks first, and up to max_listed listed nodes (y << 16 | x); counts = {tasks, far tasks,
+ * listed nodes, covered nodes}. */
+vl_status vl_plan_passes(const uint8_t* flags, int nx, int ny, int k, int v, int rows, int x0, int32_t* tasks,
+                         int64_t max_tasks, uint32_t* listed, int64_t max_listed, int64_t* counts);
 
 /* ------------------------------------------------------------------------------------
  * Channel: src/cases/channel.rs:40-155 (a body in a channel; the per-step driver).

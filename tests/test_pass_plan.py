@@ -1,0 +1,134 @@
+"""CPU checks of the K-step pass planner (vl_plan_passes, the host code behind the passes of
+vl_sim_create and vl_batch_build) on channel-like flag maps with bodies.  The GPU tests show
+bit equality with single steps on particular domains; these pin the plan's invariants on
+many: strips that cover only interior nodes and whose dependency footprints are interior, every
+interior node covered once or listed once, ring-edge bits that cover every uncovered neighbour
+of a covered node (the ring stores write the populations pointing across the marked edges), and
+far tasks first with no uncovered node within K of their outputs."""
+import numpy as np
+import pytest
+
+import vortexlab_amd as vl
+from vortexlab_amd import api
+
+
+def tb_halo(k, v):
+    return (k + v - 1) // v * v
+
+
+def tb_out(k, v):
+    return (64 * v - tb_halo(k, v) - k) // v * v
+
+
+def channel_flags(nx, ny, bodies, rects=()):
+    """0 interior, 1 boundary (the domain edge, or next to a solid node), 2 solid."""
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    solid = np.zeros((ny, nx), bool)
+    for cx, cy, r in bodies:
+        solid |= (xx - cx) ** 2 + (yy - cy) ** 2 <= r * r
+    for x0, x1, y0, y1 in rects:
+        solid[y0:y1, x0:x1] = True
+    near = np.zeros_like(solid)
+    p = np.pad(solid, 1)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            near |= p[1 + dy:1 + dy + ny, 1 + dx:1 + dx + nx]
+    edge = np.zeros_like(solid)
+    edge[0, :] = edge[-1, :] = edge[:, 0] = edge[:, -1] = True
+    fl = np.where(solid, 2, np.where(near | edge, 1, 0)).astype(np.uint8)
+    return fl
+
+
+def random_case(seed):
+    rng = np.random.default_rng(seed)
+    nx, ny = int(rng.integers(90, 320)), int(rng.integers(30, 130))
+    bodies = [(rng.uniform(0.2, 0.8) * nx, rng.uniform(0.25, 0.75) * ny, rng.uniform(2.0, 0.2 * ny))
+              for _ in range(int(rng.integers(0, 3)))]
+    rects = []
+    if rng.random() < 0.5:
+        x0 = int(rng.integers(5, nx - 20))
+        y0 = int(rng.integers(3, ny - 10))
+        rects.append((x0, x0 + int(rng.integers(2, 15)), y0, y0 + int(rng.integers(2, 8))))
+    return channel_flags(nx, ny, bodies, rects)
+
+
+def check_plan(fl, k, v, rows, x0):
+    ny, nx = fl.shape
+    tasks, nfar, listed, ncov = api.plan_passes(vl.backend(), fl, k, v, rows, x0)
+    interior = fl == 0
+    fluid = fl != 2
+    covered = np.zeros((ny, nx), np.int32)
+    owner = -np.ones((ny, nx), np.int64)
+    for i, (tx, ty, tz, tw) in enumerate(tasks):
+        w, edges = int(tw) & 0xFF, (int(tw) >> 8) & 15
+        assert (int(tw) >> 12) == 0, "the caller sets the lockstep bit"
+        assert 1 <= w <= tb_out(k, v) and w % v == 0 and (tx - tb_halo(k, v)) % v == 0
+        assert 1 <= tz <= rows
+        covered[ty:ty + tz, tx:tx + w] += 1
+        owner[ty:ty + tz, tx:tx + w] = i
+        # the dependency footprint (K - 1 around the outputs) lies inside the domain's interior
+        fy0, fy1, fx0, fx1 = ty - (k - 1), ty + tz + k - 1, tx - (k - 1), tx + w + k - 1
+        assert fy0 >= 1 and fy1 <= ny - 1 and fx0 >= 1 and fx1 <= nx - 1
+        assert interior[fy0:fy1, fx0:fx1].all(), f"task {i}: footprint not interior"
+    assert covered.max(initial=0) <= 1, "tasks overlap"
+    assert int(covered.sum()) == ncov
+    # every interior node is covered or listed, once
+    want = np.flatnonzero((interior & (covered == 0)).ravel())
+    got_y, got_x = (listed >> 16).astype(np.int64), (listed & 0xFFFF).astype(np.int64)
+    assert np.array_equal(np.sort(got_y * nx + got_x), want)
+    assert np.all(np.diff(listed.astype(np.int64)) > 0), "listed nodes in (y, x) order"
+    # ring edges: every uncovered fluid neighbour u of a covered node r lies outside r's task
+    # across sides whose bits are set
+    uncovered = fluid & (covered == 0)
+    ys, xs = np.nonzero(covered)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if dx == 0 and dy == 0:
+                continue
+            uy, ux = ys + dy, xs + dx
+            hit = uncovered[uy, ux]
+            for ry, rx, qy, qx in zip(ys[hit], xs[hit], uy[hit], ux[hit]):
+                tx, ty, tz, tw = tasks[owner[ry, rx]]
+                w, edges = int(tw) & 0xFF, (int(tw) >> 8) & 15
+                sides = [(qy < ty, 1), (qy > ty + tz - 1, 2), (qx < tx, 4), (qx > tx + w - 1, 8)]
+                assert any(out for out, _ in sides)
+                for out, bit in sides:
+                    assert not out or edges & bit, f"ring node ({rx},{ry}): edge bit {bit} missing"
+    # far tasks first: no uncovered fluid node within K of their outputs, and no ring edges
+    for i, (tx, ty, tz, tw) in enumerate(tasks):
+        w, edges = int(tw) & 0xFF, (int(tw) >> 8) & 15
+        box = uncovered[max(ty - k, 0):ty + tz + k, max(tx - k, 0):tx + w + k]
+        if i < nfar:
+            assert not box.any() and edges == 0, f"far task {i} near an uncovered node"
+        else:
+            assert box.any(), f"near task {i} has no uncovered node within K"
+    for part in (tasks[:nfar], tasks[nfar:]):
+        keys = [(int(t[1]), int(t[0])) for t in part]
+        assert keys == sorted(keys)
+    return len(tasks), nfar, len(listed), ncov
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("k,v,rows", [(2, 1, 7), (3, 1, 20), (4, 1, 3), (5, 1, 20), (3, 2, 24), (4, 2, 1)])
+def test_plan_invariants(seed, k, v, rows):
+    fl = random_case(seed)
+    x0 = 32 if seed % 2 else tb_halo(k, v) + (seed % 3) * v
+    ntask, nfar, nlisted, ncov = check_plan(fl, k, v, rows, x0)
+    assert ncov > 0 or fl.shape[0] < 2 * k + 3
+
+
+def test_plan_of_the_channel_with_a_cylinder():
+    """A 400 x 100 channel with a D = 30 cylinder: most of the interior is covered, the listed
+    nodes are the margins along the walls, inlet and outlet and around the body."""
+    fl = channel_flags(400, 100, [(100.0, 50.0, 15.0)])
+    ntask, nfar, nlisted, ncov = check_plan(fl, 5, 1, 20, 32)
+    assert ncov > 0.8 * int((fl == 0).sum())
+    assert 0 < nfar < ntask
+
+
+def test_plan_rejects_bad_arguments():
+    fl = channel_flags(50, 20, [])
+    be = vl.backend()
+    for k, v, x0 in ((1, 1, 32), (6, 1, 32), (3, 3, 32), (4, 1, 2)):
+        with pytest.raises(api.VortexError):
+            api.plan_passes(be, fl, k, v, 20, x0)

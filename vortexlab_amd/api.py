@@ -120,6 +120,8 @@ _SIGS = [
     ("sim_create", C.c_int, [_VP, C.c_int, C.POINTER(_VP)]),
     ("sim_destroy", None, [_VP]),
     ("sim_info_get", C.c_int, [_VP, C.POINTER(SimInfo)]),
+    ("plan_passes", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _VP, C.c_int64, _VP,
+                              C.c_int64, _VP]),
     ("sim_init_macro", C.c_int, [_VP, _D, _D, _D]),
     ("sim_set_vscale", C.c_int, [_VP, _D]),
     ("sim_set_outflow_target", C.c_int, [_VP, C.c_int, C.c_double]),
@@ -191,6 +193,23 @@ class Backend:
             if status == ENONFINITE:
                 raise Diverged(status, msg)
             raise VortexError(status, msg)
+
+
+def plan_passes(be: Backend, flags: np.ndarray, k: int, v: int = 1, rows: int = 20, x0: int = 32):
+    """The K-step pass plan of a domain (flags: ny x nx, 0 interior, 1 boundary, 2 solid), host
+    only: (tasks as an (n, 4) int32 array of first column, first row, rows, columns | ring-edge
+    bits << 8, far tasks first; the number of far tasks; listed nodes as (y << 16 | x); covered
+    node count)."""
+    fl = np.ascontiguousarray(flags, dtype=np.uint8)
+    ny, nx = fl.shape
+    counts = np.zeros(4, dtype=np.int64)
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    be.check(be.plan_passes(ptr(fl), nx, ny, k, v, rows, x0, None, 0, None, 0, ptr(counts)))
+    tasks = np.zeros((max(int(counts[0]), 1), 4), dtype=np.int32)
+    listed = np.zeros(max(int(counts[2]), 1), dtype=np.uint32)
+    be.check(be.plan_passes(ptr(fl), nx, ny, k, v, rows, x0, ptr(tasks), int(counts[0]), ptr(listed),
+                            int(counts[2]), ptr(counts)))
+    return tasks[:int(counts[0])], int(counts[1]), listed[:int(counts[2])], int(counts[3])
 
 
 # ---------------------------------------------------------------------------------------

@@ -1055,6 +1055,30 @@ vl_status vl_sim_create(vl_builder* b, int precision, vl_sim** out) {
 
 void vl_sim_destroy(vl_sim* s) { free_sim(s); }
 
+vl_status vl_plan_passes(const uint8_t* flags, int nx, int ny, int k, int v, int rows, int x0, int32_t* tasks,
+                         int64_t max_tasks, uint32_t* listed, int64_t max_listed, int64_t* counts) {
+  if (!flags || !counts || nx < 3 || ny < 3 || k < 2 || k > TB_MAX_K || (v != 1 && v != 2) || x0 < tb_halo(k, v))
+    return set_error(VL_EINVAL, "vl_plan_passes: flags, counts, nx, ny >= 3, k in 2..5, v in 1..2, x0 >= halo");
+  PassPlan plan;
+  std::vector<uint8_t> fl(flags, flags + (size_t)nx * ny);
+  VL_TRY(plan_passes(fl, nx, ny, round_up(nx, 64), k, v, rows, x0, plan));
+  counts[0] = (int64_t)plan.tasks.size();
+  counts[1] = plan.nfar;
+  counts[2] = (int64_t)plan.listed.size();
+  counts[3] = plan.covered;
+  if (tasks)
+    for (int64_t i = 0; i < std::min<int64_t>(max_tasks, counts[0]); ++i) {
+      const int4& t = plan.tasks[(size_t)i];
+      tasks[4 * i] = t.x;
+      tasks[4 * i + 1] = t.y;
+      tasks[4 * i + 2] = t.z;
+      tasks[4 * i + 3] = t.w;
+    }
+  if (listed)
+    for (int64_t i = 0; i < std::min<int64_t>(max_listed, counts[2]); ++i) listed[i] = plan.listed[(size_t)i];
+  return VL_OK;
+}
+
 vl_status vl_sim_info_get(vl_sim* s, vl_sim_info* o) {
   if (!s || !o) return set_error(VL_EINVAL, "null argument");
   std::memset(o, 0, sizeof(*o));
