@@ -280,6 +280,8 @@ def batch_bench(vl, api, ncases, seed, sample):
     return {"batched_cases_per_s": round(ncases / t_run, 3), "batched_glups": round(info.lattice_updates / t_run / 1e9, 2),
             "batched_cases": ncases, "batched_diverged": diverged, "batched_seconds": round(t_run, 2),
             "batched_setup_seconds": round(t_build, 2), "batched_tile": f"{info.tile_w}x{info.tile_h}",
+            "batched_pass_steps": int(info.pass_steps), "batched_pass_nodes": int(info.pass_nodes),
+            "batched_listed_nodes": int(info.listed_nodes),
             "sequential_cases_per_s": round(seq_rate_all, 3), "sequential_sample": len(pick),
             "sequential_sample_cases_per_s": round(seq_rate, 3),
             "sequential_step_us": round(coeff[0] * 1e6, 2), "sequential_update_glups": round(1e-9 / coeff[1], 1),
