@@ -8,11 +8,15 @@ centred at (2048, 2048), Re = 1000.  A "step" is one full time step of that doma
 stream-collide, boundary links, outlet controller and the FP64 force reduction.
 
 The headline value is GLUPS (fluid-node updates per second / 1e9) of the FP32-deviation
-storage path with inputs resident in HBM; the same steps are run with FP64 storage for the
-speed ratio and the Cd / Cl' / St error of FP32 against FP64.  The roofline object prices the
-bulk kernel (measured with HIP events on the library's own stream) against MI355X HBM peak.
-The CPU baseline is the oracle (C++ restatement of the reference solver, same algorithm, all
-host cores) timed on a bounded number of steps of the same domain.
+storage path over the --steps timed steps, with inputs resident in HBM.  Whatever --steps is,
+the precision gate then runs the workload's full length (20000 steps) in FP32 and in FP64 from
+the same start and reports the whole-run rates, their ratio and the Cd / Cl' / St error of FP32
+against FP64 over the second half of the run (Series::stats), with pass / fail against the
+|dCd| < 0.5 % bound.  The roofline object prices the dominant kernel (measured with HIP events
+on the stream it runs on) against MI355X HBM peak.  The batched sweep (512 seeded cases) runs in
+FP32 and again in FP64, so the drag of every finished case is checked against FP64.  The CPU
+baseline is the oracle (C++ restatement of the reference solver, same algorithm, every core the
+process may use) timed on a bounded number of steps of the same domain.
 """
 import argparse
 import json
@@ -74,7 +78,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
+def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False, kernel_timing=True):
     builder, d, _, _, _ = WORKLOADS[wl]
     spec = builder(api, scale)
     d = d / scale if wl == "notched_8192x4096" else d
@@ -84,8 +88,11 @@ def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
     log(f"[bench] {PREC_NAME[precision]}: built {info.nx}x{info.ny}, {info.fluid_nodes} fluid nodes, "
         f"{info.link_count} links in {time.time() - t0:.1f} s")
     coef_w = ch.run(warmup, d) if warmup else np.zeros((0, 2))
-    # timed region: the production path (replayed step-window graphs)
-    # long runs go in pieces of 10000 steps, each followed by a progress line
+    # timed region: `steps` steps through vl_channel_run.  Runs of one step-window graph or more
+    # (30 steps at 5 steps per pass, 32 at 4) replay captured windows; a shorter run launches its
+    # groups of K steps directly, and the fixed cost of the call (control table upload, force
+    # history readback) is not amortised, so short runs read below the full-length rate.
+    # Long runs go in pieces of 10000 steps, each followed by a progress line.
     pieces, done = [], 0
     t0 = time.perf_counter()
     while done < steps:
@@ -99,10 +106,12 @@ def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
     state_hash = ch.sim.state_hash() if wl == "wake_662x302" else None
     # kernel pass: the same step launched directly with HIP events around every bulk kernel on
     # the stream it runs on (events cannot sit inside a replayed graph)
-    ch.sim.set_timing(True)
-    ch.run(min(300, steps), d)
-    bulk_ms, bulk_n, total_ms, tsteps = ch.sim.timing()
-    ch.sim.set_timing(False)
+    bulk_ms, bulk_n, total_ms, tsteps = 0.0, 0, 0.0, 0
+    if kernel_timing:
+        ch.sim.set_timing(True)
+        ch.run(min(300, steps), d)
+        bulk_ms, bulk_n, total_ms, tsteps = ch.sim.timing()
+        ch.sim.set_timing(False)
     info = ch.sim.info()
     res = dict(precision=precision, fluid=int(info.fluid_nodes), bnodes=int(info.bnode_count), secs=secs,
                glups=info.fluid_nodes * steps / secs / 1e9, bulk_ms=bulk_ms / max(bulk_n, 1),
@@ -113,8 +122,8 @@ def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False):
                pass_far_nodes=int(info.pass_far_nodes), pass_far_tasks=int(info.pass_far_tasks))
     kname = (f"{res['pass_steps']}-step pass kernel" if res["pass_steps"] > 1
              else ("bulk kernel" if res["split"] else "step kernel"))
-    log(f"[bench] {PREC_NAME[precision]}: {steps} steps in {secs:.2f} s = {res['glups']:.2f} GLUPS, "
-        f"{kname} {res['bulk_ms'] * 1e3:.1f} us, whole step {res['step_ms'] * 1e3:.1f} us")
+    log(f"[bench] {PREC_NAME[precision]}: {steps} steps in {secs:.2f} s = {res['glups']:.2f} GLUPS"
+        + (f", {kname} {res['bulk_ms'] * 1e3:.1f} us, whole step {res['step_ms'] * 1e3:.1f} us" if kernel_timing else ""))
     if keep:
         res["channel"] = ch
     else:
@@ -143,6 +152,35 @@ def coef_stats(vl, r):
     return vl.series_stats(cd, cl, r["dt"], 0.5 * len(cd) * r["dt"])
 
 
+DCD_BOUND = 0.005  # |dCd| < 0.5 % against FP64
+
+
+def precision_gate(vl, api, wl, prec, scale):
+    """The workload's full run (its default step count, from t = 0, no warmup) in `prec` and in
+    FP64: whole-run GLUPS of both, and the force statistics of `prec` against FP64 over the
+    second half of the run (Series::stats, src/cases/channel.rs:166-189)."""
+    steps = WORKLOADS[wl][3]
+    keep = wl == "circle_16384x8192"
+    lo = run_gpu(vl, api, wl, prec, steps, 0, scale, keep=keep, kernel_timing=False)
+    hi = run_gpu(vl, api, wl, api.F64, steps, 0, scale, keep=keep, kernel_timing=False)
+    s_lo, s_hi = coef_stats(vl, lo), coef_stats(vl, hi)
+    dcd = abs(s_lo.cd_mean / s_hi.cd_mean - 1.0)
+    gate = {"steps": steps, "storage": PREC_NAME[prec], "glups": round(lo["glups"], 3),
+            "fp64_glups": round(hi["glups"], 3), "speedup_vs_fp64": round(lo["glups"] / hi["glups"], 3),
+            "stats_window": f"steps {steps // 2}-{steps} (second half)",
+            "cd_mean": s_lo.cd_mean, "cd_mean_fp64": s_hi.cd_mean, "dcd_rel": dcd,
+            "cl_rms": s_lo.cl_rms, "cl_rms_fp64": s_hi.cl_rms,
+            "dcl_rms_rel": abs(s_lo.cl_rms / s_hi.cl_rms - 1.0) if s_hi.cl_rms != 0 else None,
+            "st": s_lo.st, "st_fp64": s_hi.st, "bound_dcd_rel": DCD_BOUND, "ok": bool(dcd < DCD_BOUND)}
+    if keep:
+        gate["velocity_l2_error_vs_fp64"] = velocity_error(lo["channel"], hi["channel"])
+        lo["channel"].close()
+        hi["channel"].close()
+    log(f"[gate] {steps} steps: {PREC_NAME[prec]} {lo['glups']:.1f} GLUPS, f64 {hi['glups']:.1f} GLUPS "
+        f"({gate['speedup_vs_fp64']:.2f}x); |dCd| {dcd:.2e} (bound {DCD_BOUND}) -> {'ok' if gate['ok'] else 'FAIL'}")
+    return gate
+
+
 def velocity_error(a, b):
     """Per-node velocity L2 error of run a against run b (fluid nodes; solid nodes are 0 in both)."""
     _, ax, ay = a.sim.macros_all()
@@ -151,12 +189,36 @@ def velocity_error(a, b):
     return float(num / np.sqrt(np.sum(bx ** 2 + by ** 2)))
 
 
+def host_cpus():
+    """(cores this process may use, what nproc reports, the CPU model).  The usable count is the
+    affinity mask, further limited by a cgroup CPU quota when one is set."""
+    nproc = os.cpu_count() or 1
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            quota, period = fh.read().split()[:2]
+        if quota != "max":
+            usable = min(usable, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return usable, nproc, model
+
+
 def cpu_baseline(api, wl, seconds_target, scale, steps_full, gpu_hash=None):
-    """The oracle on the same domain, all host cores (capped at 16), a bounded number of steps
-    (the whole run when it fits in `seconds_target`)."""
+    """The oracle on the same domain, on every core this process may use, a bounded number of
+    steps (the whole run when it fits in `seconds_target`)."""
     import oracle
     be = oracle.backend()
-    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    threads, nproc, model = host_cpus()
     builder, d, _, _, _ = WORKLOADS[wl]
     spec = builder(api, scale)
     d = d / scale if wl == "notched_8192x4096" else d
@@ -180,7 +242,9 @@ def cpu_baseline(api, wl, seconds_target, scale, steps_full, gpu_hash=None):
                                                                          f"matches GPU: {match}"))
     out = {"value": round(g, 5), "unit": "GLUPS", "cores": threads, "kind": "port",
            "sample": f"oracle (C++ restatement of the reference solver), {steps + 2} steps of the "
-                     f"{spec.nx}x{spec.ny} domain from t=0, {threads} threads, {secs:.1f} s timed"}
+                     f"{spec.nx}x{spec.ny} domain from t=0, {threads} threads, {secs:.1f} s timed",
+           "host_nproc": nproc, "host_cpu_model": model,
+           "host_cores_usable": threads}
     if match is not None:
         out["full_run_state_hash_matches_gpu"] = match
     # one thread on the same domain (a few seconds), and the reference's own benchmark (cylinder
@@ -208,21 +272,43 @@ def cpu_baseline(api, wl, seconds_target, scale, steps_full, gpu_hash=None):
 def batch_bench(vl, api, ncases, seed, sample):
     """The batched sweep: `ncases` seeded towing-tank cases (random star polygons, grid
     200-1400 x 80-600, D 10-60, Re 40-250, BGK/TRT, u = 0.05, 20 D/U each) in FP32 storage, one
-    batch on one GPU; then `sample` of the same cases run one at a time (single-case path, FP32,
-    and FP64 for the drag check)."""
+    batch on one GPU; the same batch in FP64, for the drag of every case that finishes; then
+    `sample` of the cases run one at a time (single-case path, FP32) for the sequential rate."""
     from vortexlab_amd import workloads
     cases = workloads.sweep_cases(seed, ncases)
     b, t_build, t_run = workloads.run_batch(vl.backend(), cases, api.F32)
     info = b.info()
-    log(f"[batch] {ncases} cases, {info.fluid_nodes / 1e6:.1f} M fluid nodes, {info.lattice_updates / 1e12:.3f} T "
-        f"updates; set-up {t_build:.1f} s; tiles {info.tile_w}x{info.tile_h}, grid {info.grid_bulk}")
+    lu = int(info.lattice_updates)  # updates performed (a retired case counts the steps it ran)
     results = [b.result(k) for k in range(ncases)]
-    diverged = sum(1 for r in results if r[2] != 0)
-    log(f"[batch] run {t_run:.2f} s = {ncases / t_run:.2f} cases/s, {info.lattice_updates / t_run / 1e9:.1f} GLUPS; "
-        f"{diverged} diverged")
+    fluid = [b.case_fluid_nodes(k) for k in range(ncases)]
+    b.close()
+    finished = [k for k in range(ncases) if results[k][2] == 0]
+    diverged = ncases - len(finished)
+    log(f"[batch] {ncases} cases, {info.fluid_nodes / 1e6:.1f} M fluid nodes, {lu / 1e12:.3f} T updates performed; "
+        f"set-up {t_build:.1f} s; tiles {info.tile_w}x{info.tile_h}, grid {info.grid_bulk}")
+    log(f"[batch] run {t_run:.2f} s: {len(finished)} finished = {len(finished) / t_run:.2f} cases/s, "
+        f"{lu / t_run / 1e9:.1f} GLUPS; {diverged} diverged (retired)")
+    # the same cases in FP64: the drag of every finished case against its FP64 run (an FP64
+    # batch case is bit-identical to the case run alone in FP64)
+    b64, t_build64, t_run64 = workloads.run_batch(vl.backend(), cases, api.F64)
+    lu64 = int(b64.info().lattice_updates)
+    res64 = [b64.result(k) for k in range(ncases)]
+    b64.close()
+    dcd, fp64_diverged = {}, 0
+    for k in finished:
+        if res64[k][2] != 0:
+            fp64_diverged += 1
+            continue
+        h = cases[k].steps // 2
+        dcd[k] = abs(results[k][0][h:, 0].mean() / res64[k][0][h:, 0].mean() - 1.0)
+    within = [k for k in finished if k in dcd and dcd[k] < DCD_BOUND]
+    vals = np.array(list(dcd.values())) if dcd else np.array([np.nan])
+    log(f"[batch] FP64 batch {t_run64:.2f} s ({lu64 / t_run64 / 1e9:.1f} GLUPS); |dCd| FP32 vs FP64 over "
+        f"{len(dcd)} finished cases: max {np.nanmax(vals):.2e}, median {np.nanmedian(vals):.2e}; "
+        f"{len(within)} within {DCD_BOUND}; {fp64_diverged} finished in FP32 but diverged in FP64")
     rng = np.random.default_rng(seed + 1)
     pick = sorted(rng.choice(ncases, size=min(sample, ncases), replace=False).tolist())
-    seq_secs, dcd, rows, times = 0.0, [], [], []
+    seq_secs, rows, times = 0.0, [], []
     for k in pick:
         c = cases[k]
         ch = vl.make_channel(c.spec, api.F32)
@@ -237,26 +323,15 @@ def batch_bench(vl, api, ncases, seed, sample):
         rows.append((done, done * ch.sim.fluid_nodes))
         times.append(dt)
         ch.close()
-        coef, done, st = results[k]
-        if st == 0:
-            ref = vl.make_channel(c.spec, api.F64)
-            try:
-                rc = ref.run(c.steps, c.d)
-                h = c.steps // 2
-                dcd.append(abs(coef[h:, 0].mean() / rc[h:, 0].mean() - 1.0))
-            except api.Diverged:
-                pass
-            ref.close()
     seq_rate = len(pick) / seq_secs
     # one case alone costs a per-step launch time plus a per-update time: fitted on the sample,
     # summed over every case of the batch (with the steps each ran in the batch)
     coeff = np.linalg.lstsq(np.array(rows, dtype=float), np.array(times), rcond=None)[0]
-    fluid = [b.case_fluid_nodes(k) for k in range(ncases)]
     seq_total = sum(coeff[0] * results[k][1] + coeff[1] * results[k][1] * fluid[k] for k in range(ncases))
-    seq_rate_all = ncases / seq_total
+    seq_rate_all = len(finished) / seq_total
     log(f"[batch] sequential single-case runs: {len(pick)} cases in {seq_secs:.2f} s = {seq_rate:.2f} cases/s; "
-        f"model {coeff[0] * 1e6:.2f} us/step + {1e-9 / coeff[1]:.1f} GLUPS -> {seq_rate_all:.2f} cases/s over the batch; "
-        f"max |dCd| vs FP64 {max(dcd) if dcd else float('nan'):.2e} over {len(dcd)} cases")
+        f"model {coeff[0] * 1e6:.2f} us/step + {1e-9 / coeff[1]:.1f} GLUPS -> {seq_rate_all:.2f} finished cases/s "
+        f"over the batch; batched / sequential = {seq_total / t_run:.2f}")
     # The same cases driven the way the reference's case loop drives a Channel (tunnel.rs:76-85:
     # one step, then the force coefficients, per call): 1000 steps each on 8 of the sampled cases,
     # fitted and summed over the batch in the same way.
@@ -273,23 +348,25 @@ def batch_bench(vl, api, ncases, seed, sample):
         ch.close()
     coeff1 = np.linalg.lstsq(np.array(rows1, dtype=float), np.array(times1), rcond=None)[0]
     seq1_total = sum(coeff1[0] * results[k][1] + coeff1[1] * results[k][1] * fluid[k] for k in range(ncases))
-    seq1_rate = ncases / seq1_total
-    log(f"[batch] driven one step per call: {coeff1[0] * 1e6:.2f} us/step -> {seq1_rate:.3f} cases/s over the batch; "
-        f"batched / that = {(ncases / t_run) / seq1_rate:.1f}")
-    b.close()
-    return {"batched_cases_per_s": round(ncases / t_run, 3), "batched_glups": round(info.lattice_updates / t_run / 1e9, 2),
-            "batched_cases": ncases, "batched_diverged": diverged, "batched_seconds": round(t_run, 2),
-            "batched_setup_seconds": round(t_build, 2), "batched_tile": f"{info.tile_w}x{info.tile_h}",
+    log(f"[batch] driven one step per call: {coeff1[0] * 1e6:.2f} us/step; batched / that = {seq1_total / t_run:.1f}")
+    return {"batched_cases_per_s": round(len(finished) / t_run, 3),
+            "batched_solves_per_s_at_gate": round(len(within) / t_run, 3),
+            "batched_glups": round(lu / t_run / 1e9, 2), "batched_lattice_updates": lu,
+            "batched_cases": ncases, "batched_finished": len(finished), "batched_diverged": diverged,
+            "batched_seconds": round(t_run, 2), "batched_setup_seconds": round(t_build, 2),
+            "batched_tile": f"{info.tile_w}x{info.tile_h}",
             "batched_pass_steps": int(info.pass_steps), "batched_pass_nodes": int(info.pass_nodes),
             "batched_listed_nodes": int(info.listed_nodes),
+            "batched_dcd_checked": len(dcd), "batched_within_gate": len(within),
+            "batched_max_dcd_vs_fp64": float(np.nanmax(vals)), "batched_median_dcd_vs_fp64": float(np.nanmedian(vals)),
+            "batched_fp64_diverged_only": fp64_diverged, "batched_fp64_seconds": round(t_run64, 2),
+            "batched_fp64_glups": round(lu64 / t_run64 / 1e9, 2),
             "sequential_cases_per_s": round(seq_rate_all, 3), "sequential_sample": len(pick),
             "sequential_sample_cases_per_s": round(seq_rate, 3),
             "sequential_step_us": round(coeff[0] * 1e6, 2), "sequential_update_glups": round(1e-9 / coeff[1], 1),
-            "batched_vs_sequential": round((ncases / t_run) / seq_rate_all, 2),
-            "sequential_one_step_per_call_cases_per_s": round(seq1_rate, 3),
+            "batched_vs_sequential": round(seq_total / t_run, 2),
             "sequential_one_step_per_call_step_us": round(coeff1[0] * 1e6, 2),
-            "batched_vs_sequential_one_step_per_call": round((ncases / t_run) / seq1_rate, 2),
-            "batched_max_dcd_vs_fp64": (max(dcd) if dcd else None), "batched_dcd_checked": len(dcd)}
+            "batched_vs_sequential_one_step_per_call": round(seq1_total / t_run, 2)}
 
 
 def main():
@@ -299,7 +376,9 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's full run)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 100; 0 for the wake case)")
     ap.add_argument("--precision", choices=["f32", "f16", "f64"], default=None)
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs (and the gate)")
+    ap.add_argument("--no-gate", action="store_true",
+                    help="skip the full-length FP32 / FP64 runs of the precision gate")
     ap.add_argument("--cpu-seconds", type=float, default=None,
                     help="CPU-baseline sample length (default 15 s; 90 s for the wake case, which covers its whole run; 0 = skip)")
     ap.add_argument("--scale", type=int, default=1, help="divide the notched-square domain by this (testing only)")
@@ -324,28 +403,26 @@ def main():
     pname = args.precision or prec_default
     prec = {"f64": api.F64, "f32": api.F32, "f16": api.F16}[pname]
     compare = not args.no_fp64 and prec != api.F64
-    main_run = run_gpu(vl, api, wl, prec, steps, warmup, args.scale, keep=compare and wl == "circle_16384x8192")
+    main_run = run_gpu(vl, api, wl, prec, steps, warmup, args.scale)
     extra = {}
     if compare:
-        ref = run_gpu(vl, api, wl, api.F64, steps, warmup, args.scale, keep=wl == "circle_16384x8192")
-        s_main, s_ref = coef_stats(vl, main_run), coef_stats(vl, ref)
-        cl_ref = s_ref.cl_rms if s_ref.cl_rms != 0 else 1.0
+        # FP64 over the same timed steps: the speed ratio at this run length
+        ref = run_gpu(vl, api, wl, api.F64, steps, warmup, args.scale)
         extra = {
             "fp64_glups": round(ref["glups"], 3),
             "speedup_vs_fp64": round(main_run["glups"] / ref["glups"], 3),
-            "cd_mean": s_main.cd_mean, "cd_mean_fp64": s_ref.cd_mean,
-            "dcd_rel": abs(s_main.cd_mean / s_ref.cd_mean - 1.0),
-            "dcl_rms_rel": abs(s_main.cl_rms / cl_ref - 1.0),
-            "cl_rms": s_main.cl_rms, "cl_rms_fp64": s_ref.cl_rms,
-            "st": s_main.st, "st_fp64": s_ref.st,
             "fp64_kernel": dominant_kernel(ref)[0],
             "fp64_kernel_frac_hbm": round(dominant_kernel(ref)[2] / (ref["bulk_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "fp64_glups_frac_hbm": round(ref["glups"] * BYTES_PER_NODE[0] / HBM_PEAK_GBS, 4),
         }
-        if wl == "circle_16384x8192":
-            extra["velocity_l2_error_vs_fp64"] = velocity_error(main_run["channel"], ref["channel"])
-            main_run["channel"].close()
-            ref["channel"].close()
+        # the accuracy gate needs the developed force history: the workload's full length in both
+        # precisions, whatever --steps is
+        if not args.no_gate:
+            gate = precision_gate(vl, api, wl, prec, args.scale)
+            extra["precision_gate"] = gate
+            for key in ("dcd_rel", "dcl_rms_rel", "cd_mean", "cd_mean_fp64", "cl_rms", "cl_rms_fp64", "st", "st_fp64"):
+                extra[key] = gate[key]
+            extra["gate_ok"] = gate["ok"]
     kname, units, traffic_alg = dominant_kernel(main_run)
     achieved = traffic_alg / (main_run["bulk_ms"] * 1e-3) / 1e9
     traffic = None

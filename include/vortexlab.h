@@ -194,12 +194,25 @@ vl_status vl_sim_get_timing(vl_sim* s, double* bulk_ms, int64_t* bulk_launches, 
  * the planner behind the K-step passes that vl_sim_create and vl_batch_build make, exposed for
  * tests and diagnostics).  flags: nx * ny bytes, 0 interior, 1 boundary, 2 solid; k steps per
  * pass (2..5), v columns per lane (1, 2), tasks of at most `rows` output rows, strips from column
- * x0 (>= the halo).  Writes up to max_tasks tasks, 4 int32 each (first output column, first row,
- * rows, output columns | ring-edge bits << 8: first row, last row, first column, last column),
- * far tasks first, and up to max_listed listed nodes (y << 16 | x); counts = {tasks, far tasks,
- * listed nodes, covered nodes}. */
-vl_status vl_plan_passes(const uint8_t* flags, int nx, int ny, int k, int v, int rows, int x0, int32_t* tasks,
-                         int64_t max_tasks, uint32_t* listed, int64_t max_listed, int64_t* counts);
+ * x0 (the halo plus a multiple of v).  waves > 0 also sets the lockstep bits as vl_sim_create
+ * does for workgroups of that many waves, launched as far and near tasks apart (overlap != 0) or
+ * as one list (overlap == 0); waves == 0 leaves them clear.  Writes up to max_tasks tasks, 4 int32
+ * each (first output column, first row, rows, output columns | ring-edge bits << 8: first row,
+ * last row, first column, last column | lockstep << 12), far tasks first, and up to max_listed
+ * listed nodes (y << 16 | x); counts = {tasks, far tasks, listed nodes, covered nodes}. */
+vl_status vl_plan_passes(const uint8_t* flags, int nx, int ny, int k, int v, int rows, int x0, int waves,
+                         int overlap, int32_t* tasks, int64_t max_tasks, uint32_t* listed, int64_t max_listed,
+                         int64_t* counts);
+/* The lockstep bits of a pass task list, host only (no reference counterpart; the grouping that
+ * vl_sim_create and vl_batch_build apply before the pass kernels run).  tasks: ntasks x 4 int32
+ * as vl_plan_passes writes them; owner: the case of each task (NULL: one case); ranges: nranges
+ * pairs [begin, end) of the launches, each run as workgroups of `waves` consecutive tasks from
+ * begin.  Unless check_only, sets bit 12 of tasks[4i+3] on exactly the full workgroups whose
+ * tasks share first row, row count and owner, and clears it elsewhere.  Then checks that every
+ * workgroup of every range holds no marked task or only such a full group: VL_EINVAL if not
+ * (a workgroup whose waves met different barriers would hang). */
+vl_status vl_pass_lockstep(int32_t* tasks, const int32_t* owner, int64_t ntasks, const int64_t* ranges, int nranges,
+                           int waves, int check_only);
 
 /* ------------------------------------------------------------------------------------
  * Channel: src/cases/channel.rs:40-155 (a body in a channel; the per-step driver).
@@ -249,6 +262,8 @@ vl_status vl_channel_coefficients(vl_channel* c, double d, double* cd, double* c
  * ---------------------------------------------------------------------------------- */
 typedef struct vl_batch vl_batch;
 typedef struct {
+  /* lattice_updates: node updates performed so far, each case's fluid nodes times the steps it
+   * completed (a retired case counts the steps it ran). */
   int64_t ncases, total_nodes, fluid_nodes, lattice_updates, max_steps, t;
   int32_t boundary_tiles, bulk_tiles, tile_w, tile_h, grid_bulk, grid_boundary;
   double last_run_seconds;

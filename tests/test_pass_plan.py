@@ -129,6 +129,101 @@ def test_plan_of_the_channel_with_a_cylinder():
 def test_plan_rejects_bad_arguments():
     fl = channel_flags(50, 20, [])
     be = vl.backend()
-    for k, v, x0 in ((1, 1, 32), (6, 1, 32), (3, 3, 32), (4, 1, 2)):
+    for k, v, x0 in ((1, 1, 32), (6, 1, 32), (3, 3, 32), (4, 1, 2), (3, 2, 33), (4, 2, 5)):
         with pytest.raises(api.VortexError):
             api.plan_passes(be, fl, k, v, 20, x0)
+
+
+def launch_groups(ranges, waves):
+    """The workgroups of launches over `ranges`: W consecutive tasks from each begin."""
+    for b, e in ranges:
+        for g in range(b, e, waves):
+            yield g, min(g + waves, e)
+
+
+def check_lockstep(tasks, ranges, waves, owner=None):
+    """Every workgroup that holds a lockstep task is full, all its tasks are marked and share
+    first row, row count and owner (so its waves meet the same barriers); every full group of
+    that kind is marked; no marked task is left out of every launch."""
+    lock = (tasks[:, 3] >> 12) & 1
+    launched = np.zeros(len(tasks), bool)
+    nmarked = 0
+    for g0, g1 in launch_groups(ranges, waves):
+        launched[g0:g1] = True
+        grp = tasks[g0:g1]
+        same = (g1 - g0 == waves and (grp[:, 1] == grp[0, 1]).all() and (grp[:, 2] == grp[0, 2]).all()
+                and (owner is None or (owner[g0:g1] == owner[g0]).all()))
+        if lock[g0:g1].any():
+            assert lock[g0:g1].all() and same, f"workgroup at {g0}: waves would meet different barriers"
+            nmarked += 1
+        else:
+            assert not same, f"workgroup at {g0}: could sweep in step but is not marked"
+    assert not (lock.astype(bool) & ~launched).any()
+    return nmarked
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("k,v,rows,waves", [(5, 1, 20, 8), (4, 1, 16, 4), (3, 2, 24, 8), (2, 1, 3, 8)])
+@pytest.mark.parametrize("overlap", [True, False])
+def test_lockstep_groups_of_the_launched_ranges(seed, k, v, rows, waves, overlap):
+    """vl_sim_create's lockstep bits hold over the ranges it launches: far and near tasks apart
+    when they overlap, one list otherwise (also when the far count is no multiple of the
+    workgroup width)."""
+    fl = random_case(100 + seed)
+    tasks, nfar, _, _ = api.plan_passes(vl.backend(), fl, k, v, rows, 32, waves=waves, overlap=overlap)
+    ranges = [(0, nfar), (nfar, len(tasks))] if overlap else [(0, len(tasks))]
+    check_lockstep(tasks, ranges, waves)
+    # the same bits re-derived by the host-only entry agree, and pass its own check
+    again = api.pass_lockstep(vl.backend(), tasks, ranges, waves)
+    assert np.array_equal(again, tasks)
+    api.pass_lockstep(vl.backend(), tasks, ranges, waves, check_only=True)
+
+
+@pytest.mark.parametrize("nx,ny,r,k,rows", [(1400, 300, 20, 5, 20), (800, 200, 12, 4, 16), (1200, 400, 50, 3, 12)])
+def test_lockstep_check_rejects_groups_across_launch_ranges(nx, ny, r, k, rows):
+    """Bits grouped over the far / near split but launched as one list (the mistake the check
+    guards against) leave a workgroup whose waves meet different barriers: on these channels the
+    far count is no multiple of 8."""
+    be = vl.backend()
+    fl = channel_flags(nx, ny, [(0.3 * nx, 0.5 * ny, r)])
+    tasks, nfar, _, _ = api.plan_passes(be, fl, k, 1, rows, 32, waves=8, overlap=True)
+    assert nfar % 8 != 0
+    with pytest.raises(AssertionError):
+        check_lockstep(tasks, [(0, len(tasks))], 8)
+    with pytest.raises(api.VortexError):
+        api.pass_lockstep(be, tasks, [(0, len(tasks))], 8, check_only=True)
+    api.pass_lockstep(be, tasks, [(0, nfar), (nfar, len(tasks))], 8, check_only=True)
+    # planned for one launch (overlap off), the bits are consistent with that launch
+    one, nfar1, _, _ = api.plan_passes(be, fl, k, 1, rows, 32, waves=8, overlap=False)
+    assert nfar1 == nfar and check_lockstep(one, [(0, len(one))], 8) > 0
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_lockstep_groups_of_a_batch(seed):
+    """The batch's task list: each case's tasks in (row, column) order, cases one after another;
+    a group's pass launches a prefix ending at a case boundary.  Marked workgroups never span two
+    cases, nor a prefix end."""
+    be = vl.backend()
+    rng = np.random.default_rng(seed)
+    parts, owner = [], []
+    for c in range(int(rng.integers(3, 9))):
+        fl = random_case(300 + 10 * seed + c)
+        t, _, _, _ = api.plan_passes(be, fl, 5, 1, 20, 32)
+        t = t[np.lexsort((t[:, 0], t[:, 1]))]
+        parts.append(t)
+        owner += [c] * len(t)
+    tasks = np.concatenate(parts)
+    owner = np.array(owner, dtype=np.int32)
+    ends = np.cumsum([len(p) for p in parts])
+    ranges = [(0, int(e)) for e in ends]
+    marked = api.pass_lockstep(be, tasks, ranges, 8, owner=owner)
+    for r in ranges:
+        check_lockstep(marked, [r], 8, owner)
+    # without the owner, groups of equal rows may span two cases: the check with owners refuses
+    # such bits whenever they occur
+    loose = api.pass_lockstep(be, tasks, ranges, 8)
+    spans = any(len(set(owner[g:g + 8])) > 1 and (loose[g:g + 8, 3] >> 12 & 1).all()
+                for g in range(0, len(tasks) - 7, 8))
+    if spans:
+        with pytest.raises(api.VortexError):
+            api.pass_lockstep(be, loose, ranges, 8, owner=owner, check_only=True)

@@ -1,0 +1,101 @@
+"""Closed-form checks of the seeded star-polygon generator and of the batched sweep's case
+ranges (the generator has no reference counterpart; its body is solved through the polygon
+predicate of src/geometry.rs:30-49, an even-odd ray test)."""
+import math
+
+import numpy as np
+
+import vortexlab_amd as vl
+from vortexlab_amd import api, workloads
+
+
+def edge_distance(pts, cx, cy):
+    """Smallest distance from (cx, cy) to the polygon's edges (the largest centred disc inside a
+    polygon that is star-shaped about the centre)."""
+    best = math.inf
+    n = len(pts)
+    for i in range(n):
+        ax, ay = pts[i] - (cx, cy)
+        bx, by = pts[(i + 1) % n] - (cx, cy)
+        ex, ey = bx - ax, by - ay
+        t = min(1.0, max(0.0, -(ax * ex + ay * ey) / (ex * ex + ey * ey)))
+        best = min(best, math.hypot(ax + t * ex, ay + t * ey))
+    return best
+
+
+def test_star_polygon_vertex_count_radii_and_angles():
+    rng = np.random.default_rng(11)
+    counts = set()
+    for _ in range(400):
+        d = float(rng.uniform(10.0, 60.0))
+        cx, cy = float(rng.uniform(50, 500)), float(rng.uniform(40, 300))
+        pts = workloads.star_polygon(rng, d, cx, cy)
+        n = len(pts)
+        counts.add(n)
+        assert 8 <= n <= 24
+        r = np.hypot(pts[:, 0] - cx, pts[:, 1] - cy)
+        assert r.min() >= 0.6 * d / 2 * (1 - 1e-12) and r.max() <= 1.0 * d / 2 * (1 + 1e-12)
+        # evenly spaced angles, counter-clockwise: every step is 2 pi / n
+        a = np.unwrap(np.arctan2(pts[:, 1] - cy, pts[:, 0] - cx))
+        assert np.allclose(np.diff(a), 2 * np.pi / n, atol=1e-9)
+        # shoelace area = the star's closed form 1/2 sum r_i r_(i+1) sin(2 pi / n)
+        x, y = pts[:, 0], pts[:, 1]
+        shoelace = 0.5 * float(np.sum(x * np.roll(y, -1) - np.roll(x, -1) * y))
+        star = 0.5 * float(np.sum(r * np.roll(r, -1))) * math.sin(2 * math.pi / n)
+        assert abs(shoelace - star) <= 1e-9 * star
+    # every vertex count of the range occurs over 400 draws (each has odds 1/17)
+    assert counts == set(range(8, 25))
+
+
+def test_star_polygon_predicate_agrees_with_its_radii():
+    """Points nearer the centre than every edge are inside, points beyond the largest radius
+    outside (vl_shape_polygon, the predicate the solver's links bisect)."""
+    be = vl.backend()
+    rng = np.random.default_rng(5)
+    for _ in range(40):
+        d = float(rng.uniform(10.0, 60.0))
+        cx, cy = 300.5, 200.0
+        pts = workloads.star_polygon(rng, d, cx, cy)
+        rin = edge_distance(pts, cx, cy)
+        rout = float(np.hypot(pts[:, 0] - cx, pts[:, 1] - cy).max())
+        # the polygon is star-shaped about its centre: that disc reaches at least the inscribed
+        # circle of the regular n-gon of the smallest radius
+        assert rin >= 0.6 * d / 2 * math.cos(math.pi / len(pts)) * (1 - 1e-12)
+        h = api.polygon(pts).make(be)
+        for _ in range(200):
+            ang = rng.uniform(0, 2 * math.pi)
+            rr = rng.uniform(0, 1.3) * rout
+            x, y = cx + rr * math.cos(ang), cy + rr * math.sin(ang)
+            got = be.shape_inside(h, x, y)
+            if rr < rin * (1 - 1e-9):
+                assert got == 1
+            elif rr > rout * (1 + 1e-9):
+                assert got == 0
+        # along a ray from the centre there is exactly one outward crossing: sample a ray per
+        # vertex direction, inside then outside, never back inside
+        for k in range(len(pts)):
+            ang = math.atan2(pts[k, 1] - cy, pts[k, 0] - cx) + 0.5 * 2 * math.pi / len(pts)
+            seq = [be.shape_inside(h, cx + s * math.cos(ang), cy + s * math.sin(ang))
+                   for s in np.linspace(0.0, 1.2 * rout, 97)]
+            first_out = seq.index(0)
+            assert all(v == 1 for v in seq[:first_out]) and all(v == 0 for v in seq[first_out:])
+        be.shape_free(h)
+
+
+def test_sweep_cases_span_the_stated_ranges():
+    cases = workloads.sweep_cases(2024, 512)
+    nx = np.array([c.spec.nx for c in cases])
+    ny = np.array([c.spec.ny for c in cases])
+    d = np.array([c.d for c in cases])
+    re = np.array([c.re for c in cases])
+    assert nx.min() >= 200 and nx.max() <= 1400 and ny.min() >= 80 and ny.max() <= 600
+    assert d.min() >= 10 and d.max() <= 60 and re.min() >= 40 and re.max() <= 250
+    bgk = sum(1 for c in cases if c.spec.collision == api.BGK)
+    assert 200 < bgk < 312  # BGK or TRT with equal odds
+    for c in cases:
+        assert c.steps == int(20.0 * c.d / 0.05)  # 20 convective units at u = 0.05
+        assert c.spec.u == 0.05 and abs(c.spec.nu - 0.05 * c.d / c.re) < 1e-15
+        assert 8 <= len(c.outline) <= 24
+    # seeded: the same seed gives the same cases
+    again = workloads.sweep_cases(2024, 512)
+    assert all(np.array_equal(a.outline, b.outline) and a.steps == b.steps for a, b in zip(cases, again))

@@ -120,8 +120,9 @@ _SIGS = [
     ("sim_create", C.c_int, [_VP, C.c_int, C.POINTER(_VP)]),
     ("sim_destroy", None, [_VP]),
     ("sim_info_get", C.c_int, [_VP, C.POINTER(SimInfo)]),
-    ("plan_passes", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _VP, C.c_int64, _VP,
-                              C.c_int64, _VP]),
+    ("plan_passes", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _VP,
+                              C.c_int64, _VP, C.c_int64, _VP]),
+    ("pass_lockstep", C.c_int, [_VP, _VP, C.c_int64, _VP, C.c_int, C.c_int, C.c_int]),
     ("sim_init_macro", C.c_int, [_VP, _D, _D, _D]),
     ("sim_set_vscale", C.c_int, [_VP, _D]),
     ("sim_set_outflow_target", C.c_int, [_VP, C.c_int, C.c_double]),
@@ -195,21 +196,36 @@ class Backend:
             raise VortexError(status, msg)
 
 
-def plan_passes(be: Backend, flags: np.ndarray, k: int, v: int = 1, rows: int = 20, x0: int = 32):
+def plan_passes(be: Backend, flags: np.ndarray, k: int, v: int = 1, rows: int = 20, x0: int = 32, waves: int = 0,
+                overlap: bool = True):
     """The K-step pass plan of a domain (flags: ny x nx, 0 interior, 1 boundary, 2 solid), host
     only: (tasks as an (n, 4) int32 array of first column, first row, rows, columns | ring-edge
-    bits << 8, far tasks first; the number of far tasks; listed nodes as (y << 16 | x); covered
-    node count)."""
+    bits << 8 | lockstep << 12, far tasks first; the number of far tasks; listed nodes as
+    (y << 16 | x); covered node count).  waves > 0 sets the lockstep bits as a simulation with
+    workgroups of that many waves does (far and near tasks launched apart when `overlap`)."""
     fl = np.ascontiguousarray(flags, dtype=np.uint8)
     ny, nx = fl.shape
     counts = np.zeros(4, dtype=np.int64)
     ptr = lambda a: a.ctypes.data_as(C.c_void_p)
-    be.check(be.plan_passes(ptr(fl), nx, ny, k, v, rows, x0, None, 0, None, 0, ptr(counts)))
+    be.check(be.plan_passes(ptr(fl), nx, ny, k, v, rows, x0, waves, int(overlap), None, 0, None, 0, ptr(counts)))
     tasks = np.zeros((max(int(counts[0]), 1), 4), dtype=np.int32)
     listed = np.zeros(max(int(counts[2]), 1), dtype=np.uint32)
-    be.check(be.plan_passes(ptr(fl), nx, ny, k, v, rows, x0, ptr(tasks), int(counts[0]), ptr(listed),
-                            int(counts[2]), ptr(counts)))
+    be.check(be.plan_passes(ptr(fl), nx, ny, k, v, rows, x0, waves, int(overlap), ptr(tasks), int(counts[0]),
+                            ptr(listed), int(counts[2]), ptr(counts)))
     return tasks[:int(counts[0])], int(counts[1]), listed[:int(counts[2])], int(counts[3])
+
+
+def pass_lockstep(be: Backend, tasks: np.ndarray, ranges, waves: int, owner=None, check_only: bool = False):
+    """The lockstep bits of a pass task list (host only): a copy of `tasks` with bit 12 set as the
+    library sets it for launches over `ranges` ([(begin, end), ...], workgroups of `waves` tasks
+    from each begin) and `owner` (the case of each task, or None); raises VortexError if the result
+    (or, with check_only, the given bits) would let one workgroup's waves meet different barriers."""
+    t = np.ascontiguousarray(tasks, dtype=np.int32).copy()
+    r = np.ascontiguousarray(np.array(ranges, dtype=np.int64).reshape(-1, 2))
+    o = None if owner is None else np.ascontiguousarray(owner, dtype=np.int32)
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    be.check(be.pass_lockstep(ptr(t), None if o is None else ptr(o), len(t), ptr(r), len(r), waves, int(check_only)))
+    return t
 
 
 # ---------------------------------------------------------------------------------------

@@ -299,7 +299,7 @@ struct vl_batch {
   // per case (host)
   std::vector<BatchCase> hcases;
   std::vector<Tables> tabs;
-  int64_t total_nodes = 0, total_fluid = 0, total_lu = 0, max_steps = 0;
+  int64_t total_nodes = 0, total_fluid = 0, max_steps = 0;
   // device
   hipStream_t stream = nullptr, side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -777,7 +777,6 @@ vl_status build(vl_batch* b) {
     c.inflow_h = (double)(tab.ny - 2);
     b->total_nodes += (int64_t)tab.nx * tab.ny;
     b->total_fluid += tab.fluid_nodes;
-    b->total_lu += tab.fluid_nodes * p.steps;
     b->max_steps = std::max(b->max_steps, p.steps);
     (void)inlet;
   }
@@ -1117,16 +1116,6 @@ vl_status build(vl_batch* b) {
         b->pass_nodes += pl.covered;
         b->listed_nodes += (int64_t)pl.listed.size();
       }
-    // Lockstep: the waves of a workgroup sweep in step when its tasks belong to one case and
-    // cover the same rows (so they meet the same barriers, and a retired case stops them all).
-    const size_t W = (size_t)b->pass_waves;
-    for (size_t g = 0; g + W <= task.size(); g += W) {
-      bool same = true;
-      for (size_t i = 1; i < W; ++i)
-        same = same && tcase[g + i] == tcase[g] && task[g + i].y == task[g].y && task[g + i].z == task[g].z;
-      if (same)
-        for (size_t i = 0; i < W; ++i) task[g + i].w |= 1 << 12;
-    }
     // per group start t: the tasks, listed tiles and bulk tiles of the cases with steps >= t + K
     const size_t nt = (size_t)b->max_steps + 2;
     vl_batch::TaskList& tl = b->tasks;
@@ -1141,6 +1130,19 @@ vl_status build(vl_batch* b) {
           b->h_ln_at[tt] = lend[(size_t)ci];
           b->h_kfirst_at[tt] = kend[(size_t)ci];
         }
+    // Lockstep: the waves of a workgroup sweep in step when its tasks belong to one case and
+    // cover the same rows (so they meet the same barriers, and a retired case stops them all).
+    // A group's pass launches the prefix of h_n_at[t] tasks from task 0, in workgroups of W.
+    {
+      TaskRanges ranges;
+      std::vector<int32_t> ends(tl.h_n_at.begin(), tl.h_n_at.end());
+      ends.push_back((int32_t)task.size());
+      std::sort(ends.begin(), ends.end());
+      ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
+      for (int32_t e : ends) ranges.emplace_back(0, e);
+      mark_lockstep(task, tcase.data(), ranges, b->pass_waves);
+      VL_TRYB(check_lockstep(task, tcase.data(), ranges, b->pass_waves));
+    }
     tl.n = (int)task.size();
     VL_TRYB(dupload(b, tl.task, task));
     VL_TRYB(dupload(b, tl.tcase, tcase));
@@ -1327,7 +1329,16 @@ vl_status vl_batch_info_get(vl_batch* b, vl_batch_info* o) {
   o->ncases = (int64_t)(b->built ? b->hcases.size() : b->pending.size());
   o->total_nodes = b->total_nodes;
   o->fluid_nodes = b->total_fluid;
-  o->lattice_updates = b->total_lu;
+  // node updates performed so far: each case's fluid nodes times the steps it has completed (a
+  // retired case counts the steps it ran before its drag turned non-finite)
+  o->lattice_updates = 0;
+  if (b->built && !b->hcases.empty()) {
+    VL_HIP(hipStreamSynchronize(b->stream));
+    std::vector<int64_t> done(b->hcases.size());
+    VL_HIP(hipMemcpy(done.data(), b->d_done, done.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    for (size_t c = 0; c < done.size(); ++c)
+      o->lattice_updates += b->tabs[c].fluid_nodes * std::min<int64_t>(done[c], b->hcases[c].steps);
+  }
   o->max_steps = b->max_steps;
   o->t = b->t;
   o->boundary_tiles = b->nbnd_tiles;

@@ -642,26 +642,64 @@ vl_status plan_passes(const std::vector<uint8_t>& fl, int nx, int ny, int64_t pi
   return VL_OK;
 }
 
+static bool lockstep_group_ok(const std::vector<int4>& tasks, const int32_t* owner, int64_t g0, int W) {
+  for (int64_t i = g0 + 1; i < g0 + W; ++i)
+    if (tasks[(size_t)i].y != tasks[(size_t)g0].y || tasks[(size_t)i].z != tasks[(size_t)g0].z ||
+        (owner && owner[i] != owner[g0]))
+      return false;
+  return true;
+}
+
+void mark_lockstep(std::vector<int4>& tasks, const int32_t* owner, const TaskRanges& ranges, int W) {
+  for (int4& t : tasks) t.w &= ~(1 << 12);
+  for (const auto& [begin, end] : ranges)
+    for (int64_t g0 = begin; g0 + W <= end; g0 += W)
+      if (lockstep_group_ok(tasks, owner, g0, W))
+        for (int64_t i = g0; i < g0 + W; ++i) tasks[(size_t)i].w |= 1 << 12;
+}
+
+vl_status check_lockstep(const std::vector<int4>& tasks, const int32_t* owner, const TaskRanges& ranges, int W) {
+  if (W < 1) return set_error(VL_EINVAL, "internal: lockstep workgroups of no waves");
+  std::vector<uint8_t> launched(tasks.size(), 0);
+  for (const auto& [begin, end] : ranges) {
+    if (begin < 0 || end > (int64_t)tasks.size() || begin > end)
+      return set_error(VL_EINVAL, "internal: a pass launch range outside the task list");
+    for (int64_t g0 = begin; g0 < end; g0 += W) {
+      const int64_t g1 = std::min<int64_t>(g0 + W, end);
+      int marked = 0;
+      for (int64_t i = g0; i < g1; ++i) {
+        marked += (tasks[(size_t)i].w >> 12) & 1;
+        launched[(size_t)i] = 1;
+      }
+      if (marked == 0) continue;
+      if (marked != W || g1 - g0 != W || !lockstep_group_ok(tasks, owner, g0, W))
+        return set_error(VL_EINVAL, "internal: a lockstep workgroup whose waves would meet different barriers");
+    }
+  }
+  for (size_t i = 0; i < tasks.size(); ++i)
+    if (((tasks[i].w >> 12) & 1) && !launched[i])
+      return set_error(VL_EINVAL, "internal: a lockstep task outside every launch range");
+  return VL_OK;
+}
+
+// The launch ranges of a domain's task list: far tasks, then near tasks, when the far pass runs
+// on its own stream; otherwise the whole list in one launch.
+static TaskRanges pass_ranges(int64_t ntasks, int64_t nfar, bool overlap) {
+  if (!overlap) return {{0, ntasks}};
+  return {{0, nfar}, {nfar, ntasks}};
+}
+
 static vl_status build_passes(vl_sim* s, int rows) {
   PassPlan plan;
   VL_TRY(plan_passes(s->tab.flag, s->g.nx, s->g.ny, s->g.pitch, s->tb_k, s->tb_v, rows, s->tb_x0, plan));
   std::vector<int4>& tasks = plan.tasks;
   const int nfar = plan.nfar;
   s->tb_far_nodes = plan.far_nodes;
-  // Lockstep bit: set on the tasks of a workgroup when they cover the same rows, so that its
-  // waves execute the same number of barriers.  Far and near tasks are launched separately,
-  // each set in workgroups of its own.
-  if (s->tb_sync) {
-    const size_t per = (size_t)s->tb_waves;
-    for (const auto [first, last] : {std::pair<size_t, size_t>(0, (size_t)nfar), {(size_t)nfar, tasks.size()}})
-      for (size_t b = first; b + per <= last; b += per) {
-        bool same = true;
-        for (size_t i = 1; i < per; ++i)
-          same = same && tasks[b + i].y == tasks[b].y && tasks[b + i].z == tasks[b].z;
-        if (same)
-          for (size_t i = 0; i < per; ++i) tasks[b + i].w |= 1 << 12;
-      }
-  }
+  // Lockstep bit, over the ranges that launch_group launches (far and near tasks separately
+  // when they overlap, else the whole list).
+  const TaskRanges ranges = pass_ranges((int64_t)tasks.size(), nfar, s->tb_overlap);
+  if (s->tb_sync) mark_lockstep(tasks, nullptr, ranges, s->tb_waves);
+  VL_TRY(check_lockstep(tasks, nullptr, ranges, s->tb_waves));
   s->tb_rows = rows;
   s->ntasks = (int)tasks.size();
   s->ntasks_far = nfar;
@@ -1055,13 +1093,22 @@ vl_status vl_sim_create(vl_builder* b, int precision, vl_sim** out) {
 
 void vl_sim_destroy(vl_sim* s) { free_sim(s); }
 
-vl_status vl_plan_passes(const uint8_t* flags, int nx, int ny, int k, int v, int rows, int x0, int32_t* tasks,
-                         int64_t max_tasks, uint32_t* listed, int64_t max_listed, int64_t* counts) {
-  if (!flags || !counts || nx < 3 || ny < 3 || k < 2 || k > TB_MAX_K || (v != 1 && v != 2) || x0 < tb_halo(k, v))
-    return set_error(VL_EINVAL, "vl_plan_passes: flags, counts, nx, ny >= 3, k in 2..5, v in 1..2, x0 >= halo");
+vl_status vl_plan_passes(const uint8_t* flags, int nx, int ny, int k, int v, int rows, int x0, int waves,
+                         int overlap, int32_t* tasks, int64_t max_tasks, uint32_t* listed, int64_t max_listed,
+                         int64_t* counts) {
+  if (!flags || !counts || nx < 3 || ny < 3 || k < 2 || k > TB_MAX_K || (v != 1 && v != 2) || x0 < tb_halo(k, v) ||
+      (x0 - tb_halo(k, v)) % v != 0 || waves < 0)
+    return set_error(VL_EINVAL,
+                     "vl_plan_passes: flags, counts, nx, ny >= 3, k in 2..5, v in 1..2, x0 = halo + a multiple of v, "
+                     "waves >= 0");
   PassPlan plan;
   std::vector<uint8_t> fl(flags, flags + (size_t)nx * ny);
   VL_TRY(plan_passes(fl, nx, ny, round_up(nx, 64), k, v, rows, x0, plan));
+  if (waves > 0) {  // the lockstep bits as vl_sim_create sets them
+    const TaskRanges ranges = pass_ranges((int64_t)plan.tasks.size(), plan.nfar, overlap != 0);
+    mark_lockstep(plan.tasks, nullptr, ranges, waves);
+    VL_TRY(check_lockstep(plan.tasks, nullptr, ranges, waves));
+  }
   counts[0] = (int64_t)plan.tasks.size();
   counts[1] = plan.nfar;
   counts[2] = (int64_t)plan.listed.size();
@@ -1077,6 +1124,24 @@ vl_status vl_plan_passes(const uint8_t* flags, int nx, int ny, int k, int v, int
   if (listed)
     for (int64_t i = 0; i < std::min<int64_t>(max_listed, counts[2]); ++i) listed[i] = plan.listed[(size_t)i];
   return VL_OK;
+}
+
+vl_status vl_pass_lockstep(int32_t* tasks, const int32_t* owner, int64_t ntasks, const int64_t* ranges, int nranges,
+                           int waves, int check_only) {
+  if ((!tasks && ntasks > 0) || ntasks < 0 || (!ranges && nranges > 0) || nranges < 0 || waves < 1)
+    return set_error(VL_EINVAL, "vl_pass_lockstep: tasks, ntasks >= 0, ranges, waves >= 1");
+  std::vector<int4> t((size_t)ntasks);
+  for (int64_t i = 0; i < ntasks; ++i)
+    t[(size_t)i] = make_int4(tasks[4 * i], tasks[4 * i + 1], tasks[4 * i + 2], tasks[4 * i + 3]);
+  TaskRanges r;
+  for (int i = 0; i < nranges; ++i) r.emplace_back(ranges[2 * i], ranges[2 * i + 1]);
+  for (const auto& [b, e] : r)
+    if (b < 0 || e > ntasks || b > e) return set_error(VL_EINVAL, "vl_pass_lockstep: a range outside the list");
+  if (!check_only) {
+    mark_lockstep(t, owner, r, waves);
+    for (int64_t i = 0; i < ntasks; ++i) tasks[4 * i + 3] = t[(size_t)i].w;
+  }
+  return check_lockstep(t, owner, r, waves);
 }
 
 vl_status vl_sim_info_get(vl_sim* s, vl_sim_info* o) {

@@ -71,6 +71,17 @@ struct PassPlan {
 vl_status plan_passes(const std::vector<uint8_t>& flags, int nx, int ny, int64_t pitch, int K, int V, int rows,
                       int x0_grid, PassPlan& plan);
 
+// Lockstep bit (1 << 12 in a task's w) of a pass task list.  The list is launched as one or
+// more ranges [begin, end), each as workgroups of W consecutive tasks from `begin` (the last one
+// possibly partial).  The bit lets a workgroup's waves sweep in step, one barrier per row, so it
+// must only be set where every wave of the workgroup executes the same barriers:
+// mark_lockstep sets it on exactly the full workgroups of some range whose W tasks have equal
+// first row and row count (and, with `owner`, belong to one case), and clears it elsewhere;
+// check_lockstep verifies that property for every range and returns VL_EINVAL otherwise.
+using TaskRanges = std::vector<std::pair<int64_t, int64_t>>;
+void mark_lockstep(std::vector<int4>& tasks, const int32_t* owner, const TaskRanges& ranges, int W);
+vl_status check_lockstep(const std::vector<int4>& tasks, const int32_t* owner, const TaskRanges& ranges, int W);
+
 }  // namespace vl
 
 struct vl_sim {
