@@ -169,6 +169,16 @@ vl_status vl_sim_step(vl_sim* s, int64_t nsteps);
 /* nsteps steps with per-step controls ctl[k]; if force_hist is non-NULL it receives
  * Sim::force after every step: force_hist[(k * force_groups + g) * 2 + {0,1}]. */
 vl_status vl_sim_run(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist);
+/* vl_sim_run without any device allocation during the call (no reference counterpart; for callers
+ * that cannot allocate: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as vl_sim_run, bit for bit.  The per-step control table
+ * and the force-history staging live in a fixed scratch area of the library (one per GPU, about
+ * 450 KB, set up when the first simulation is created on that GPU) instead of the handle's own
+ * buffers, and steps are launched directly rather than through step-window graphs (which would be
+ * instantiated on first use).  At most 4096 steps per call and 4 force groups, else VL_EINVAL;
+ * VL_EINVAL also while kernel timing is on.  Every handle on a GPU shares its scratch area: calls
+ * that overlap on one GPU race. */
+vl_status vl_sim_run_noalloc(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist);
 /* Sim::force after the last step (force_groups * 2 doubles). */
 vl_status vl_sim_get_force(vl_sim* s, double* out);
 /* Populations of A (the current state) in the reference layout, as f_i in FP64. */

@@ -128,6 +128,7 @@ _SIGS = [
     ("sim_set_outflow_target", C.c_int, [_VP, C.c_int, C.c_double]),
     ("sim_step", C.c_int, [_VP, C.c_int64]),
     ("sim_run", C.c_int, [_VP, C.c_int64, C.POINTER(Ctl), _D]),
+    ("sim_run_noalloc", C.c_int, [_VP, C.c_int64, C.POINTER(Ctl), _D]),
     ("sim_get_force", C.c_int, [_VP, _D]),
     ("sim_read", C.c_int, [_VP, _D]),
     ("sim_write", C.c_int, [_VP, _D]),

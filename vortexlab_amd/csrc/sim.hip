@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -138,10 +139,10 @@ static StepArgs step_args(vl_sim* s, int64_t t, const int64_t* kbase, int k_loca
   const int64_t ns2 = (int64_t)std::max(dt.nslots, 1) * 2;
   sa.fc_cur = dt.fcontrib + (t & 1) * ns2;
   sa.fc_prev = dt.fcontrib + ((t + 1) & 1) * ns2;
-  sa.ctls = s->d_ctl;
+  sa.ctls = s->run_ctl;
   sa.kbase = kbase;
   sa.k_local = k_local;
-  sa.hist = dt.ngroups > 0 ? s->dhist : nullptr;
+  sa.hist = dt.ngroups > 0 ? s->run_hist : nullptr;
   const int64_t np1 = std::max(dt.np, 1);
   sa.pv_cur = dt.pvel + (t & 1) * np1;
   sa.pv_next = dt.pvel + ((t + 1) & 1) * np1;
@@ -710,6 +711,42 @@ static vl_status build_passes(vl_sim* s, int rows) {
   return VL_OK;
 }
 
+// Scratch area of vl_sim_run_noalloc: the per-step control table and the force-history staging
+// of one call, in place of the buffers a handle allocates for vl_sim_run.  One copy per GPU,
+// shared by every handle on it and held until the process ends; a call leaves its contents
+// behind, and calls that overlap on one GPU race.  Host code never passes `scratch` itself to a
+// kernel or a copy (on the host the name is only a stand-in for the symbol): it uses the device
+// address that noalloc_scratch_setup takes with hipGetSymbolAddress, once per device.
+constexpr int64_t NOALLOC_MAX_STEPS = 4096;
+constexpr int64_t NOALLOC_MAX_GROUPS = 4;
+constexpr size_t NOALLOC_CTL_BYTES = (NOALLOC_MAX_STEPS * sizeof(StepCtl) + 255) / 256 * 256;
+constexpr size_t NOALLOC_HIST_BYTES = NOALLOC_MAX_STEPS * NOALLOC_MAX_GROUPS * 2 * sizeof(double);
+static __device__ __align__(256) char scratch[NOALLOC_CTL_BYTES + NOALLOC_HIST_BYTES];
+static_assert(NOALLOC_MAX_STEPS <= RUN_CHUNK, "the pinned control staging holds RUN_CHUNK steps");
+
+static std::mutex g_scratch_mu;
+static std::vector<char*> g_scratch_ptr;  // device address of `scratch`, per device
+
+// Takes the current device's address of the scratch area (set-up of a simulation on it).
+static vl_status noalloc_scratch_setup(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (g_scratch_ptr[(size_t)device]) return VL_OK;
+  void* p = nullptr;
+  VL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return set_error(VL_EDEVICE, "hipGetSymbolAddress returned no address for the scratch area");
+  g_scratch_ptr[(size_t)device] = (char*)p;
+  return VL_OK;
+}
+
+static vl_status noalloc_scratch(int device, char** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || (size_t)device >= g_scratch_ptr.size() || !g_scratch_ptr[(size_t)device])
+    return set_error(VL_EDEVICE, "internal: the scratch area of this device was not set up");
+  *out = g_scratch_ptr[(size_t)device];
+  return VL_OK;
+}
+
 static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   if (precision != VL_F64 && precision != VL_F32 && precision != VL_F16)
     return set_error(VL_EINVAL, "precision must be VL_F64, VL_F32 or VL_F16");
@@ -737,6 +774,7 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   s->g.plane = round_up((int64_t)s->g.pitch * d.ny, 256) + 2560 / (int64_t)s->elem;
   s->pop_bytes = (size_t)(Q * s->g.plane) * s->elem;
   VL_HIP(hipGetDevice(&s->device));
+  VL_TRY(noalloc_scratch_setup(s->device));
   VL_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   VL_HIP(hipMalloc(&s->buf[0], s->pop_bytes));
   VL_HIP(hipMalloc(&s->buf[1], s->pop_bytes));
@@ -955,14 +993,26 @@ static vl_status get_window_graph(vl_sim* s, hipGraphExec_t* out) {
   return VL_OK;
 }
 
-static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist) {
+// Device buffers of a run: the control table and the force-history staging of chunks of up to
+// `chunk` steps, and whether step-window graphs may be replayed (graphs hold the handle's own
+// buffers, so only runs on those may use them).
+struct RunBufs {
+  StepCtl* ctl;
+  double* hist;
+  int64_t chunk;
+  bool graphs;
+};
+
+static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist, const RunBufs& rb) {
   if (nsteps < 0) return set_error(VL_EINVAL, "negative step count");
   const int64_t ng = s->dt.ngroups;
-  const int64_t chunk = RUN_CHUNK;
+  const int64_t chunk = rb.chunk;
+  s->run_ctl = rb.ctl;
+  s->run_hist = rb.hist;
   if (s->timing) VL_TRY(ensure_events(s));
   if (nsteps > 0 && s->dt.np > 0) VL_TRY(prime_outlet(s));
   // Graph replay is used without kernel timing (events cannot sit inside the window).
-  const bool graphs = s->use_graphs && !s->timing;
+  const bool graphs = rb.graphs && s->use_graphs && !s->timing;
   for (int64_t k0 = 0; k0 < nsteps; k0 += chunk) {
     const int64_t m = std::min(chunk, nsteps - k0);
     // controls of this chunk -> pinned staging -> device table
@@ -979,7 +1029,7 @@ static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double*
       }
       if (s->dt.np == 0) c.has_target = 0;
     }
-    VL_HIP(hipMemcpyAsync(s->d_ctl, s->h_ctl, (size_t)m * sizeof(StepCtl), hipMemcpyHostToDevice, s->stream));
+    VL_HIP(hipMemcpyAsync(s->run_ctl, s->h_ctl, (size_t)m * sizeof(StepCtl), hipMemcpyHostToDevice, s->stream));
     if (s->timing) {
       s->ev_used = 0;
       VL_HIP(hipEventRecord(s->ev_chunk[0], s->stream));
@@ -1002,13 +1052,13 @@ static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double*
       k += n;
     }
     VL_TRY(join_overlap(s));
-    if (ng > 0) VL_TRY(finish_forces(s, s->dhist + (m - 1) * ng * 2));
+    if (ng > 0) VL_TRY(finish_forces(s, s->run_hist + (m - 1) * ng * 2));
     if (s->timing) VL_HIP(hipEventRecord(s->ev_chunk[1], s->stream));
     if (ng > 0) {
       if (force_hist)
-        VL_HIP(hipMemcpyAsync(force_hist + k0 * ng * 2, s->dhist, (size_t)(m * ng * 2) * sizeof(double),
+        VL_HIP(hipMemcpyAsync(force_hist + k0 * ng * 2, s->run_hist, (size_t)(m * ng * 2) * sizeof(double),
                               hipMemcpyDeviceToHost, s->stream));
-      VL_HIP(hipMemcpyAsync(s->last_force.data(), s->dhist + (m - 1) * ng * 2, (size_t)(ng * 2) * sizeof(double),
+      VL_HIP(hipMemcpyAsync(s->last_force.data(), s->run_hist + (m - 1) * ng * 2, (size_t)(ng * 2) * sizeof(double),
                             hipMemcpyDeviceToHost, s->stream));
     }
     VL_HIP(hipStreamSynchronize(s->stream));
@@ -1204,12 +1254,27 @@ vl_status vl_sim_set_outflow_target(vl_sim* s, int enabled, double target) {
 
 vl_status vl_sim_step(vl_sim* s, int64_t nsteps) {
   if (!s) return set_error(VL_EINVAL, "null argument");
-  return run_steps(s, nsteps, nullptr, nullptr);
+  return run_steps(s, nsteps, nullptr, nullptr, RunBufs{s->d_ctl, s->dhist, RUN_CHUNK, true});
 }
 
 vl_status vl_sim_run(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist) {
   if (!s || (!ctl && nsteps > 0)) return set_error(VL_EINVAL, "null argument");
-  return run_steps(s, nsteps, ctl, force_hist);
+  return run_steps(s, nsteps, ctl, force_hist, RunBufs{s->d_ctl, s->dhist, RUN_CHUNK, true});
+}
+
+vl_status vl_sim_run_noalloc(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist) {
+  if (!s || (!ctl && nsteps > 0)) return set_error(VL_EINVAL, "null argument");
+  if (nsteps > NOALLOC_MAX_STEPS)
+    return set_error(VL_EINVAL, "vl_sim_run_noalloc: at most " + std::to_string(NOALLOC_MAX_STEPS) +
+                                    " steps per call (got " + std::to_string(nsteps) + ")");
+  if (s->dt.ngroups > NOALLOC_MAX_GROUPS)
+    return set_error(VL_EINVAL, "vl_sim_run_noalloc: at most " + std::to_string(NOALLOC_MAX_GROUPS) +
+                                    " force groups (the simulation has " + std::to_string(s->dt.ngroups) + ")");
+  if (s->timing) return set_error(VL_EINVAL, "vl_sim_run_noalloc: switch kernel timing off first");
+  char* p = nullptr;
+  VL_TRY(noalloc_scratch(s->device, &p));
+  return run_steps(s, nsteps, ctl, force_hist,
+                   RunBufs{(StepCtl*)p, (double*)(p + NOALLOC_CTL_BYTES), NOALLOC_MAX_STEPS, false});
 }
 
 vl_status vl_device_copy_bandwidth(size_t bytes, int iters, double* gbs) {

@@ -135,6 +135,10 @@ struct vl_sim {
   vl::DevTables dt;
   double* dhist = nullptr;  // device force history of a run chunk (hist_cap steps)
   vl::StepCtl* d_ctl = nullptr;  // controls of a run chunk (device)
+  // The control table and force-history staging the steps of the current run read and write:
+  // d_ctl / dhist, or the library's fixed scratch area for vl_sim_run_noalloc.
+  vl::StepCtl* run_ctl = nullptr;
+  double* run_hist = nullptr;
   vl::StepCtl* h_ctl = nullptr;  // pinned staging for them
   int64_t* d_kbase = nullptr;    // window counter for replayed step graphs
   int64_t* d_kzero = nullptr;    // a device zero (directly launched steps)
