@@ -156,8 +156,9 @@ struct Dev {
     jy = (d[2] - d[4]) + (d[5] - d[8]) + (d[6] - d[7]);
   }
 
+  // The general form (all three models, one pair at a time); the forced model uses it.
   template <int M>
-  __device__ static inline void collide(float* d, float op, float om, float gx, float gy) {
+  __device__ static inline void collide_general(float* d, float op, float om, float gx, float gy) {
 #pragma clang fp contract(on)
     float drho, jx, jy;
     moments(d, drho, jx, jy);
@@ -211,6 +212,79 @@ struct Dev {
     }
     const float eq0 = drho - moving;
     d[0] = d[0] - op * (d[0] - eq0) - source;
+  }
+
+  // BGK and TRT in fewer operations (the pass kernels are bound by vector instruction issue):
+  // the pair sums s = d_a + d_b and differences a = d_a - d_b give the moments and the TRT
+  // symmetric / antisymmetric parts at once; c.u of the four first members is ux, uy, ux + uy,
+  // uy - ux (no products with zero components); the equilibrium deviation of a pair,
+  //   eqs = w (drho + rho (4.5 (c.u)^2 - 1.5 u^2)) = w drho + 1.5 w rho t,  t = 3 (c.u)^2 - u^2,
+  //   eqa = w rho 3 c.u,
+  // enters with the relaxation rates folded into its per-weight factors.  The updates keep the
+  // difference form (the non-equilibrium part is formed before it is scaled), and the rest
+  // population takes the remainder drho - sum of the moving equilibria, so mass is conserved as
+  // in the reference's "rho - moving".
+  template <int M>
+  __device__ static inline void collide(float* d, float op, float om, float gx, float gy) {
+    if constexpr (M == MODEL_FORCED) {
+      collide_general<M>(d, op, om, gx, gy);
+    } else {
+#pragma clang fp contract(on)
+      constexpr float WA = 1.0f / 9.0f, WD = 1.0f / 36.0f;
+      const float s13 = d[1] + d[3], s24 = d[2] + d[4], s57 = d[5] + d[7], s68 = d[6] + d[8];
+      const float a13 = d[1] - d[3], a24 = d[2] - d[4], a57 = d[5] - d[7], a68 = d[6] - d[8];
+      const float drho = ((d[0] + s13) + (s24 + s57)) + s68;
+      const float jx = a13 + (a57 - a68), jy = a24 + (a57 + a68);
+      const float rho = 1.0f + drho;
+      // the hardware reciprocal (1 ulp): one instruction instead of the ~10 of a correctly
+      // rounded FP32 division; every kernel shares this operator, so they agree bit for bit
+      const float inv = __builtin_amdgcn_rcpf(rho);
+      const float ux = jx * inv, uy = jy * inv;
+      const float uu = ux * ux + uy * uy;
+      const float sp[4] = {s13, s24, s57, s68}, ap[4] = {a13, a24, a57, a68};
+      // c.u of the first member of pair p (1, 2, 5, 6), formed where the pair is updated so that
+      // only one pair's terms are live at a time
+      auto cu_of = [&](int p) { return p == 0 ? ux : (p == 1 ? uy : (p == 2 ? ux + uy : uy - ux)); };
+      if constexpr (M == MODEL_BGK) {
+        float moving = 0.0f;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const float w = p < 2 ? WA : WD;
+          const int ia = pair_a(p), ib = pair_b(p);
+          const float c = cu_of(p), k = 3.0f * c, t = k * c - uu;
+          const float eqs = (1.5f * w * rho) * t + w * drho;
+          const float eqa = (w * rho) * k;
+          const float ea = eqs + eqa, eb = eqs - eqa;
+          d[ia] = d[ia] - op * (d[ia] - ea);
+          d[ib] = d[ib] - op * (d[ib] - eb);
+          moving = 2.0f * eqs + moving;
+        }
+        d[0] = d[0] - op * (d[0] - (drho - moving));
+      } else {
+        const float hp = 0.5f * op, hm = 0.5f * om;
+        float tsum[2];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          const float w = p < 2 ? WA : WD;
+          const int ia = pair_a(p), ib = pair_b(p);
+          const float c = cu_of(p), k = 3.0f * c, t = k * c - uu;
+          if (p % 2 == 0)
+            tsum[p / 2] = t;
+          else
+            tsum[p / 2] = tsum[p / 2] + t;
+          const float ope = ((1.5f * op * w) * rho) * t + (op * w) * drho;  // op * eqs
+          const float ome = ((om * w) * rho) * k;                            // om * eqa
+          const float dp = hp * sp[p] - ope;                                 // op (s/2 - eqs)
+          const float dm = hm * ap[p] - ome;                                 // om (a/2 - eqa)
+          d[ia] = d[ia] - dp - dm;
+          d[ib] = d[ib] - dp + dm;
+        }
+        // sum of the moving equilibria: 2 sum eqs = rho (3 WA (t1 + t2) + 3 WD (t5 + t6)) + 4 (WA + WD) drho
+        const float m = (3.0f * WA) * tsum[0] + (3.0f * WD) * tsum[1];
+        const float moving = rho * m + (4.0f * (WA + WD)) * drho;
+        d[0] = d[0] - op * (d[0] - (drho - moving));
+      }
+    }
   }
 };
 

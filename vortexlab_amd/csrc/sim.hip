@@ -70,9 +70,11 @@ __global__ __launch_bounds__(BULK_T) void k_bulk(const typename S::T* __restrict
 }
 
 // K steps of the covered interior in one pass: one task per wave, W waves per workgroup, V
-// columns per lane, D rows of loads in flight.
+// columns per lane, D rows of loads in flight.  FP32 and FP16 storage keep 4 waves per SIMD
+// (at most 128 registers; the FP16 pass needed 130 unbounded, and 3 waves ran 11 % slower);
+// FP64 needs about 185 and runs at 2.
 template <class S, int M, int W, int V, int D, int K>
-__global__ __launch_bounds__(W * TB_LANES) void k_pass(const typename S::T* __restrict__ A,
+__global__ __launch_bounds__(W * TB_LANES, sizeof(typename S::T) < 8 ? 4 : 1) void k_pass(const typename S::T* __restrict__ A,
                                                       PassOut<typename S::T> po, const int4* __restrict__ tasks,
                                                       int ntasks, Geom g, Coll c) {
   const int w = __builtin_amdgcn_readfirstlane((int)blockIdx.x * W + (int)(threadIdx.x / TB_LANES));

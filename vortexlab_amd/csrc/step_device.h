@@ -660,6 +660,32 @@ __device__ inline T elem(const typename Pack<T, V>::type& v, int j) {
     return v[j];
 }
 
+// Store of V consecutive values to plane i of a population array at a wave-uniform row base plus
+// a per-lane byte offset, as a buffer store.  The plane's row base goes into a resource
+// descriptor built by scalar instructions, so no vector instruction forms a 64-bit address (the
+// pass kernels are bound by vector instruction issue).  Word 3 is the gfx9 raw-buffer format
+// (32-bit data format); the record count spans 4 GiB from the row base, past any byte offset a
+// lane uses.
+template <class T, int V>
+__device__ inline void store_plane(T* rowbase, int64_t plane, int i, uint32_t byte_off,
+                                   typename Pack<T, V>::type v) {
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(rowbase + i * plane), (short)0, -1, 0x00020000);
+  constexpr int B = (int)sizeof(T) * V;
+  if constexpr (B == 2) {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, byte_off, 0, 0);
+  } else if constexpr (B == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, byte_off, 0, 0);
+  } else {
+    static_assert(B == 8 || B == 16, "store width");
+    using U = unsigned __attribute__((ext_vector_type(B / 4)));
+    if constexpr (B == 8)
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, byte_off, 0, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U, v), r, byte_off, 0, 0);
+  }
+}
+
 // Output of a pass: level K goes to `out`, levels 1..K-1 of the ring nodes to `ring[L - 1]`.
 template <class T>
 struct PassOut {
@@ -805,7 +831,7 @@ __device__ __attribute__((always_inline)) inline void multi_step_strip(const int
 #pragma unroll
               for (int i = 1; i < Q; ++i)
                 if ((ring_lo && cy(i) < 0) || (ring_hi && cy(i) > 0) || (rl && cx(i) < 0) || (rr && cx(i) > 0))
-                  *plane_at(rowp, P, i, V * lane + j) = S::enc(e[i][j]);
+                  store_plane<T, 1>(rowp, P, i, (uint32_t)(V * lane + j) * (uint32_t)sizeof(T), S::enc(e[i][j]));
             }
           } else {
 #pragma unroll
@@ -820,7 +846,7 @@ __device__ __attribute__((always_inline)) inline void multi_step_strip(const int
         if (L == K && out_lane) {
           T* rowp = po.out + (int64_t)s * pitch + xs;
 #pragma unroll
-          for (int i = 0; i < Q; ++i) *(gptr<PT>)plane_at(rowp, P, i, V * lane) = outv[i];
+          for (int i = 0; i < Q; ++i) store_plane<T, V>(rowp, P, i, (uint32_t)(V * lane) * (uint32_t)sizeof(T), outv[i]);
         }
       }
       st[L - 1].push(fr);
