@@ -52,15 +52,14 @@ def test_training_is_reproducible(gpu):
 def test_surrogate_gains_over_solver_alone(gpu):
     """Small end-to-end run: solver-labelled training set from the batched path, a 4-member
     ensemble, held-out shapes.  The ensemble must predict the drag better than the training
-    mean does, its spread must rank its own errors (the quarter of the answers it is surest of
-    is more accurate than the quarter it is least sure of), and an answer must cost less than
-    a batched solve of the same case.  At this test's 10 cells per diameter the cases are tiny
+    mean does; the fallback threshold on its spread, calibrated on half the held-out shapes for
+    a 90th-percentile relative Cd error of 4 %, must avoid a share of the solves on the other half
+    and hold the stated error there; and an answer must cost less than a
+    batched solve of the same case.  At this test's 10 cells per diameter the cases are tiny
     (160 x 60 nodes) and a batched solve costs well under a millisecond; at the production
     resolution the gap is two orders of magnitude (the full-size surrogate report)."""
     # 10 cells per diameter is resolved up to Re ~ 90 (the solver itself diverges near 117)
     kw = dict(d=10.0, units=24.0, re=(30.0, 90.0), upstream=4.0, downstream=12.0, height=6.0)
-    # 1536 training and 384 held-out shapes: the self-judgement below compares two halves of
-    # the held-out set, which takes a few hundred shapes to be more than noise
     tr = surrogate.make_dataset(gpu, 21, 1536, **kw)
     te = surrogate.make_dataset(gpu, 22, 384, **kw)
     assert tr.ok.mean() > 0.8 and te.ok.mean() > 0.8
@@ -73,19 +72,25 @@ def test_surrogate_gains_over_solver_alone(gpu):
     truth = te.y[te.ok]
     err_net = np.sqrt(np.mean((mean[:, 0] - truth[:, 0]) ** 2))
     err_mean = np.sqrt(np.mean((ens.y_mu[0] - truth[:, 0]) ** 2))
-    print(f"[surrogate] Cd rmse {err_net:.4f} vs mean predictor {err_mean:.4f}; "
-          f"accepted at 2 % {surrogate.accept(mean, std, ens.y_sd).mean():.2f}, "
-          f"at 5 % {surrogate.accept(mean, std, ens.y_sd, 0.05).mean():.2f}", flush=True)
     assert err_net < 0.6 * err_mean
-    # the ensemble judges its own error: the answers with the smallest ensemble spread are more
-    # accurate than the ones with the largest, which it would send back to the solver
-    spread = std[:, 0] / np.abs(mean[:, 0])
-    lo, hi = np.quantile(spread, [0.25, 0.75])
     rel = np.abs(mean[:, 0] - truth[:, 0]) / np.abs(truth[:, 0])
-    surest, least = np.median(rel[spread <= lo]), np.median(rel[spread >= hi])
+    spread = surrogate.spread_of(mean, std, ens.y_sd)
+    half = len(truth) // 2
+    target = 0.04
+    th = surrogate.calibrate_threshold(spread[:half], rel[:half], target)
+    cal = surrogate.fallback_report(spread[:half], rel[:half], th)
+    ev = surrogate.fallback_report(spread[half:], rel[half:], th)
     per_answer, per_solve = t_pred / len(truth), te.solve_seconds / len(te.ok)
-    print(f"[surrogate] median Cd error: surest quarter {surest:.4f}, least sure quarter {least:.4f}; "
-          f"{per_answer * 1e6:.0f} us per answer, {per_solve * 1e6:.0f} us per batched solve", flush=True)
-    assert surest < least
+    print(f"[surrogate] Cd rmse {err_net:.4f} vs mean predictor {err_mean:.4f}; threshold {th:.4f}: calibration "
+          f"{cal}; evaluation {ev}; {per_answer * 1e6:.0f} us per answer, {per_solve * 1e6:.0f} us per batched solve",
+          flush=True)
+    assert cal["p90_rel_err_accepted"] is not None and cal["p90_rel_err_accepted"] <= target
+    # the rule avoids a share of the solves on the other half...
+    assert ev["accepted_fraction"] >= 0.15
+    # ...and holds its stated error there: over all queries (the rejected ones are solved) and,
+    # with room for 190 answers' sampling noise, over the accepted answers alone
+    assert ev["p90_rel_err_all_queries"] <= target
+    assert ev["p90_rel_err_accepted"] <= 1.5 * target
+    assert ev["p90_rel_err_all_queries"] <= ev["p90_rel_err_surrogate_alone"]
     assert per_answer < per_solve
     ens.close()

@@ -61,3 +61,22 @@ def test_acceptance_rule():
     std = np.array([[0.01, 0.003, 0.002], [0.05, 0.003, 0.002]])
     acc = surrogate.accept(mean, std, np.array([0.5, 0.2, 0.05]), 0.02)
     assert acc.tolist() == [True, False]
+
+
+def test_calibrated_threshold_meets_the_stated_error():
+    """calibrate_threshold returns the largest spread threshold whose accepted answers meet the
+    target 90th-percentile error: answers sorted by spread whose errors grow with it."""
+    rng = np.random.default_rng(9)
+    spread = rng.uniform(0.0, 0.1, size=400)
+    err = spread * rng.uniform(0.5, 1.5, size=400)  # errors grow with the spread
+    th = surrogate.calibrate_threshold(spread, err, 0.05)
+    acc = spread <= th
+    assert np.quantile(err[acc], 0.9) <= 0.05
+    # one more answer (the next larger spread) would break the target
+    nxt = np.sort(spread[~acc])[0]
+    assert np.quantile(err[spread <= nxt], 0.9) > 0.05
+    # nothing meets an impossible target; everything meets a lax one
+    assert surrogate.calibrate_threshold(spread, err + 1.0, 0.05) == 0.0
+    assert surrogate.calibrate_threshold(spread, err, 1.0) == spread.max()
+    rep = surrogate.fallback_report(spread, err, th)
+    assert rep["accepted_fraction"] == acc.mean() and rep["p90_rel_err_all_queries"] <= rep["p90_rel_err_surrogate_alone"]
