@@ -286,6 +286,11 @@ typedef struct {
    * K steps per pass and its other interior nodes (listed_nodes) one step at a time. */
   int32_t pass_steps, pass_rows, pass_tasks, listed_tiles;
   int64_t pass_nodes, listed_nodes;
+  /* Set-up (vl_batch_build): 1 when the cell kinds, node flags and boundary tables were built on
+   * the device (every body a polygon or a circle), 0 when by host threads; the time spent on the
+   * cases and their tables, the K-step pass plans, packing and uploads, and the autotuner (ms). */
+  int32_t setup_on_device, pad_;
+  double setup_ms[4];
 } vl_batch_info;
 vl_status vl_batch_create(int precision, vl_batch** out);
 /* Adds a case (the channel of vl_channel_create; `body` consumed) that runs `steps` steps;
@@ -311,6 +316,13 @@ vl_status vl_batch_read_case(vl_batch* b, int id, double* f);
 /* Grid, fluid-node count and requested steps of case `id` (after vl_batch_build); any output
  * pointer may be NULL.  Like Sim::fluid_nodes (src/sim.rs:640) for one case of the batch. */
 vl_status vl_batch_case_info(vl_batch* b, int id, int64_t* nx, int64_t* ny, int64_t* fluid_nodes, int64_t* steps);
+/* FNV-1a digests of case `id`'s device tables after vl_batch_build, for comparing two builds (no
+ * reference counterpart): its pitched node flags, then bx, by, bfirst, dir, kind, vgroup, lflags,
+ * lnx, lny, fslot, q, uwx, uwy, cu6, rho, px, py, pnx, pny, gfirst, and the counts (nodes, links,
+ * outlet nodes, outlet-controller nodes, force slots, force groups, fluid nodes): 22 values, the
+ * first n written.  Equal digests for the device and the host set-up (VL_BATCH_SETUP=host) of the
+ * same case mean byte-identical tables. */
+vl_status vl_batch_table_digests(vl_batch* b, int id, uint64_t* out, int n);
 void vl_batch_destroy(vl_batch* b);
 
 /* ------------------------------------------------------------------------------------

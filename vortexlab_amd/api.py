@@ -79,7 +79,8 @@ class BatchInfoDesc(C.Structure):
                 ("last_run_seconds", C.c_double), ("device_bytes", C.c_int64),
                 ("autotune_ms", (C.c_double * 4) * 3),
                 ("pass_steps", C.c_int32), ("pass_rows", C.c_int32), ("pass_tasks", C.c_int32),
-                ("listed_tiles", C.c_int32), ("pass_nodes", C.c_int64), ("listed_nodes", C.c_int64)]
+                ("listed_tiles", C.c_int32), ("pass_nodes", C.c_int64), ("listed_nodes", C.c_int64),
+                ("setup_on_device", C.c_int32), ("pad_", C.c_int32), ("setup_ms", C.c_double * 4)]
 
 
 class VortexError(RuntimeError):
@@ -162,6 +163,7 @@ _SIGS = [
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("batch_set_timing", C.c_int, [_VP, C.c_int]),
     ("batch_get_timing", C.c_int, [_VP, _D, _D, _D, C.POINTER(C.c_int64)]),
+    ("batch_table_digests", C.c_int, [_VP, C.c_int, C.POINTER(C.c_uint64), C.c_int]),
     ("batch_destroy", None, [_VP]),
 ]
 
@@ -694,6 +696,15 @@ class Batch:
         out = np.zeros((Q, ny, nx))
         self.be.check(self.be.batch_read_case(self.h, k, _dptr(out)))
         return out
+
+    TABLE_NAMES = ("flags", "bx", "by", "bfirst", "dir", "kind", "vgroup", "lflags", "lnx", "lny", "fslot", "q",
+                   "uwx", "uwy", "cu6", "rho", "px", "py", "pnx", "pny", "gfirst", "counts")
+
+    def table_digests(self, k: int) -> dict:
+        """FNV-1a digests of case k's device tables, by name (vl_batch_table_digests)."""
+        out = (C.c_uint64 * len(self.TABLE_NAMES))()
+        self.be.check(self.be.batch_table_digests(self.h, k, out, len(out)))
+        return dict(zip(self.TABLE_NAMES, list(out)))
 
     def close(self):
         if self.h:

@@ -28,6 +28,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "batch_setup.h"
 #include "step_device.h"
 
 namespace vl {
@@ -298,8 +299,10 @@ struct vl_batch {
   bool built = false;
   // per case (host)
   std::vector<BatchCase> hcases;
-  std::vector<Tables> tabs;
+  std::vector<Tables> tabs;  // with device set-up: only nx, ny, fluid_nodes and flag are filled
   int64_t total_nodes = 0, total_fluid = 0, max_steps = 0;
+  bool setup_on_device = false;  // cell kinds, flags and boundary tables built by the set-up kernels
+  double setup_ms[4] = {};      // set-up time: cases and tables, pass plans, packing and uploads, autotuning
   // device
   hipStream_t stream = nullptr, side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -717,13 +720,38 @@ vl_status build(vl_batch* b) {
   if (nc == 0) return set_error(VL_EINVAL, "batch has no cases");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(VL_EDEVICE, "no HIP device");
-  // ---- host set-up of every case, in parallel ----
+  using clk = std::chrono::steady_clock;
+  auto t_mark = clk::now();
+  auto lap = [&](int k) {
+    const auto now = clk::now();
+    b->setup_ms[k] += std::chrono::duration<double, std::milli>(now - t_mark).count();
+    t_mark = now;
+  };
+  VL_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  VL_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  VL_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+  VL_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+  // ---- set-up of every case: on the device when every body is a polygon or a circle (the set-up
+  // kernels evaluate those predicates themselves), else on host threads; VL_BATCH_SETUP=host
+  // forces the host builder ----
   b->tabs.assign(nc, Tables{});
   std::vector<vl_velocity> inlet(nc);
   std::vector<int32_t> npb(nc, 0);
   std::vector<vl_status> st(nc, VL_OK);
   std::vector<std::string> err(nc);
-  {
+  std::vector<ShapeDesc> desc(nc);
+  b->setup_on_device = true;
+  if (const char* e = std::getenv("VL_BATCH_SETUP")) b->setup_on_device = std::string(e) != "host";
+  for (size_t i = 0; i < nc && b->setup_on_device; ++i) {
+    desc[i] = b->pending[i].body->s->describe();
+    b->setup_on_device = desc[i].kind != ShapeDesc::NONE;
+  }
+  if (b->setup_on_device) {
+    for (size_t i = 0; i < nc; ++i) {
+      b->tabs[i].nx = b->pending[i].spec.nx;
+      b->tabs[i].ny = b->pending[i].spec.ny;
+    }
+  } else {
     const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> pool;
     for (unsigned w = 0; w < nth; ++w)
@@ -776,10 +804,64 @@ vl_status build(vl_batch* b) {
     c.inflow_u = p.spec.u;
     c.inflow_h = (double)(tab.ny - 2);
     b->total_nodes += (int64_t)tab.nx * tab.ny;
-    b->total_fluid += tab.fluid_nodes;
     b->max_steps = std::max(b->max_steps, p.steps);
     (void)inlet;
   }
+  // ---- device set-up: flags straight into the batch's pitched flag array, tables into packed
+  // pools at the host builder's offsets; the flags come back for the pass planner ----
+  SetupTables dtab{};
+  std::vector<SetupCounts> dcount;
+  if (b->setup_on_device) {
+    std::vector<SetupCase> sc(nc);
+    std::vector<double> verts;
+    for (size_t i = 0; i < nc; ++i) {
+      const auto& sp = b->pending[i].spec;
+      const BatchCase& c = b->hcases[i];
+      SetupCase& k = sc[i];
+      k = SetupCase{};
+      k.nx = c.g.nx;
+      k.ny = c.g.ny;
+      k.pitch = c.g.pitch;
+      k.inflow = sp.inflow;
+      k.cell0 = c.flag_off;
+      k.u = sp.u;
+      k.h = (double)(sp.ny - 2);
+      k.has_kick = sp.has_kick != 0;
+      k.kcx = sp.kick.cx;
+      k.kcy = sp.kick.cy;
+      k.body = desc[i].kind;
+      k.bcx = desc[i].cx;
+      k.bcy = desc[i].cy;
+      k.br = desc[i].r;
+      k.vert0 = (int64_t)verts.size() / 2;
+      k.nverts = (int32_t)desc[i].pts.size();
+      for (const auto& v : desc[i].pts) {
+        verts.push_back(v[0]);
+        verts.push_back(v[1]);
+      }
+    }
+    VL_TRYB(dalloc(b, b->dflags, (size_t)flag_bytes));
+    const SetupAlloc alloc = [b](void** p, size_t bytes) {
+      VL_HIP(hipMalloc(p, bytes));
+      b->allocs.push_back(*p);
+      b->device_bytes += (int64_t)bytes;
+      return VL_OK;
+    };
+    VL_TRYB(device_setup(sc, verts, b->dflags, flag_bytes, b->stream, alloc, dcount, dtab));
+    std::vector<uint8_t> fl((size_t)flag_bytes);
+    VL_HIP(hipMemcpy(fl.data(), b->dflags, (size_t)flag_bytes, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nc; ++i) {
+      Tables& tab = b->tabs[i];
+      const BatchCase& c = b->hcases[i];
+      tab.fluid_nodes = dcount[i].fluid;
+      tab.flag.resize((size_t)(tab.nx * tab.ny));
+      for (int64_t y = 0; y < tab.ny; ++y)
+        std::memcpy(&tab.flag[(size_t)(y * tab.nx)], &fl[(size_t)(c.flag_off + y * c.g.pitch)], (size_t)tab.nx);
+      npb[i] = (int32_t)dcount[i].npb;
+    }
+  }
+  for (size_t i = 0; i < nc; ++i) b->total_fluid += b->tabs[i].fluid_nodes;
+  lap(0);
   // ---- K-step passes: the plan of every case (its strips' tasks and listed nodes) ----
   // Defaults: FP32 5 steps per pass (the 512-case sweep on MI355X: 93 GLUPS against 87 at 4 and
   // 67 in single steps), FP64 3 (the 4-step FP64 pass of both collides needs all 256 registers),
@@ -817,15 +899,12 @@ vl_status build(vl_batch* b) {
     for (size_t i = 0; i < nc; ++i)
       if (st[i] != VL_OK) return set_error(st[i], "case " + std::to_string(i) + ": " + err[i]);
   }
+  lap(1);
   b->nbuf = b->pass_k > 1 ? b->pass_k + 1 : 2;
-  VL_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  VL_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
-  VL_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
-  VL_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
   for (int j = 0; j < b->nbuf; ++j) VL_HIP(hipMalloc(&b->pool[j], (size_t)pop_elems * b->elem));
   b->device_bytes += b->nbuf * pop_elems * (int64_t)b->elem;
-  // flags (pitched; padding columns are "not fluid")
-  {
+  // flags (pitched; padding columns are "not fluid"), unless the set-up kernels wrote them
+  if (!b->setup_on_device) {
     std::vector<uint8_t> fl((size_t)flag_bytes, 2);
     for (size_t i = 0; i < nc; ++i) {
       const Tables& tab = b->tabs[i];
@@ -836,7 +915,71 @@ vl_status build(vl_batch* b) {
     VL_TRYB(dupload(b, b->dflags, fl));
   }
   // packed boundary tables
-  {
+  if (b->setup_on_device) {
+    std::vector<int32_t> gfirst;
+    std::vector<int64_t> o_g(nc), o_fc(nc), o_pv(nc);
+    int64_t fc_len = 0, pv_len = 0;
+    int64_t ob = 0, obf = 0, ol = 0, op = 0;
+    std::vector<int64_t> o_b(nc), o_bf(nc), o_l(nc), o_p(nc);
+    for (size_t i = 0; i < nc; ++i) {
+      const SetupCounts& k = dcount[i];
+      o_b[i] = ob;
+      o_bf[i] = obf;
+      o_l[i] = ol;
+      o_p[i] = op;
+      ob += k.nb;
+      obf += k.nb + 1;
+      ol += k.nl;
+      op += k.npb;
+      o_g[i] = (int64_t)gfirst.size();
+      gfirst.push_back(0);  // the body's force group, when it has force links
+      if (k.nslots > 0) gfirst.push_back((int32_t)k.nslots);
+      o_fc[i] = fc_len;
+      fc_len += 4 * std::max<int64_t>(k.nslots, 1);
+      o_pv[i] = pv_len;
+      pv_len += 2 * std::max<int64_t>(k.npb, 1);
+    }
+    int32_t* d_gf;
+    double *d_fc, *d_pv, *d_drho;
+    VL_TRYB(dupload(b, d_gf, gfirst));
+    std::vector<double> zeros_fc((size_t)fc_len, 0.0), zeros_pv((size_t)pv_len, 0.0), zeros_d(nc, 0.0);
+    VL_TRYB(dupload(b, d_fc, zeros_fc));
+    VL_TRYB(dupload(b, d_pv, zeros_pv));
+    VL_TRYB(dupload(b, d_drho, zeros_d));
+    for (size_t i = 0; i < nc; ++i) {
+      const SetupCounts& k = dcount[i];
+      DevTables& t = b->hcases[i].t;
+      t.nb = (int32_t)k.nb;
+      t.nl = (int32_t)k.nl;
+      t.np = (int32_t)k.npb;
+      t.npb = (int32_t)k.npb;
+      t.nslots = (int32_t)k.nslots;
+      t.ngroups = k.nslots > 0 ? 1 : 0;
+      t.bx = dtab.bx + o_b[i];
+      t.by = dtab.by + o_b[i];
+      t.bfirst = dtab.bfirst + o_bf[i];
+      t.dir = dtab.dir + o_l[i];
+      t.kind = dtab.kind + o_l[i];
+      t.vgroup = dtab.vgroup + o_l[i];
+      t.lflags = dtab.lflags + o_l[i];
+      t.lnx = dtab.lnx + o_l[i];
+      t.lny = dtab.lny + o_l[i];
+      t.fslot = dtab.fslot + o_l[i];
+      t.q = dtab.q + o_l[i];
+      t.uwx = dtab.uwx + o_l[i];
+      t.uwy = dtab.uwy + o_l[i];
+      t.cu6 = dtab.cu6 + o_l[i];
+      t.rho = dtab.rho + o_l[i];
+      t.px = dtab.px + o_p[i];
+      t.py = dtab.py + o_p[i];
+      t.pnx = dtab.pnx + o_p[i];
+      t.pny = dtab.pny + o_p[i];
+      t.gfirst = d_gf + o_g[i];
+      t.fcontrib = d_fc + o_fc[i];
+      t.pvel = d_pv + o_pv[i];
+      t.drho = d_drho + i;
+    }
+  } else {
     std::vector<int32_t> bx, by, bfirst, fslot, px, py, gfirst;
     std::vector<uint8_t> dir, kind, vgroup, lflags;
     std::vector<int8_t> lnx, lny, pnx, pny;
@@ -1019,6 +1162,7 @@ vl_status build(vl_batch* b) {
     VL_HIP(hipGetLastError());
   }
   VL_HIP(hipStreamSynchronize(b->stream));
+  lap(2);
   // ---- autotune the bulk tile shape and the grid policy on the real batch ----
   if (const char* e = std::getenv("VL_BATCH_SERIAL")) b->serial = e[0] == '1';
   hipDeviceProp_t prop;
@@ -1157,7 +1301,7 @@ vl_status build(vl_batch* b) {
   }
   b->use_graphs = true;
   if (const char* e = std::getenv("VL_GRAPHS")) b->use_graphs = e[0] != '0';
-
+  lap(3);
   b->built = true;
   return VL_OK;
 }
@@ -1357,6 +1501,8 @@ vl_status vl_batch_info_get(vl_batch* b, vl_batch_info* o) {
   o->listed_tiles = b->nlisted_tiles;
   o->pass_nodes = b->pass_nodes;
   o->listed_nodes = b->listed_nodes;
+  o->setup_on_device = b->setup_on_device ? 1 : 0;
+  for (int k = 0; k < 4; ++k) o->setup_ms[k] = b->setup_ms[k];
   return VL_OK;
 }
 
@@ -1412,6 +1558,51 @@ vl_status vl_batch_read_case(vl_batch* b, int id, double* f) {
           v = wgt(i) + (double)((float)((const _Float16*)host.data())[e] * F16_INV);
         f[i * n + y * nx + x] = v;
       }
+  return VL_OK;
+}
+
+vl_status vl_batch_table_digests(vl_batch* b, int id, uint64_t* out, int n) {
+  if (!b || !b->built || id < 0 || (size_t)id >= b->hcases.size() || !out || n < 0)
+    return set_error(VL_EINVAL, "bad case id or output");
+  VL_HIP(hipStreamSynchronize(b->stream));
+  const BatchCase& c = b->hcases[id];
+  const DevTables& t = c.t;
+  std::vector<uint64_t> d;
+  auto digest = [&](const void* dev, size_t bytes) -> vl_status {
+    std::vector<unsigned char> h(bytes);
+    if (bytes) VL_HIP(hipMemcpy(h.data(), dev, bytes, hipMemcpyDeviceToHost));
+    uint64_t x = 0xcbf29ce484222325ull;  // FNV-1a
+    for (unsigned char v : h) x = (x ^ v) * 0x100000001b3ull;
+    d.push_back(x);
+    return VL_OK;
+  };
+  const size_t nb = (size_t)t.nb, nl = (size_t)t.nl, np = (size_t)t.np;
+  VL_TRYB(digest(b->dflags + c.flag_off, (size_t)c.g.pitch * (size_t)c.g.ny));
+  VL_TRYB(digest(t.bx, nb * 4));
+  VL_TRYB(digest(t.by, nb * 4));
+  VL_TRYB(digest(t.bfirst, (nb + 1) * 4));
+  VL_TRYB(digest(t.dir, nl));
+  VL_TRYB(digest(t.kind, nl));
+  VL_TRYB(digest(t.vgroup, nl));
+  VL_TRYB(digest(t.lflags, nl));
+  VL_TRYB(digest(t.lnx, nl));
+  VL_TRYB(digest(t.lny, nl));
+  VL_TRYB(digest(t.fslot, nl * 4));
+  VL_TRYB(digest(t.q, nl * 8));
+  VL_TRYB(digest(t.uwx, nl * 8));
+  VL_TRYB(digest(t.uwy, nl * 8));
+  VL_TRYB(digest(t.cu6, nl * 8));
+  VL_TRYB(digest(t.rho, nl * 8));
+  VL_TRYB(digest(t.px, np * 4));
+  VL_TRYB(digest(t.py, np * 4));
+  VL_TRYB(digest(t.pnx, np));
+  VL_TRYB(digest(t.pny, np));
+  VL_TRYB(digest(t.gfirst, ((size_t)t.ngroups + 1) * 4));
+  const int64_t cnt[7] = {t.nb, t.nl, t.np, t.npb, t.nslots, t.ngroups, b->tabs[id].fluid_nodes};
+  uint64_t x = 0xcbf29ce484222325ull;
+  for (size_t k = 0; k < sizeof(cnt); ++k) x = (x ^ ((const unsigned char*)cnt)[k]) * 0x100000001b3ull;
+  d.push_back(x);
+  for (int k = 0; k < std::min<int>(n, (int)d.size()); ++k) out[k] = d[(size_t)k];
   return VL_OK;
 }
 

@@ -28,6 +28,14 @@ struct Disc final : Shape {  // geometry.rs:16-27
     double dx = x - cx, dy = y - cy;
     return dx * dx + dy * dy < r * r;
   }
+  ShapeDesc describe() const override {
+    ShapeDesc d;
+    d.kind = ShapeDesc::CIRCLE;
+    d.cx = cx;
+    d.cy = cy;
+    d.r = r;
+    return d;
+  }
 };
 struct EvenOdd final : Shape {  // geometry.rs:30-49
   std::vector<std::array<double, 2>> p;
@@ -39,6 +47,12 @@ struct EvenOdd final : Shape {  // geometry.rs:30-49
       if ((yi > y) != (yj > y) && x < (xj - xi) * (y - yi) / (yj - yi) + xi) in = !in;
     }
     return in;
+  }
+  ShapeDesc describe() const override {
+    ShapeDesc d;
+    d.kind = ShapeDesc::POLYGON;
+    d.pts = p;
+    return d;
   }
 };
 struct RoundSq final : Shape {  // geometry.rs:52-66

@@ -18,9 +18,17 @@ namespace vl {
 vl_status set_error(vl_status st, const std::string& msg);
 
 // ---- shapes ------------------------------------------------------------------------------
+// A shape the device set-up of a batch can evaluate itself: a polygon (even-odd rule over its
+// vertices) or a circle; other shapes are evaluated on the host.
+struct ShapeDesc {
+  enum Kind { NONE = 0, POLYGON = 1, CIRCLE = 2 } kind = NONE;
+  std::vector<std::array<double, 2>> pts;
+  double cx = 0.0, cy = 0.0, r = 0.0;
+};
 struct Shape {
   virtual ~Shape() = default;
   virtual bool inside(double x, double y) const = 0;
+  virtual ShapeDesc describe() const { return {}; }
 };
 std::unique_ptr<Shape> make_polygon(std::vector<std::array<double, 2>> pts);
 std::vector<std::array<double, 2>> section_outline(int corner, double b, double cx, double cy, double d);
