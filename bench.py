@@ -30,6 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# vector instruction issue: one wave64 VALU instruction per CU per clock, 256 CUs at 2.4 GHz (the
+# rate behind the published 78.6 TFLOP/s FP64 vector peak: 64 lanes x 2 flops per FMA)
+VALU_PEAK_G = 256 * 2.4
 METRIC = "D2Q9 GLUPS & %HBM peak, FP32 vs FP64; batched solves/s at |dCd|<0.5%"
 BYTES_PER_NODE = {0: 144, 1: 72, 2: 36}  # 9 reads + 9 writes per fluid-node update
 PREC_NAME = {0: "f64", 1: "f32", 2: "f16"}
@@ -281,7 +284,21 @@ def batch_bench(vl, api, ncases, seed, sample):
     lu = int(info.lattice_updates)  # updates performed (a retired case counts the steps it ran)
     results = [b.result(k) for k in range(ncases)]
     fluid = [b.case_fluid_nodes(k) for k in range(ncases)]
+    setup = {"on_device": bool(info.setup_on_device),
+             "ms": dict(zip(("cases_and_tables", "pass_plans", "packing", "autotune"),
+                            [round(v, 1) for v in info.setup_ms]))}
     b.close()
+    # the same cases' set-up on host threads (the host builder), build only
+    os.environ["VL_BATCH_SETUP"] = "host"
+    try:
+        bh, t_build_host = workloads.build_batch(vl.backend(), cases, api.F32)
+        setup["host_builder_seconds"] = round(t_build_host, 3)
+        setup["host_builder_ms"] = dict(zip(setup["ms"], [round(v, 1) for v in bh.info().setup_ms]))
+        bh.close()
+    finally:
+        del os.environ["VL_BATCH_SETUP"]
+    log(f"[batch] set-up {t_build:.2f} s on the device {setup['ms']}; host builder {t_build_host:.2f} s "
+        f"{setup['host_builder_ms']}")
     finished = [k for k in range(ncases) if results[k][2] == 0]
     diverged = ncases - len(finished)
     log(f"[batch] {ncases} cases, {info.fluid_nodes / 1e6:.1f} M fluid nodes, {lu / 1e12:.3f} T updates performed; "
@@ -353,7 +370,7 @@ def batch_bench(vl, api, ncases, seed, sample):
             "batched_solves_per_s_at_gate": round(len(within) / t_run, 3),
             "batched_glups": round(lu / t_run / 1e9, 2), "batched_lattice_updates": lu,
             "batched_cases": ncases, "batched_finished": len(finished), "batched_diverged": diverged,
-            "batched_seconds": round(t_run, 2), "batched_setup_seconds": round(t_build, 2),
+            "batched_seconds": round(t_run, 2), "batched_setup_seconds": round(t_build, 3), "batched_setup": setup,
             "batched_tile": f"{info.tile_w}x{info.tile_h}",
             "batched_pass_steps": int(info.pass_steps), "batched_pass_nodes": int(info.pass_nodes),
             "batched_listed_nodes": int(info.listed_nodes),
@@ -367,6 +384,21 @@ def batch_bench(vl, api, ncases, seed, sample):
             "batched_vs_sequential": round(seq_total / t_run, 2),
             "sequential_one_step_per_call_step_us": round(coeff1[0] * 1e6, 2),
             "batched_vs_sequential_one_step_per_call": round(seq1_total / t_run, 2)}
+
+
+def surrogate_setup(vl, api, n=9216, seed=5):
+    """Set-up time (build only) of the surrogate's solver batch: n towing-tank cases of 24 cells
+    per diameter (8192 training + 1024 held-out shapes), FP32."""
+    from vortexlab_amd import surrogate, workloads
+    cases, _, _ = surrogate.sample_cases(seed, n)
+    b, secs = workloads.build_batch(vl.backend(), cases, api.F32)
+    info = b.info()
+    out = {"cases": n, "seconds": round(secs, 3), "on_device": bool(info.setup_on_device),
+           "ms": dict(zip(("cases_and_tables", "pass_plans", "packing", "autotune"), [round(v, 1) for v in info.setup_ms])),
+           "fluid_nodes": int(info.fluid_nodes)}
+    b.close()
+    log(f"[batch] set-up of {n} surrogate cases: {secs:.2f} s {out['ms']}")
+    return out
 
 
 def main():
@@ -428,15 +460,24 @@ def main():
     traffic = None
     tname = kname[:kname.index("<")] + (f"{main_run['pass_steps']}" if main_run["pass_steps"] > 1 else "")
     tf = os.path.join(ROOT, "profiles", f"{tname}_traffic_{wl}_{PREC_NAME[prec]}.json")
+    valu = None
     if os.path.exists(tf) and args.scale == 1:
         with open(tf) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            prof = json.load(fh)
+        traffic = prof.get("hbm_bytes_per_launch")
+        if prof.get("valu_insts_per_launch"):
+            # vector instructions per launch (SQ_INSTS_VALU of the committed counter run) over this
+            # run's launch time, against one wave64 VALU instruction per CU per clock at 2.4 GHz
+            rate = prof["valu_insts_per_launch"] / (main_run["bulk_ms"] * 1e-3) / 1e9
+            valu = {"bound": "valu", "achieved": round(rate, 1), "peak": VALU_PEAK_G, "unit": "G wave64 instructions/s",
+                    "frac": round(rate / VALU_PEAK_G, 4), "insts_per_launch": prof["valu_insts_per_launch"]}
     import ctypes
     copy = ctypes.c_double()
     vl.backend().check(vl.backend().device_copy_bandwidth(1 << 30, 20, ctypes.byref(copy)))
     log(f"[bench] plain device copy (1 GiB): {copy.value:.0f} GB/s")
     if batch_cases > 0:
         extra.update(batch_bench(vl, api, batch_cases, args.batch_seed, args.batch_sample))
+        extra["batched_setup_9216_surrogate_cases"] = surrogate_setup(vl, api)
     # the CPU run covers the whole wake job when it fits its time budget: then its final state
     # hash is compared with the GPU run's
     gpu_hash = main_run["state_hash"] if prec == api.F64 and warmup == 0 else None
@@ -456,7 +497,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": kname, "kernel_us": round(main_run["bulk_ms"] * 1e3, 2),
                      "algorithmic_bytes_per_launch": traffic_alg, "steps_per_launch": max(main_run["pass_steps"], 1),
-                     "copy_gbs": round(copy.value, 1), "frac_of_copy": round(achieved / copy.value, 4)},
+                     "copy_gbs": round(copy.value, 1), "frac_of_copy": round(achieved / copy.value, 4),
+                     "valu": valu},
         # the metric's "%HBM peak in GLUPS": GLUPS x the single-step bytes per update over the
         # peak; above 1 when a pass carries more than one step per HBM round trip
         "glups_frac_hbm": round(main_run["glups"] * BYTES_PER_NODE[prec] / HBM_PEAK_GBS, 4),

@@ -280,7 +280,8 @@ typedef struct {
   int64_t device_bytes;
   double autotune_ms[3][4]; /* time of a trial step 0 (boundary kernel beside the bulk) per
                                (tile shape 64x4 / 128x2 / 256x1, bulk grid: one workgroup per
-                               tile / 4 / 8 / 16 persistent per CU) */
+                               tile / 4 / 8 / 16 persistent per CU); only the first column is
+                               timed unless VL_BATCH_AUTOTUNE=full, the others stay 0 */
   /* K-step passes (pass_steps > 1): a case that runs a whole group of K steps advances its
    * covered interior (pass_nodes over all cases, in pass_tasks strips of up to pass_rows rows)
    * K steps per pass and its other interior nodes (listed_nodes) one step at a time. */

@@ -245,8 +245,8 @@ __global__ __launch_bounds__(BT) void k_batch_listed(const typename S::T* __rest
 template <class S>
 __global__ __launch_bounds__(BT) void k_batch_init(typename S::T* __restrict__ A, typename S::T* __restrict__ B,
                                                    const uint8_t* __restrict__ flags,
-                                                   const BatchCase* __restrict__ cases, int ci) {
-  const BatchCase& cs = cases[ci];
+                                                   const BatchCase* __restrict__ cases, int case0) {
+  const BatchCase& cs = cases[case0 + (int)blockIdx.y];  // one row of workgroups per case
   const int64_t n = (int64_t)cs.g.pitch * cs.g.ny;
   for (int64_t e = (int64_t)blockIdx.x * BT + threadIdx.x; e < n; e += (int64_t)gridDim.x * BT) {
     const int y = (int)(e / cs.g.pitch), x = (int)(e - (int64_t)y * cs.g.pitch);
@@ -272,6 +272,22 @@ __global__ __launch_bounds__(BT) void k_batch_init(typename S::T* __restrict__ A
       S::st(A + cs.pop_off + q * cs.g.plane + e, v);
       S::st(B + cs.pop_off + q * cs.g.plane + e, v);
     }
+  }
+}
+
+// Outward velocities of every case's outlet-controller nodes from its populations in pool A (start
+// of a run): k_outlet_prime of the single domain, one row of workgroups per case.
+template <class S>
+__global__ __launch_bounds__(BT) void k_batch_outlet_prime(const typename S::T* __restrict__ A,
+                                                           const BatchCase* __restrict__ cases, int case0) {
+  const BatchCase& cs = cases[case0 + (int)blockIdx.y];
+  const DevTables& t = cs.t;
+  for (int k = (int)(blockIdx.x * BT + threadIdx.x); k < t.np; k += (int)(gridDim.x * BT)) {
+    const int64_t idx = (int64_t)t.py[k] * cs.g.pitch + t.px[k];
+    typename S::C f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = S::ld(A + cs.pop_off + i * cs.g.plane + idx);
+    t.pvel[k] = normal_velocity_of<S>(f, cs.c, t.pnx[k], t.pny[k]);
   }
 }
 
@@ -1120,44 +1136,41 @@ vl_status build(vl_batch* b) {
   VL_HIP(hipMemset(b->d_hist, 0, (size_t)std::max<int64_t>(hist_len, 2) * sizeof(double)));
   if (kick_tab.empty()) kick_tab.push_back(0.0);
   VL_TRYB(dupload(b, b->d_kick, kick_tab));
-  // ---- initial state, in every pool (two at a time) ----
-  for (int j = 0; j < b->nbuf; j += 2) {
-    void* pa = b->pool[j];
-    void* pb = b->pool[std::min(j + 1, b->nbuf - 1)];
-    for (size_t i = 0; i < nc; ++i) {
+  // ---- initial state, in every pool (two at a time), all cases per launch (rows of 64
+  // workgroups, up to 65535 cases a launch) ----
+  for (size_t c0 = 0; c0 < nc; c0 += 65535) {
+    const unsigned rows = (unsigned)std::min<size_t>(65535, nc - c0);
+    for (int j = 0; j < b->nbuf; j += 2) {
+      void* pa = b->pool[j];
+      void* pb = b->pool[std::min(j + 1, b->nbuf - 1)];
       switch (b->precision) {
         case VL_F64:
-          hipLaunchKernelGGL(k_batch_init<StoreF64>, dim3(1024), dim3(BT), 0, b->stream, (double*)pa, (double*)pb,
-                             b->dflags, b->dcases, (int)i);
+          hipLaunchKernelGGL(k_batch_init<StoreF64>, dim3(64, rows), dim3(BT), 0, b->stream, (double*)pa, (double*)pb,
+                             b->dflags, b->dcases, (int)c0);
           break;
         case VL_F16:
-          hipLaunchKernelGGL(k_batch_init<StoreF16>, dim3(1024), dim3(BT), 0, b->stream, (_Float16*)pa,
-                             (_Float16*)pb, b->dflags, b->dcases, (int)i);
+          hipLaunchKernelGGL(k_batch_init<StoreF16>, dim3(64, rows), dim3(BT), 0, b->stream, (_Float16*)pa,
+                             (_Float16*)pb, b->dflags, b->dcases, (int)c0);
           break;
         default:
-          hipLaunchKernelGGL(k_batch_init<StoreF32>, dim3(1024), dim3(BT), 0, b->stream, (float*)pa, (float*)pb,
-                             b->dflags, b->dcases, (int)i);
+          hipLaunchKernelGGL(k_batch_init<StoreF32>, dim3(64, rows), dim3(BT), 0, b->stream, (float*)pa, (float*)pb,
+                             b->dflags, b->dcases, (int)c0);
       }
       VL_HIP(hipGetLastError());
     }
-  }
-  // pressure-node velocities of the initial state: the boundary update writes them from then on
-  for (size_t i = 0; i < nc; ++i) {
-    const BatchCase& c = b->hcases[i];
-    if (c.t.np == 0) continue;
-    const unsigned grid = (unsigned)((c.t.np + 255) / 256);
+    // pressure-node velocities of the initial state: the boundary update writes them from then on
     switch (b->precision) {
       case VL_F64:
-        hipLaunchKernelGGL(k_outlet_prime<StoreF64>, dim3(grid), dim3(256), 0, b->stream,
-                           (const double*)b->pool[0] + c.pop_off, c.g, c.c, c.t, c.t.pvel);
+        hipLaunchKernelGGL(k_batch_outlet_prime<StoreF64>, dim3(4, rows), dim3(BT), 0, b->stream,
+                           (const double*)b->pool[0], b->dcases, (int)c0);
         break;
       case VL_F16:
-        hipLaunchKernelGGL(k_outlet_prime<StoreF16>, dim3(grid), dim3(256), 0, b->stream,
-                           (const _Float16*)b->pool[0] + c.pop_off, c.g, c.c, c.t, c.t.pvel);
+        hipLaunchKernelGGL(k_batch_outlet_prime<StoreF16>, dim3(4, rows), dim3(BT), 0, b->stream,
+                           (const _Float16*)b->pool[0], b->dcases, (int)c0);
         break;
       default:
-        hipLaunchKernelGGL(k_outlet_prime<StoreF32>, dim3(grid), dim3(256), 0, b->stream,
-                           (const float*)b->pool[0] + c.pop_off, c.g, c.c, c.t, c.t.pvel);
+        hipLaunchKernelGGL(k_batch_outlet_prime<StoreF32>, dim3(4, rows), dim3(BT), 0, b->stream,
+                           (const float*)b->pool[0], b->dcases, (int)c0);
     }
     VL_HIP(hipGetLastError());
   }
@@ -1178,6 +1191,13 @@ vl_status build(vl_batch* b) {
   VL_HIP(hipEventCreate(&e0));
   VL_HIP(hipEventCreate(&e1));
   int best_s = 0, best_m = 1;
+  // The persistent grids lost to one workgroup per tile on every batch measured (the 512-case
+  // sweep: 1.78-1.82 ms against 2.0-2.6 ms a trial step), so by default only the three tile
+  // shapes are timed, one workgroup per tile, over two steps each; VL_BATCH_AUTOTUNE=full times all
+  // twelve policies over four.
+  bool full = false;
+  if (const char* e = std::getenv("VL_BATCH_AUTOTUNE")) full = std::string(e) == "full";
+  const int nmult = full ? 4 : 1, reps = full ? 4 : 2;
   for (int si = 0; si < 3; ++si) {
     bulk_tiles(b, order, 1 << shapes[si][0], shapes[si][1], kc, ky, kx, kend);
     int32_t *tkc, *tky, *tkx, *tact;
@@ -1186,18 +1206,18 @@ vl_status build(vl_batch* b) {
     VL_TRYB(dupload(b, tky, ky));
     VL_TRYB(dupload(b, tkx, kx));
     VL_TRYB(dupload(b, tact, act1));
-    for (int mi = 0; mi < 4; ++mi) {
+    for (int mi = 0; mi < nmult; ++mi) {
       const int grid = mults[mi] == 0 ? std::max(1, (int)kc.size())
                                       : std::max(1, std::min((int)kc.size(), cus * mults[mi]));
       // whole trial steps 0 (boundary beside bulk), A -> B from the same A every time
       VL_TRYB(trial_step(b, grid, shapes[si][0], tkc, tky, tkx, tact));
       VL_HIP(hipEventRecord(e0, b->stream));
-      for (int r = 0; r < 4; ++r) VL_TRYB(trial_step(b, grid, shapes[si][0], tkc, tky, tkx, tact));
+      for (int r = 0; r < reps; ++r) VL_TRYB(trial_step(b, grid, shapes[si][0], tkc, tky, tkx, tact));
       VL_HIP(hipEventRecord(e1, b->stream));
       VL_HIP(hipEventSynchronize(e1));
       float ms = 0.0f;
       VL_HIP(hipEventElapsedTime(&ms, e0, e1));
-      b->autotune_ms[si][mi] = ms / 4.0;
+      b->autotune_ms[si][mi] = ms / reps;
       if (ms < best) {
         best = ms;
         best_s = si;

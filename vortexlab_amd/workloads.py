@@ -65,16 +65,23 @@ def sweep_cases(seed: int, n: int = 512, w=(200, 1400), h=(80, 600), d=(10.0, 60
     return cases
 
 
-def run_batch(be, cases: list, precision: int = api.F32):
-    """All cases in one batch; returns (batch, set-up seconds, run seconds).  The caller closes
-    the batch."""
+def build_batch(be, cases: list, precision: int = api.F32):
+    """All cases in one batch, built (set-up, packing, autotuning) but not run; returns (batch,
+    set-up seconds).  The caller closes the batch."""
     import time
     t0 = time.perf_counter()
     b = api.Batch(be, precision)
     for c in cases:
         b.add(c.spec, c.steps, c.d)
     b.build()
-    t_build = time.perf_counter() - t0
+    return b, time.perf_counter() - t0
+
+
+def run_batch(be, cases: list, precision: int = api.F32):
+    """All cases in one batch; returns (batch, set-up seconds, run seconds).  The caller closes
+    the batch."""
+    import time
+    b, t_build = build_batch(be, cases, precision)
     t0 = time.perf_counter()
     b.run()
     return b, t_build, time.perf_counter() - t0
