@@ -74,33 +74,22 @@ __global__ void k_gather(const float* __restrict__ masks, const float* __restric
   }
 }
 
-// conv 3x3, pad 1, + bias, ReLU: out[n][co][y][x]
-__global__ void k_conv_fwd(const float* __restrict__ in, const float* __restrict__ w, const float* __restrict__ b,
-                           float* __restrict__ out, int n, int ci_n, int co_n, int side) {
-  const int64_t total = (int64_t)n * co_n * side * side;
-  for (int64_t e = (int64_t)blockIdx.x * T + threadIdx.x; e < total; e += (int64_t)gridDim.x * T) {
-    const int x = (int)(e % side);
-    const int y = (int)((e / side) % side);
-    const int co = (int)((e / ((int64_t)side * side)) % co_n);
-    const int64_t s = e / ((int64_t)side * side * co_n);
-    float acc = b[co];
-    for (int ci = 0; ci < ci_n; ++ci) {
-      const float* ip = in + ((s * ci_n + ci) * side) * side;
-      const float* wp = w + ((int64_t)co * ci_n + ci) * 9;
-#pragma unroll
-      for (int ky = 0; ky < 3; ++ky) {
-        const int yy = y + ky - 1;
-        if (yy < 0 || yy >= side) continue;
-#pragma unroll
-        for (int kx = 0; kx < 3; ++kx) {
-          const int xx = x + kx - 1;
-          if (xx < 0 || xx >= side) continue;
-          acc += wp[ky * 3 + kx] * ip[yy * side + xx];
-        }
-      }
-    }
-    out[e] = acc > 0.0f ? acc : 0.0f;
-  }
+// ---- convolutions as implicit GEMMs on the matrix cores (FP32 in, FP32 accumulate) ----------
+// One wave computes a 16 x 16 tile D[i][j] = sum_k A[i][k] B[k][j] with v_mfma_f32_16x16x4f32,
+// four k at a time: lane l supplies A[i0 + l % 16][k0 + l / 16] and B[k0 + l / 16][j0 + l % 16]
+// and receives D[i0 + 4 (l / 16) + r][j0 + l % 16] in accumulator register r.  A tile's k run in
+// one fixed order, so every result is reproducible bit for bit.  Pixel tiles are 16 consecutive
+// pixels of one sample; out-of-range taps read as 0.
+typedef float f4 __attribute__((ext_vector_type(4)));
+constexpr int MW = 4;  // waves per workgroup of the matrix-core kernels
+
+
+// Tap k = ci * 9 + ky * 3 + kx of a 3x3 kernel.
+__device__ inline void tap(int k, int& ci, int& ky, int& kx) {
+  ci = k / 9;
+  const int r = k - ci * 9;
+  ky = r / 3;
+  kx = r - ky * 3;
 }
 
 // 2x2 max-pool: out side = side/2
@@ -136,97 +125,210 @@ __global__ void k_pool_bwd(const float* __restrict__ act, const float* __restric
   }
 }
 
-// input gradient of a conv: din[n][ci][y][x] = sum_co sum_k w[co][ci][k] dz[n][co][y-ky+1][x-kx+1]
-__global__ void k_conv_bwd_data(const float* __restrict__ dz, const float* __restrict__ w, float* __restrict__ din, int n,
-                                int ci_n, int co_n, int side) {
-  const int64_t total = (int64_t)n * ci_n * side * side;
-  for (int64_t e = (int64_t)blockIdx.x * T + threadIdx.x; e < total; e += (int64_t)gridDim.x * T) {
-    const int x = (int)(e % side);
-    const int y = (int)((e / side) % side);
-    const int ci = (int)((e / ((int64_t)side * side)) % ci_n);
-    const int64_t s = e / ((int64_t)side * side * ci_n);
-    float acc = 0.0f;
-    for (int co = 0; co < co_n; ++co) {
-      const float* dp = dz + ((s * co_n + co) * side) * side;
-      const float* wp = w + ((int64_t)co * ci_n + ci) * 9;
+// The forward conv and the conv's input gradient as one implicit GEMM, D = A B:
+// * forward (FWD): out[n][co][p] = relu(b[co] + sum_(ci,ky,kx) w[co][ci][ky][kx] in[n][ci][p + (ky-1, kx-1)]),
+//   A = w (co x ci*9), B = the input's taps;
+// * input gradient: din[n][ci][p] = sum_(co,ky,kx) w[co][ci][ky][kx] dz[n][co][p - (ky-1, kx-1)],
+//   A[ci][(co,ky,kx)] = w, B = dz's taps.
+// A wave covers 16 rows of D (output channels) x PT pixel tiles: one weight load and tap decode
+// per k serve PT products, and the k loop issues the loads of U steps before their products.
+// With `splits` > 1 the k range is cut into fixed pieces of `kper` and each piece's sums go to
+// part[piece][...]; k_conv_split_sum then adds the pieces in order.  The pieces depend only on
+// the layer, never on the batch, so a sample's result does not depend on its batch.
+constexpr int U = 4;
+template <int PT, bool FWD>
+__global__ __launch_bounds__(MW * 64) void k_conv_mm(const float* __restrict__ src, const float* __restrict__ w,
+                                                     const float* __restrict__ b, float* __restrict__ dst,
+                                                     float* __restrict__ part, int n, int ci_n, int co_n, int side,
+                                                     int splits, int kper) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int S2 = side * side;
+  const int M = FWD ? co_n : ci_n, KC = FWD ? ci_n : co_n, K = KC * 9;
+  const int mtiles = M / 16, ptiles = n * S2 / (16 * PT);
+  const int t = (int)blockIdx.x * MW + (int)(threadIdx.x >> 6);
+  if (t >= ptiles * mtiles * splits) return;
+  const int m0 = (t % mtiles) * 16, pt = (t / mtiles) % ptiles, piece = t / (mtiles * ptiles);
+  const int smp = pt * 16 * PT / S2, p0 = pt * 16 * PT - smp * S2;
+  const int i = lane & 15, kq = lane >> 4;
+  int y[PT], x[PT];
 #pragma unroll
-      for (int ky = 0; ky < 3; ++ky) {
-        const int yy = y - ky + 1;
-        if (yy < 0 || yy >= side) continue;
+  for (int q = 0; q < PT; ++q) {
+    y[q] = (p0 + 16 * q + i) / side;
+    x[q] = (p0 + 16 * q + i) - y[q] * side;
+  }
+  const float* sbase = src + (int64_t)smp * KC * S2;
+  const int kbeg = piece * kper, kend = min(K, kbeg + kper);
+  f4 acc[PT];
 #pragma unroll
-        for (int kx = 0; kx < 3; ++kx) {
-          const int xx = x - kx + 1;
-          if (xx < 0 || xx >= side) continue;
-          acc += wp[ky * 3 + kx] * dp[yy * side + xx];
+  for (int q = 0; q < PT; ++q) acc[q] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = kbeg; k0 < kend; k0 += 4 * U) {
+    float a[U], v[U][PT];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + 4 * u + kq;
+      a[u] = 0.0f;
+#pragma unroll
+      for (int q = 0; q < PT; ++q) v[u][q] = 0.0f;
+      if (k < kend) {
+        int c, ky, kx;
+        tap(k, c, ky, kx);
+        a[u] = FWD ? w[(int64_t)(m0 + i) * K + k] : w[((int64_t)c * ci_n + m0 + i) * 9 + ky * 3 + kx];
+        const int dy = FWD ? ky - 1 : 1 - ky, dx = FWD ? kx - 1 : 1 - kx;
+        const float* cp = sbase + (int64_t)c * S2;
+#pragma unroll
+        for (int q = 0; q < PT; ++q) {
+          const int yy = y[q] + dy, xx = x[q] + dx;
+          if (yy >= 0 && yy < side && xx >= 0 && xx < side) v[u][q] = cp[yy * side + xx];
         }
       }
     }
-    din[e] = acc;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < PT; ++q) acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], v[u][q], acc[q], 0, 0, 0);
+  }
+  const int64_t total = (int64_t)n * M * S2;
+  float* out = splits > 1 ? part + piece * total : dst;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int m = m0 + 4 * kq + r;
+    float* op = out + ((int64_t)smp * M + m) * S2 + p0 + i;
+    const float bc = (FWD && splits == 1) ? b[m] : 0.0f;
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+      const float z = acc[q][r] + bc;
+      op[16 * q] = (FWD && splits == 1) ? (z > 0.0f ? z : 0.0f) : z;
+    }
   }
 }
 
-// weight and bias gradients of a conv: one workgroup per (co, ci); 9 (+1) sums over
-// (sample, y, x) each by a fixed strided partial per thread and a fixed LDS tree.
-__global__ __launch_bounds__(T) void k_conv_bwd_w(const float* __restrict__ dz, const float* __restrict__ in,
-                                                  float* __restrict__ dw, float* __restrict__ db, int n, int ci_n,
-                                                  int co_n, int side) {
-  __shared__ float red[T];
-  const int co = blockIdx.x / ci_n, ci = blockIdx.x - co * ci_n;
-  const int64_t per = (int64_t)side * side;
-  const int64_t total = (int64_t)n * per;
-  float acc[10];
+// dst[e] = sum of the pieces' sums in piece order (+ bias and ReLU for the forward conv).
+__global__ void k_conv_split_sum(const float* __restrict__ part, const float* __restrict__ b, float* __restrict__ dst,
+                                 int64_t total, int M, int S2, int splits, int fwd) {
+  for (int64_t e = (int64_t)blockIdx.x * T + threadIdx.x; e < total; e += (int64_t)gridDim.x * T) {
+    float z = 0.0f;
+    for (int sp = 0; sp < splits; ++sp) z += part[sp * total + e];
+    if (fwd) {
+      z += b[(int)((e / S2) % M)];
+      z = z > 0.0f ? z : 0.0f;
+    }
+    dst[e] = z;
+  }
+}
+
+// Weight and bias gradients of a conv, split over the batch-pixel sum in fixed chunks:
+// part[c][co][j] = sum over k = (sample, pixel) in chunk c of dz[k][co] * B[k][j], with
+// B[k][(ci,ky,kx)] the input's taps and B[k][ci_n * 9] = 1 (the bias).  A[co][k] = dz.  A wave
+// covers 16 output channels x JT column tiles (one dz load and pixel decode per k serve JT
+// products).  nj = ci_n * 9 + 1 columns, padded to a multiple of 16 JT (`njp`); co_n a multiple
+// of 16.
+template <int JT>
+__global__ __launch_bounds__(MW * 64) void k_conv_bwd_w(const float* __restrict__ dz, const float* __restrict__ in,
+                                                        float* __restrict__ part, int n, int ci_n, int co_n, int side,
+                                                        int chunk, int nchunks, int njp) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int S2 = side * side, ctiles = co_n / 16, jgroups = njp / (16 * JT), nj = ci_n * 9 + 1;
+  const int t = (int)blockIdx.x * MW + (int)(threadIdx.x >> 6);
+  if (t >= nchunks * ctiles * jgroups) return;
+  const int jg = t % jgroups, ct = (t / jgroups) % ctiles, c = t / (jgroups * ctiles);
+  const int co0 = ct * 16, j0 = jg * 16 * JT;
+  const int i = lane & 15, kq = lane >> 4;
+  // column g's tap offset into the sample's input planes (-1: the bias column, -2: padding)
+  int64_t off[JT];
+  int dy[JT], dx[JT];
 #pragma unroll
-  for (int k = 0; k < 10; ++k) acc[k] = 0.0f;
-  for (int64_t e = threadIdx.x; e < total; e += T) {
-    const int64_t s = e / per, p = e - s * per;
-    const int y = (int)(p / side), x = (int)(p - (int64_t)y * side);
-    const float g = dz[(s * co_n + co) * per + p];
-    const float* ip = in + (s * ci_n + ci) * per;
+  for (int g = 0; g < JT; ++g) {
+    const int j = j0 + 16 * g + i;
+    int ci = 0, ky = 1, kx = 1;
+    if (j < nj - 1) tap(j, ci, ky, kx);
+    off[g] = j < nj - 1 ? (int64_t)ci * S2 : (j == nj - 1 ? -1 : -2);
+    dy[g] = ky - 1;
+    dx[g] = kx - 1;
+  }
+  const int64_t kbeg = (int64_t)c * chunk, kend = std::min<int64_t>(kbeg + chunk, (int64_t)n * S2);
+  f4 acc[JT];
 #pragma unroll
-    for (int ky = 0; ky < 3; ++ky) {
-      const int yy = y + ky - 1;
+  for (int g = 0; g < JT; ++g) acc[g] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int64_t k0 = kbeg; k0 < kend; k0 += 4 * U) {
+    float a[U], v[U][JT];
 #pragma unroll
-      for (int kx = 0; kx < 3; ++kx) {
-        const int xx = x + kx - 1;
-        const float v = (yy >= 0 && yy < side && xx >= 0 && xx < side) ? ip[yy * side + xx] : 0.0f;
-        acc[ky * 3 + kx] += g * v;
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = k0 + 4 * u + kq;  // n * S2 and the chunk are multiples of 4
+      a[u] = 0.0f;
+#pragma unroll
+      for (int g = 0; g < JT; ++g) v[u][g] = 0.0f;
+      if (k < kend) {
+        const int smp = (int)(k / S2), p = (int)(k - (int64_t)smp * S2);
+        const int y = p / side, x = p - y * side;
+        a[u] = dz[((int64_t)smp * co_n + co0 + i) * S2 + p];
+        const float* ip = in + (int64_t)smp * ci_n * S2;
+#pragma unroll
+        for (int g = 0; g < JT; ++g) {
+          const int yy = y + dy[g], xx = x + dx[g];
+          if (off[g] >= 0) {
+            if (yy >= 0 && yy < side && xx >= 0 && xx < side) v[u][g] = ip[off[g] + yy * side + xx];
+          } else if (off[g] == -1) {
+            v[u][g] = 1.0f;
+          }
+        }
       }
     }
-    acc[9] += g;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int g = 0; g < JT; ++g) acc[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], v[u][g], acc[g], 0, 0, 0);
   }
-  const int nk = ci == 0 ? 10 : 9;
-  for (int k = 0; k < nk; ++k) {
-    red[threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int wdt = T / 2; wdt > 0; wdt >>= 1) {
-      if ((int)threadIdx.x < wdt) red[threadIdx.x] += red[threadIdx.x + wdt];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      if (k < 9)
-        dw[((int64_t)co * ci_n + ci) * 9 + k] = red[0];
-      else
-        db[co] = red[0];
-    }
-    __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    float* pp = part + ((int64_t)c * co_n + co0 + 4 * kq + r) * njp + j0 + i;
+#pragma unroll
+    for (int g = 0; g < JT; ++g) pp[16 * g] = acc[g][r];
+  }
+}
+
+// Sums the chunks' partial gradients into dw[co][ci][ky][kx] and db[co]: one wave per output,
+// lane l adds chunks l, l + 64, ... in order, then a fixed butterfly over the lanes.
+__global__ __launch_bounds__(T) void k_conv_bwd_w_sum(const float* __restrict__ part, float* __restrict__ dw,
+                                                      float* __restrict__ db, int ci_n, int co_n, int nchunks,
+                                                      int njp) {
+  const int nj = ci_n * 9 + 1;
+  const int lane = (int)(threadIdx.x & 63);
+  const int e = (int)(blockIdx.x * (T / 64) + (threadIdx.x >> 6));
+  if (e >= co_n * nj) return;
+  const int co = e / nj, j = e - co * nj;
+  float acc = 0.0f;
+  for (int c = lane; c < nchunks; c += 64) acc += part[((int64_t)c * co_n + co) * njp + j];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+  if (lane == 0) {
+    if (j < nj - 1)
+      dw[(int64_t)co * (nj - 1) + j] = acc;
+    else
+      db[co] = acc;
   }
 }
 
 // fully connected: out[s][m] = b[m] + sum_k w[m][k] x[s][k] (optional ReLU).  The FC1 input is the
 // flattened last pooled map followed by the scalar feature.
-__global__ void k_fc_fwd(const float* __restrict__ x, const float* __restrict__ xs, int kx, const float* __restrict__ w,
-                         const float* __restrict__ b, float* __restrict__ out, int n, int m_n, int relu) {
-  const int64_t total = (int64_t)n * m_n;
+__global__ __launch_bounds__(T) void k_fc_fwd(const float* __restrict__ x, const float* __restrict__ xs, int kx,
+                                              const float* __restrict__ w, const float* __restrict__ b,
+                                              float* __restrict__ out, int n, int m_n, int relu) {
+  // one wave per output: lane l sums k = l, l + 64, ... in order, then a fixed butterfly over the lanes
+  const int lane = (int)(threadIdx.x & 63);
+  const int64_t e = (int64_t)blockIdx.x * (T / 64) + (threadIdx.x >> 6);
+  if (e >= (int64_t)n * m_n) return;
   const int kin = kx + (xs ? 1 : 0);
-  for (int64_t e = (int64_t)blockIdx.x * T + threadIdx.x; e < total; e += (int64_t)gridDim.x * T) {
-    const int m = (int)(e % m_n);
-    const int64_t s = e / m_n;
-    const float* wp = w + (int64_t)m * kin;
-    const float* xp = x + s * kx;
-    float acc = b[m];
-    for (int k = 0; k < kx; ++k) acc += wp[k] * xp[k];
-    if (xs) acc += wp[kx] * xs[s];
-    out[e] = relu ? fmaxf(acc, 0.0f) : acc;
+  const int m = (int)(e % m_n);
+  const int64_t smp = e / m_n;
+  const float* wp = w + (int64_t)m * kin;
+  const float* xp = x + smp * kx;
+  float acc = 0.0f;
+  for (int k = lane; k < kin; k += 64) acc += wp[k] * (k < kx ? xp[k] : xs[smp]);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+  if (lane == 0) {
+    const float z = b[m] + acc;
+    out[e] = relu ? fmaxf(z, 0.0f) : z;
   }
 }
 
@@ -317,6 +419,7 @@ struct vl_nn {
   float *h = nullptr, *y = nullptr, *dy = nullptr, *dh = nullptr, *dflat = nullptr;
   float* dact = nullptr;         // scratch: gradient of the largest activation
   float* dpool = nullptr;        // scratch: gradient of a pooled map
+  float* wpart = nullptr;        // scratch: chunk partials of a conv's weight gradient
   float* loss = nullptr;
   int32_t* idx = nullptr;
   // dataset on the device
@@ -328,13 +431,81 @@ struct vl_nn {
 namespace {
 
 unsigned blocks_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + T - 1) / T, 65536)); }
+// Workgroups of a matrix-core kernel for `tiles` 16 x 16 tiles (one wave each, no grid stride).
+unsigned mma_blocks(int64_t tiles) { return (unsigned)std::max<int64_t>(1, (tiles + MW - 1) / MW); }
+// Per conv layer and direction: 16-pixel tiles per wave and k pieces, so that a batch of 64
+// samples gives each of the 1024 SIMDs a few waves.  The input gradient of layer 0 is unused.
+struct ConvPlan {
+  int pt, splits;
+};
+constexpr ConvPlan FWD_PLAN[NB] = {{4, 1}, {4, 1}, {1, 1}, {1, 4}};
+constexpr ConvPlan BWD_PLAN[NB] = {{1, 1}, {4, 1}, {1, 2}, {1, 4}};
+static_assert(SP[NB - 1] * SP[NB - 1] % 64 == 0, "every conv layer's side^2 must be a multiple of 16 * 4");
+
+// k per piece of a layer's k range (a multiple of 4 * U).
+int conv_kper(int K, int splits) {
+  const int step = 4 * U;
+  return ((K + splits - 1) / splits + step - 1) / step * step;
+}
+
+// Workgroups of T threads for one wave per item.
+unsigned wave_blocks(int64_t items) { return (unsigned)std::max<int64_t>(1, (items + T / 64 - 1) / (T / 64)); }
+// Batch-pixel terms per chunk of a conv's weight-gradient sum (a multiple of 4): 64 matrix-core
+// steps per wave, and enough waves for the first layer (32 x 10 outputs).
+constexpr int WCHUNK = 256;
+
+// Weight-gradient column tiles per wave of layer l (the first layer has 10 columns, one tile),
+// and the padded column count.
+int conv_jt(int l) { return CH[l] * 9 + 1 > 16 ? 4 : 1; }
+int conv_njp(int l) {
+  const int w = 16 * conv_jt(l);
+  return (CH[l] * 9 + 1 + w - 1) / w * w;
+}
+
+// Floats of the partial sums (weight-gradient chunks, conv k pieces) for batches of up to `b`
+// samples.
+int64_t wpart_floats(int64_t b) {
+  int64_t m = 0;
+  for (int l = 0; l < NB; ++l) {
+    const int64_t njp = conv_njp(l);
+    const int64_t nchunks = (b * SP[l] * SP[l] + WCHUNK - 1) / WCHUNK;
+    m = std::max(m, nchunks * CH[l + 1] * njp);
+    m = std::max(m, FWD_PLAN[l].splits * b * CH[l + 1] * SP[l] * SP[l]);
+    m = std::max(m, BWD_PLAN[l].splits * b * CH[l] * SP[l] * SP[l]);
+  }
+  return m;
+}
+
+// One conv layer's forward pass (FWD) or input gradient through k_conv_mm (+ k_conv_split_sum).
+template <bool FWD>
+void launch_conv(vl_nn* q, int l, const float* src, float* dst, int n) {
+  const ConvPlan pl = FWD ? FWD_PLAN[l] : BWD_PLAN[l];
+  const int side = SP[l], S2 = side * side;
+  const int M = FWD ? CH[l + 1] : CH[l], K = (FWD ? CH[l] : CH[l + 1]) * 9;
+  const int kper = conv_kper(K, pl.splits);
+  const int splits = (K + kper - 1) / kper;
+  const int64_t waves = (int64_t)n * S2 / (16 * pl.pt) * (M / 16) * splits;
+  const float* w = q->p + q->L.cw[l];
+  const float* b = q->p + q->L.cb[l];
+  if (pl.pt == 4)
+    hipLaunchKernelGGL((k_conv_mm<4, FWD>), dim3(mma_blocks(waves)), dim3(MW * 64), 0, q->stream, src, w, b, dst,
+                       q->wpart, n, CH[l], CH[l + 1], side, splits, kper);
+  else
+    hipLaunchKernelGGL((k_conv_mm<1, FWD>), dim3(mma_blocks(waves)), dim3(MW * 64), 0, q->stream, src, w, b, dst,
+                       q->wpart, n, CH[l], CH[l + 1], side, splits, kper);
+  if (splits > 1) {
+    const int64_t total = (int64_t)n * M * S2;
+    hipLaunchKernelGGL(k_conv_split_sum, dim3(blocks_for(total)), dim3(T), 0, q->stream, q->wpart, b, dst, total, M,
+                       S2, splits, FWD ? 1 : 0);
+  }
+}
 
 void free_nn(vl_nn* q) {
   if (!q) return;
   if (q->stream) (void)hipStreamSynchronize(q->stream);
   for (void* ptr : {(void*)q->p, (void*)q->g, (void*)q->m, (void*)q->v, (void*)q->x0, (void*)q->s, (void*)q->tgt,
                     (void*)q->h, (void*)q->y, (void*)q->dy, (void*)q->dh, (void*)q->dflat, (void*)q->dact,
-                    (void*)q->dpool, (void*)q->loss, (void*)q->idx, (void*)q->d_masks, (void*)q->d_feat,
+                    (void*)q->dpool, (void*)q->wpart, (void*)q->loss, (void*)q->idx, (void*)q->d_masks, (void*)q->d_feat,
                     (void*)q->d_targ})
     if (ptr) (void)hipFree(ptr);
   for (int l = 0; l < NB; ++l) {
@@ -350,15 +521,14 @@ vl_status forward(vl_nn* q, int n) {
   const float* in = q->x0;
   for (int l = 0; l < NB; ++l) {
     const int side = SP[l];
-    hipLaunchKernelGGL(k_conv_fwd, dim3(blocks_for((int64_t)n * CH[l + 1] * side * side)), dim3(T), 0, q->stream, in,
-                       q->p + q->L.cw[l], q->p + q->L.cb[l], q->act[l], n, CH[l], CH[l + 1], side);
+    launch_conv<true>(q, l, in, q->act[l], n);
     hipLaunchKernelGGL(k_pool_fwd, dim3(blocks_for((int64_t)n * CH[l + 1] * (side / 2) * (side / 2))), dim3(T), 0,
                        q->stream, q->act[l], q->pool[l], n * CH[l + 1], side);
     in = q->pool[l];
   }
-  hipLaunchKernelGGL(k_fc_fwd, dim3(blocks_for((int64_t)n * HID)), dim3(T), 0, q->stream, q->pool[NB - 1], q->s, FLAT,
+  hipLaunchKernelGGL(k_fc_fwd, dim3(wave_blocks((int64_t)n * HID)), dim3(T), 0, q->stream, q->pool[NB - 1], q->s, FLAT,
                      q->p + q->L.w1, q->p + q->L.b1, q->h, n, HID, 1);
-  hipLaunchKernelGGL(k_fc_fwd, dim3(blocks_for((int64_t)n * OUT)), dim3(T), 0, q->stream, q->h, (const float*)nullptr,
+  hipLaunchKernelGGL(k_fc_fwd, dim3(wave_blocks((int64_t)n * OUT)), dim3(T), 0, q->stream, q->h, (const float*)nullptr,
                      HID, q->p + q->L.w2, q->p + q->L.b2, q->y, n, OUT, 0);
   VL_HIP(hipGetLastError());
   return VL_OK;
@@ -382,11 +552,19 @@ vl_status backward(vl_nn* q, int n) {
     hipLaunchKernelGGL(k_pool_bwd, dim3(blocks_for((int64_t)n * CH[l + 1] * side * side)), dim3(T), 0, q->stream,
                        q->act[l], dp, q->dact, n * CH[l + 1], side);
     const float* in = l == 0 ? q->x0 : q->pool[l - 1];
-    hipLaunchKernelGGL(k_conv_bwd_w, dim3((unsigned)(CH[l + 1] * CH[l])), dim3(T), 0, q->stream, q->dact, in,
-                       q->g + L.cw[l], q->g + L.cb[l], n, CH[l], CH[l + 1], side);
+    const int njp = conv_njp(l), jt = conv_jt(l);
+    const int nchunks = (int)(((int64_t)n * side * side + WCHUNK - 1) / WCHUNK);
+    const unsigned wblocks = mma_blocks((int64_t)nchunks * (CH[l + 1] / 16) * (njp / (16 * jt)));
+    if (jt == 4)
+      hipLaunchKernelGGL(k_conv_bwd_w<4>, dim3(wblocks), dim3(MW * 64), 0, q->stream, q->dact, in, q->wpart, n, CH[l],
+                         CH[l + 1], side, WCHUNK, nchunks, njp);
+    else
+      hipLaunchKernelGGL(k_conv_bwd_w<1>, dim3(wblocks), dim3(MW * 64), 0, q->stream, q->dact, in, q->wpart, n, CH[l],
+                         CH[l + 1], side, WCHUNK, nchunks, njp);
+    hipLaunchKernelGGL(k_conv_bwd_w_sum, dim3(wave_blocks((int64_t)CH[l + 1] * (CH[l] * 9 + 1))), dim3(T), 0,
+                       q->stream, q->wpart, q->g + L.cw[l], q->g + L.cb[l], CH[l], CH[l + 1], nchunks, njp);
     if (l > 0) {
-      hipLaunchKernelGGL(k_conv_bwd_data, dim3(blocks_for((int64_t)n * CH[l] * side * side)), dim3(T), 0, q->stream,
-                         q->dact, q->p + L.cw[l], q->dpool, n, CH[l], CH[l + 1], side);
+      launch_conv<false>(q, l, q->dact, q->dpool, n);
       dp = q->dpool;
     }
   }
@@ -432,6 +610,7 @@ vl_status vl_nn_create(int max_batch, uint64_t seed, vl_nn** out) {
   }
   VL_HIP(hipMalloc((void**)&q->dact, (size_t)biggest * sizeof(float)));
   VL_HIP(hipMalloc((void**)&q->dpool, (size_t)biggest * sizeof(float)));
+  VL_HIP(hipMalloc((void**)&q->wpart, (size_t)wpart_floats(B) * sizeof(float)));
   VL_HIP(hipMalloc((void**)&q->h, (size_t)(B * HID) * sizeof(float)));
   VL_HIP(hipMalloc((void**)&q->dh, (size_t)(B * HID) * sizeof(float)));
   VL_HIP(hipMalloc((void**)&q->y, (size_t)(B * OUT) * sizeof(float)));

@@ -212,6 +212,26 @@ def solve(be: api.Backend, cases, discard_units: float = 10.0, precision: int = 
     return y, ok, secs, lu
 
 
+def save_datasets(path: str, **sets: Dataset) -> None:
+    """Writes solver-labelled datasets to one .npz file (arrays only, no pickled objects)."""
+    arrs = {}
+    for name, ds in sets.items():
+        arrs.update({f"{name}_masks": ds.masks, f"{name}_re": ds.re, f"{name}_y": ds.y, f"{name}_ok": ds.ok,
+                     f"{name}_meta": np.array([ds.solve_seconds, ds.lattice_updates], dtype=np.float64)})
+    np.savez(path, **arrs)
+
+
+def load_datasets(path: str, *names: str):
+    """The datasets `names` from a file written by save_datasets."""
+    with np.load(path) as f:
+        out = []
+        for name in names:
+            meta = f[f"{name}_meta"]
+            out.append(Dataset(f[f"{name}_masks"], f[f"{name}_re"], f[f"{name}_y"], f[f"{name}_ok"], float(meta[0]),
+                               int(meta[1])))
+    return out
+
+
 def make_dataset(be, seed, n, **kw) -> Dataset:
     cases, masks, re = sample_cases(seed, n, **kw)
     y, ok, secs, lu = solve(be, cases)
@@ -302,9 +322,13 @@ def main():
     ap.add_argument("--test", type=int, default=1024)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--members", type=int, default=8, help="ensemble members")
     ap.add_argument("--target-p90", type=float, default=0.05,
                     help="stated error of accepted answers: their 90th-percentile relative Cd error")
     ap.add_argument("--out", default="")
+    ap.add_argument("--save-datasets", default="", help="write the solver-labelled sets to this .npz file")
+    ap.add_argument("--load-datasets", default="",
+                    help="train and evaluate on the sets of a --save-datasets file instead of solving")
     args = ap.parse_args()
     import sys
 
@@ -315,12 +339,18 @@ def main():
 
     be = vl.backend()
     t0 = time.perf_counter()
-    tr = make_dataset(be, args.seed, args.train)
+    if args.load_datasets:
+        tr, te = load_datasets(args.load_datasets, "train", "test")
+        args.train, args.test = len(tr.ok), len(te.ok)
+    else:
+        tr = make_dataset(be, args.seed, args.train)
+        te = make_dataset(be, args.seed + 1, args.test)
     log(f"[surrogate] training set: {tr.ok.sum()} of {args.train} cases solved in {tr.solve_seconds:.1f} s "
         f"({tr.lattice_updates / tr.solve_seconds / 1e9:.1f} GLUPS)")
-    te = make_dataset(be, args.seed + 1, args.test)
     log(f"[surrogate] held-out set: {te.ok.sum()} of {args.test} cases solved in {te.solve_seconds:.1f} s")
-    ens = Ensemble(be, 4, seed=args.seed)
+    if args.save_datasets:
+        save_datasets(args.save_datasets, train=tr, test=te)
+    ens = Ensemble(be, args.members, seed=args.seed)
     t1 = time.perf_counter()
     ens.fit(tr.masks[tr.ok], tr.re[tr.ok], tr.y[tr.ok], epochs=args.epochs, log=log)
     t_train = time.perf_counter() - t1
@@ -337,6 +367,8 @@ def main():
     threshold = calibrate_threshold(spread[cal], rel[cal, 0], args.target_p90)
     rep_cal = fallback_report(spread[cal], rel[cal, 0], threshold)
     rep_ev = fallback_report(spread[ev], rel[ev, 0], threshold)
+    # how well the spread ranks the answers by their error (Spearman's rank correlation)
+    rank_corr = float(np.corrcoef(np.argsort(np.argsort(spread)), np.argsort(np.argsort(rel[:, 0])))[0, 1])
     log(f"[surrogate] threshold {threshold:.4f} for p90 <= {args.target_p90}: calibration accepts "
         f"{rep_cal['accepted_fraction']:.2f}; evaluation half accepts {rep_ev['accepted_fraction']:.2f} with p90 "
         f"{rep_ev['p90_rel_err_accepted']}")
@@ -354,6 +386,7 @@ def main():
         "rule": f"accept when the ensemble spread of Cd is <= the threshold calibrated on half the held-out "
                 f"shapes for a p90 relative Cd error <= {args.target_p90:.0%}; otherwise solve",
         "target_p90_rel_err_cd": args.target_p90,
+        "members": args.members, "spread_error_rank_correlation": round(rank_corr, 4),
         "calibration_half": rep_cal, "evaluation_half": rep_ev,
         "accepted_fraction": round(frac, 4),
         "median_rel_error_all": {k: float(np.median(rel[ev, i])) for i, k in enumerate(OUT_NAMES)},
