@@ -139,7 +139,7 @@ typedef struct {
   int64_t pass_tasks;      /* its tasks (one wave each) */
   int64_t listed_nodes;    /* interior nodes stepped one step at a time beside it */
   int32_t pass_rows;       /* output rows per task, at most */
-  int32_t pad_;
+  int32_t pass_joined_tasks; /* tasks in joined strips (a workgroup's waves sweep one strip) */
   int64_t pass_far_nodes;  /* covered nodes of the far tasks (their pass overlaps the single steps) */
   int64_t pass_far_tasks;
 } vl_sim_info;
@@ -204,14 +204,18 @@ vl_status vl_sim_get_timing(vl_sim* s, double* bulk_ms, int64_t* bulk_launches, 
  * the planner behind the K-step passes that vl_sim_create and vl_batch_build make, exposed for
  * tests and diagnostics).  flags: nx * ny bytes, 0 interior, 1 boundary, 2 solid; k steps per
  * pass (2..5), v columns per lane (1, 2), tasks of at most `rows` output rows, strips from column
- * x0 (the halo plus a multiple of v).  waves > 0 also sets the lockstep bits as vl_sim_create
- * does for workgroups of that many waves, launched as far and near tasks apart (overlap != 0) or
- * as one list (overlap == 0); waves == 0 leaves them clear.  Writes up to max_tasks tasks, 4 int32
- * each (first output column, first row, rows, output columns | ring-edge bits << 8: first row,
- * last row, first column, last column | lockstep << 12), far tasks first, and up to max_listed
- * listed nodes (y << 16 | x); counts = {tasks, far tasks, listed nodes, covered nodes}. */
+ * x0 (the halo plus a multiple of v).  join != 0 (with waves >= 2) plans joined strips, each
+ * swept by a workgroup of `waves` waves together, where they fit.  waves > 0 also sets the
+ * lockstep bits as vl_sim_create does for workgroups of that many waves, far and near tasks
+ * launched apart; waves == 0 leaves them clear.  Writes up to max_tasks tasks, 4 int32 each
+ * (first output column, first row, rows, output columns | ring-edge bits << 8: first row, last
+ * row, first column, last column | lockstep << 12 | joined << 13 | the wave's first output column
+ * less its first loaded column << 16, for joined tasks), in the order far joined groups, far
+ * single-wave tasks, near joined groups, near single-wave tasks; and up to max_listed listed
+ * nodes (y << 16 | x); counts (6 values) = {tasks, far tasks, listed nodes, covered nodes, tasks
+ * in far joined groups, tasks in near joined groups}. */
 vl_status vl_plan_passes(const uint8_t* flags, int nx, int ny, int k, int v, int rows, int x0, int waves,
-                         int overlap, int32_t* tasks, int64_t max_tasks, uint32_t* listed, int64_t max_listed,
+                         int join, int32_t* tasks, int64_t max_tasks, uint32_t* listed, int64_t max_listed,
                          int64_t* counts);
 /* The lockstep bits of a pass task list, host only (no reference counterpart; the grouping that
  * vl_sim_create and vl_batch_build apply before the pass kernels run).  tasks: ntasks x 4 int32
@@ -220,7 +224,9 @@ vl_status vl_plan_passes(const uint8_t* flags, int nx, int ny, int k, int v, int
  * begin.  Unless check_only, sets bit 12 of tasks[4i+3] on exactly the full workgroups whose
  * tasks share first row, row count and owner, and clears it elsewhere.  Then checks that every
  * workgroup of every range holds no marked task or only such a full group: VL_EINVAL if not
- * (a workgroup whose waves met different barriers would hang). */
+ * (a workgroup whose waves met different barriers would hang).  Joined tasks (bit 13, from
+ * vl_plan_passes) keep their bits and must fill whole workgroups whose waves load adjacent runs
+ * of 64 or 128 columns. */
 vl_status vl_pass_lockstep(int32_t* tasks, const int32_t* owner, int64_t ntasks, const int64_t* ranges, int nranges,
                            int waves, int check_only);
 

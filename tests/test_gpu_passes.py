@@ -22,13 +22,14 @@ KS = [2, 3, 4]
 
 
 def passes(monkeypatch, k, rows=None, graphs=True, layout=None):
-    """layout = (waves per workgroup, lockstep, first strip column, columns per lane), or None
-    for the defaults."""
+    """layout = (waves per workgroup, lockstep, first strip column, columns per lane, joined
+    strips), or None for the defaults (trailing entries may be left out)."""
     monkeypatch.setenv("VL_PASS_STEPS", str(k))
     monkeypatch.delenv("VL_STEP_MODE", raising=False)
     monkeypatch.delenv("VL_BULK_V", raising=False)
-    names = ("VL_TB_WAVES", "VL_TB_SYNC", "VL_TB_X0", "VL_TB_V")
-    for name, value in zip(names, layout or (None,) * len(names)):
+    names = ("VL_TB_WAVES", "VL_TB_SYNC", "VL_TB_X0", "VL_TB_V", "VL_TB_JOIN")
+    layout = tuple(layout or ()) + (None,) * (len(names) - len(layout or ()))
+    for name, value in zip(names, layout):
         if value is None:
             monkeypatch.delenv(name, raising=False)
         else:
@@ -52,15 +53,18 @@ def single_step(monkeypatch, mode="split"):
 
 @pytest.mark.parametrize("k", KS)
 @pytest.mark.parametrize("rows,graphs,layout", [(64, True, None), (7, True, None), (1, False, None),
-                                                (12, True, (4, 0, 2, None)), (5, True, (8, 1, 7, None))])
+                                                (12, True, (4, 0, 2, None)), (5, True, (8, 1, 7, None)),
+                                                (16, True, (8, 1, None, None, 1)), (9, True, (4, 1, 35, None, 1))])
 def test_pass_published_hash(gpu, monkeypatch, k, rows, graphs, layout):
     """cylinder::bench n=40, 3000 steps (docs/report/index.html:59) with K-step passes, in the
-    default layout (8 waves per workgroup in step, strips from x = 32) and others."""
+    default layout (4 waves per workgroup for FP64, strips from x = 32) and others, joined strips
+    among them."""
     print(f"[passes] hash n=40 K={k} rows={rows} graphs={graphs} layout={layout}", flush=True)
     passes(monkeypatch, k, rows, graphs, layout)
     ch = api.Channel(gpu, cylinder_rig_spec(40), api.F64)
     info = ch.sim.info()
     assert info.pass_steps == k and info.split == 1
+    assert (info.pass_joined_tasks > 0) == (layout is not None and len(layout) == 5 and layout[4] == 1)
     assert info.pass_nodes > 0.7 * info.fluid_nodes
     assert info.pass_nodes + info.listed_nodes + info.bnode_count == info.fluid_nodes
     ch.run(3000, 40.0)
@@ -107,6 +111,31 @@ def test_pass_equals_single_step_reduced_precision(gpu, monkeypatch, k, prec, ro
     ch.close()
 
 
+@pytest.mark.parametrize("k,prec,rows,v", [(k, p, r, v) for k in KS for p in (api.F32, api.F16)
+                                            for r, v in ((64, 1), (3, 1), (12, 2))]
+                         + [(5, api.F32, 20, 1), (5, api.F32, 3, 1)])
+def test_joined_pass_equals_single_step(gpu, monkeypatch, k, prec, rows, v):
+    """Joined strips (a workgroup's waves sweep one strip and hand each other their edge values
+    at every level) on the tunnel with a square body, in workgroups of 4 waves (256 V columns, so
+    that they fit its width): the single-step bits, populations and forces."""
+    waves = 4
+    print(f"[passes] joined K={k} prec={prec} rows={rows} V={v} waves={waves}", flush=True)
+    spec = tunnel_spec(d=16 if v == 1 else 32, re=100.0)
+    single_step(monkeypatch)
+    ref = api.Channel(gpu, spec, prec)
+    ref_coef = ref.run(1201, 16.0)
+    ref_pops = ref.sim.read()
+    ref.close()
+    passes(monkeypatch, k, rows, layout=(waves, 1, None, v, 1))
+    ch = api.Channel(gpu, spec, prec)
+    info = ch.sim.info()
+    assert info.pass_joined_tasks > 0 and info.pass_joined_tasks % waves == 0
+    coef = ch.run(1201, 16.0)
+    assert np.array_equal(ch.sim.read(), ref_pops), "populations differ"
+    assert np.array_equal(coef, ref_coef), "forces differ"
+    ch.close()
+
+
 @pytest.mark.parametrize("workload,prec,steps", [("notched_8192x4096", api.F32, 43),
                                                  ("circle_16384x8192", api.F16, 23)])
 def test_pass_full_size_equals_single_step(gpu, monkeypatch, workload, prec, steps):
@@ -123,12 +152,14 @@ def test_pass_full_size_equals_single_step(gpu, monkeypatch, workload, prec, ste
     ref_coef = ref.run(steps, d)
     ref_hash = ref.sim.state_hash()
     ref.close()
-    for name in ("VL_PASS_STEPS", "VL_STEP_MODE", "VL_TB_ROWS", "VL_TB_WAVES", "VL_TB_SYNC", "VL_TB_X0", "VL_TB_V"):
+    for name in ("VL_PASS_STEPS", "VL_STEP_MODE", "VL_TB_ROWS", "VL_TB_WAVES", "VL_TB_SYNC", "VL_TB_X0", "VL_TB_V",
+                 "VL_TB_JOIN"):
         monkeypatch.delenv(name, raising=False)
     ch = api.Channel(gpu, spec, prec)
     info = ch.sim.info()
     assert info.pass_steps == (3 if prec == api.F16 else 5)
     assert info.pass_nodes > 0.98 * info.fluid_nodes
+    assert info.pass_joined_tasks == 0
     coef = ch.run(steps, d)
     assert ch.sim.state_hash() == ref_hash
     assert np.array_equal(coef, ref_coef)

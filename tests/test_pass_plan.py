@@ -52,17 +52,52 @@ def random_case(seed):
     return channel_flags(nx, ny, bodies, rects)
 
 
-def check_plan(fl, k, v, rows, x0):
+def joined_groups(tasks, nfar, waves):
+    """The joined groups of a plan: (first task, end) for each, checked to be whole workgroups at
+    the start of the far range and of the near range."""
+    joined = (tasks[:, 3] >> 13) & 1
+    groups = []
+    for b, e in ((0, nfar), (nfar, len(tasks))):
+        n = b
+        while n < e and joined[n]:
+            n += 1
+        assert (n - b) % waves == 0, "joined tasks fill whole workgroups"
+        assert not joined[n:e].any(), "joined groups come first in their range"
+        groups += [(g, g + waves) for g in range(b, n, waves)]
+    return groups
+
+
+def check_plan(fl, k, v, rows, x0, waves=0):
+    """The plan's invariants; waves > 0 plans joined strips for workgroups of that many waves."""
     ny, nx = fl.shape
-    tasks, nfar, listed, ncov = api.plan_passes(vl.backend(), fl, k, v, rows, x0)
+    tasks, nfar, listed, ncov = api.plan_passes(vl.backend(), fl, k, v, rows, x0, join=waves > 0,
+                                                waves=waves if waves else 0)
     interior = fl == 0
     fluid = fl != 2
     covered = np.zeros((ny, nx), np.int32)
     owner = -np.ones((ny, nx), np.int64)
+    groups = joined_groups(tasks, nfar, waves) if waves else []
+    group_of = -np.ones(len(tasks), np.int64)
+    for gi, (g0, g1) in enumerate(groups):
+        group_of[g0:g1] = gi
+        grp = tasks[g0:g1]
+        assert (grp[:, 1] == grp[0, 1]).all() and (grp[:, 2] == grp[0, 2]).all()
+        assert ((grp[:, 3] >> 12) & 3 == 3).all(), "joined tasks are lockstep"
+        loaded = grp[:, 0] - ((grp[:, 3] >> 16) & 0xFF)  # each wave's first loaded column
+        assert (np.diff(loaded) == 64 * v).all(), "the waves load adjacent strips"
+        assert loaded[0] % v == 0, "aligned vector loads"
+        ends = grp[:, 0] + (grp[:, 3] & 0xFF)
+        assert (grp[1:, 0] == ends[:-1]).all(), "the waves' outputs are contiguous"
+        # outputs where level K is valid: K in from each end of the loaded columns
+        assert grp[0, 0] - loaded[0] >= k and loaded[-1] + 64 * v - ends[-1] >= k
     for i, (tx, ty, tz, tw) in enumerate(tasks):
         w, edges = int(tw) & 0xFF, (int(tw) >> 8) & 15
-        assert (int(tw) >> 12) == 0, "the caller sets the lockstep bit"
-        assert 1 <= w <= tb_out(k, v) and w % v == 0 and (tx - tb_halo(k, v)) % v == 0
+        if group_of[i] < 0:
+            # the lockstep bit only as the caller's grouping sets it (waves > 0), never joined
+            assert (int(tw) >> (13 if waves else 12)) == 0
+            assert 1 <= w <= tb_out(k, v) and w % v == 0 and (tx - tb_halo(k, v)) % v == 0
+        else:
+            assert 1 <= w <= 64 * v and w % v == 0
         assert 1 <= tz <= rows
         covered[ty:ty + tz, tx:tx + w] += 1
         owner[ty:ty + tz, tx:tx + w] = i
@@ -94,17 +129,26 @@ def check_plan(fl, k, v, rows, x0):
                 assert any(out for out, _ in sides)
                 for out, bit in sides:
                     assert not out or edges & bit, f"ring node ({rx},{ry}): edge bit {bit} missing"
-    # far tasks first: no uncovered fluid node within K of their outputs, and no ring edges
-    for i, (tx, ty, tz, tw) in enumerate(tasks):
-        w, edges = int(tw) & 0xFF, (int(tw) >> 8) & 15
-        box = uncovered[max(ty - k, 0):ty + tz + k, max(tx - k, 0):tx + w + k]
+    # far tasks first: no uncovered fluid node within K of their outputs, and no ring edges; a
+    # joined group is near when one of its waves is
+    def near_box(t):
+        tx, ty, tz, tw = (int(a) for a in t)
+        w = tw & 0xFF
+        return uncovered[max(ty - k, 0):ty + tz + k, max(tx - k, 0):tx + w + k].any()
+
+    for i, t in enumerate(tasks):
+        edges = (int(t[3]) >> 8) & 15
         if i < nfar:
-            assert not box.any() and edges == 0, f"far task {i} near an uncovered node"
+            assert not near_box(t) and edges == 0, f"far task {i} near an uncovered node"
+        elif group_of[i] < 0:
+            assert near_box(t), f"near task {i} has no uncovered node within K"
         else:
-            assert box.any(), f"near task {i} has no uncovered node within K"
-    for part in (tasks[:nfar], tasks[nfar:]):
-        keys = [(int(t[1]), int(t[0])) for t in part]
-        assert keys == sorted(keys)
+            g0, g1 = groups[group_of[i]]
+            assert any(near_box(u) for u in tasks[g0:g1]), f"near group at {g0} has no uncovered node within K"
+    for b, e in ((0, nfar), (nfar, len(tasks))):
+        single = [(int(t[1]), int(t[0])) for i, t in enumerate(tasks[b:e], b) if group_of[i] < 0]
+        heads = [(int(tasks[g0][1]), int(tasks[g0][0])) for g0, _ in groups if b <= g0 < e]
+        assert single == sorted(single) and heads == sorted(heads)
     return len(tasks), nfar, len(listed), ncov
 
 
@@ -162,16 +206,45 @@ def check_lockstep(tasks, ranges, waves, owner=None):
     return nmarked
 
 
+def wide_case(seed):
+    """A channel wide enough for joined strips, with bodies and a wall block."""
+    rng = np.random.default_rng(seed)
+    nx, ny = int(rng.integers(600, 1500)), int(rng.integers(40, 220))
+    bodies = [(rng.uniform(0.1, 0.9) * nx, rng.uniform(0.25, 0.75) * ny, rng.uniform(2.0, 0.2 * ny))
+              for _ in range(int(rng.integers(1, 4)))]
+    x0 = int(rng.integers(5, nx - 40))
+    y0 = int(rng.integers(3, ny - 12))
+    return channel_flags(nx, ny, bodies, [(x0, x0 + int(rng.integers(2, 30)), y0, y0 + int(rng.integers(2, 9)))])
+
+
+@pytest.mark.parametrize("seed", range(5))
+@pytest.mark.parametrize("k,v,rows,waves", [(5, 1, 20, 8), (4, 1, 16, 4), (3, 2, 24, 8), (2, 1, 5, 2)])
+def test_joined_plan_invariants(seed, k, v, rows, waves):
+    """Joined strips (a workgroup's waves sweep one strip, handing each other their edge values)
+    keep every invariant of the plan, and the rows a body takes from a joined strip's wide
+    footprint go to single-wave strips over its columns, not to the single-step list."""
+    fl = wide_case(seed)
+    ntask, nfar, nlisted, ncov = check_plan(fl, k, v, rows, 32, waves=waves)
+    _, _, nlisted1, ncov1 = check_plan(fl, k, v, rows, 32)
+    tasks, _, _, _ = api.plan_passes(vl.backend(), fl, k, v, rows, 32, waves=waves, join=True)
+    gout = (64 * v * waves - tb_halo(k, v) - k) // v * v
+    if fl.shape[1] >= 32 + gout + 2 * k:
+        assert ((tasks[:, 3] >> 13) & 1).any(), "some joined strips on a channel this wide"
+    # the fallback strips lie on the single-wave grid, cut at the joined strips' ends, so they
+    # leave no more single-step nodes than the single-wave plan
+    assert nlisted <= nlisted1 and ncov >= ncov1
+
+
 @pytest.mark.parametrize("seed", range(6))
 @pytest.mark.parametrize("k,v,rows,waves", [(5, 1, 20, 8), (4, 1, 16, 4), (3, 2, 24, 8), (2, 1, 3, 8)])
-@pytest.mark.parametrize("overlap", [True, False])
-def test_lockstep_groups_of_the_launched_ranges(seed, k, v, rows, waves, overlap):
-    """vl_sim_create's lockstep bits hold over the ranges it launches: far and near tasks apart
-    when they overlap, one list otherwise (also when the far count is no multiple of the
-    workgroup width)."""
-    fl = random_case(100 + seed)
-    tasks, nfar, _, _ = api.plan_passes(vl.backend(), fl, k, v, rows, 32, waves=waves, overlap=overlap)
-    ranges = [(0, nfar), (nfar, len(tasks))] if overlap else [(0, len(tasks))]
+@pytest.mark.parametrize("join", [False, True])
+def test_lockstep_groups_of_the_launched_ranges(seed, k, v, rows, waves, join):
+    """vl_sim_create's lockstep bits hold over the ranges it launches, far and near tasks apart
+    (also when the far count is no multiple of the workgroup width), with and without joined
+    strips."""
+    fl = wide_case(100 + seed) if join else random_case(100 + seed)
+    tasks, nfar, _, _ = api.plan_passes(vl.backend(), fl, k, v, rows, 32, waves=waves, join=join)
+    ranges = [(0, nfar), (nfar, len(tasks))]
     check_lockstep(tasks, ranges, waves)
     # the same bits re-derived by the host-only entry agree, and pass its own check
     again = api.pass_lockstep(vl.backend(), tasks, ranges, waves)
@@ -186,16 +259,13 @@ def test_lockstep_check_rejects_groups_across_launch_ranges(nx, ny, r, k, rows):
     far count is no multiple of 8."""
     be = vl.backend()
     fl = channel_flags(nx, ny, [(0.3 * nx, 0.5 * ny, r)])
-    tasks, nfar, _, _ = api.plan_passes(be, fl, k, 1, rows, 32, waves=8, overlap=True)
+    tasks, nfar, _, _ = api.plan_passes(be, fl, k, 1, rows, 32, waves=8)
     assert nfar % 8 != 0
     with pytest.raises(AssertionError):
         check_lockstep(tasks, [(0, len(tasks))], 8)
     with pytest.raises(api.VortexError):
         api.pass_lockstep(be, tasks, [(0, len(tasks))], 8, check_only=True)
     api.pass_lockstep(be, tasks, [(0, nfar), (nfar, len(tasks))], 8, check_only=True)
-    # planned for one launch (overlap off), the bits are consistent with that launch
-    one, nfar1, _, _ = api.plan_passes(be, fl, k, 1, rows, 32, waves=8, overlap=False)
-    assert nfar1 == nfar and check_lockstep(one, [(0, len(one))], 8) > 0
 
 
 @pytest.mark.parametrize("seed", range(4))

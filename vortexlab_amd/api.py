@@ -45,7 +45,7 @@ class SimInfo(C.Structure):
                 ("omega_plus", C.c_double), ("omega_minus", C.c_double),
                 ("outlet_density_offset", C.c_double), ("device_bytes", C.c_int64), ("split", C.c_int32),
                 ("pass_steps", C.c_int32), ("pass_nodes", C.c_int64), ("pass_tasks", C.c_int64),
-                ("listed_nodes", C.c_int64), ("pass_rows", C.c_int32), ("pad_", C.c_int32),
+                ("listed_nodes", C.c_int64), ("pass_rows", C.c_int32), ("pass_joined_tasks", C.c_int32),
                 ("pass_far_nodes", C.c_int64), ("pass_far_tasks", C.c_int64)]
 
 
@@ -200,20 +200,22 @@ class Backend:
 
 
 def plan_passes(be: Backend, flags: np.ndarray, k: int, v: int = 1, rows: int = 20, x0: int = 32, waves: int = 0,
-                overlap: bool = True):
+                join: bool = False):
     """The K-step pass plan of a domain (flags: ny x nx, 0 interior, 1 boundary, 2 solid), host
     only: (tasks as an (n, 4) int32 array of first column, first row, rows, columns | ring-edge
-    bits << 8 | lockstep << 12, far tasks first; the number of far tasks; listed nodes as
-    (y << 16 | x); covered node count).  waves > 0 sets the lockstep bits as a simulation with
-    workgroups of that many waves does (far and near tasks launched apart when `overlap`)."""
+    bits << 8 | lockstep << 12 | joined << 13 | offset of the first output column in the wave's
+    columns << 16, in the order far joined groups, far single-wave tasks, near joined groups, near
+    single-wave tasks; the number of far tasks; listed nodes as (y << 16 | x); covered node count).
+    `join` plans joined strips for workgroups of `waves` waves; waves > 0 sets the lockstep bits
+    as a simulation with workgroups of that many waves does (far and near tasks launched apart)."""
     fl = np.ascontiguousarray(flags, dtype=np.uint8)
     ny, nx = fl.shape
-    counts = np.zeros(4, dtype=np.int64)
+    counts = np.zeros(6, dtype=np.int64)
     ptr = lambda a: a.ctypes.data_as(C.c_void_p)
-    be.check(be.plan_passes(ptr(fl), nx, ny, k, v, rows, x0, waves, int(overlap), None, 0, None, 0, ptr(counts)))
+    be.check(be.plan_passes(ptr(fl), nx, ny, k, v, rows, x0, waves, int(join), None, 0, None, 0, ptr(counts)))
     tasks = np.zeros((max(int(counts[0]), 1), 4), dtype=np.int32)
     listed = np.zeros(max(int(counts[2]), 1), dtype=np.uint32)
-    be.check(be.plan_passes(ptr(fl), nx, ny, k, v, rows, x0, waves, int(overlap), ptr(tasks), int(counts[0]),
+    be.check(be.plan_passes(ptr(fl), nx, ny, k, v, rows, x0, waves, int(join), ptr(tasks), int(counts[0]),
                             ptr(listed), int(counts[2]), ptr(counts)))
     return tasks[:int(counts[0])], int(counts[1]), listed[:int(counts[2])], int(counts[3])
 

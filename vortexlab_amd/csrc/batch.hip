@@ -907,7 +907,7 @@ vl_status build(vl_batch* b) {
         for (size_t i = w; i < nc; i += nth) {
           const Tables& tab = b->tabs[i];
           st[i] = plan_passes(tab.flag, (int)tab.nx, (int)tab.ny, b->hcases[i].g.pitch, K, V, b->pass_rows, x0_grid,
-                              plans[i]);
+                              0, plans[i]);
           if (st[i] != VL_OK) err[i] = vl_last_error();
         }
       });

@@ -70,16 +70,25 @@ __global__ __launch_bounds__(BULK_T) void k_bulk(const typename S::T* __restrict
 }
 
 // K steps of the covered interior in one pass: one task per wave, W waves per workgroup, V
-// columns per lane, D rows of loads in flight.  FP32 and FP16 storage keep 4 waves per SIMD
-// (at most 128 registers; the FP16 pass needed 130 unbounded, and 3 waves ran 11 % slower);
-// FP64 needs about 185 and runs at 2.
-template <class S, int M, int W, int V, int D, int K>
+// columns per lane, D rows of loads in flight; JOIN: joined tasks only (a workgroup's waves
+// sweep one strip), else single-wave tasks only (the two forms are separate kernels, each with
+// its own register allocation).  FP32 and FP16 storage keep 4 waves per SIMD (at most 128
+// registers; the FP16 pass needed 130 unbounded, and 3 waves ran 11 % slower); FP64 needs about
+// 185 and runs at 2.
+template <class S, int M, int W, int V, int D, int K, bool JOIN>
 __global__ __launch_bounds__(W * TB_LANES, sizeof(typename S::T) < 8 ? 4 : 1) void k_pass(const typename S::T* __restrict__ A,
                                                       PassOut<typename S::T> po, const int4* __restrict__ tasks,
                                                       int ntasks, Geom g, Coll c) {
   const int w = __builtin_amdgcn_readfirstlane((int)blockIdx.x * W + (int)(threadIdx.x / TB_LANES));
-  if (w >= ntasks) return;
-  multi_step_strip<S, M, V, D, K>(tasks[w], A, po, g, c);
+  if constexpr (JOIN) {
+    __shared__ typename S::C xch[2 * W * TB_XCH];  // edge values of the waves
+    if (w >= ntasks) return;  // never within a workgroup: joined tasks fill whole workgroups
+    multi_step_strip<S, M, V, D, K, true>(tasks[w], A, po, g, c, xch,
+                                          __builtin_amdgcn_readfirstlane((int)(threadIdx.x / TB_LANES)), W);
+  } else {
+    if (w >= ntasks) return;
+    multi_step_strip<S, M, V, D, K, false>(tasks[w], A, po, g, c);
+  }
 }
 
 // One single step of the nodes a K-step pass does not cover, in one launch: workgroups
@@ -241,43 +250,52 @@ static vl_status launch_step(vl_sim* s, const int64_t* kbase, int k_local, bool 
   return VL_OK;
 }
 
-// The pass kernel over the tasks [tasks, tasks + n) on stream `st`.
+// The pass kernel over the tasks [tasks, tasks + n) on stream `st`, of which the first nj are
+// joined (whole workgroups): one launch for those, one for the rest.
+template <class S, int M, int W, int V, int D, int K>
+static void launch_pass_w(vl_sim* s, hipStream_t st, const int4* tasks, int n, int nj, const typename S::T* A,
+                          const PassOut<typename S::T>& po) {
+  if (nj > 0)
+    hipLaunchKernelGGL((k_pass<S, M, W, V, D, K, true>), dim3((unsigned)(nj / W)), dim3(W * TB_LANES), 0, st, A, po,
+                       tasks, nj, s->g, s->coll);
+  if (n > nj)
+    hipLaunchKernelGGL((k_pass<S, M, W, V, D, K, false>), dim3((unsigned)((n - nj + W - 1) / W)), dim3(W * TB_LANES),
+                       0, st, A, po, tasks + nj, n - nj, s->g, s->coll);
+}
+
 template <class S, int M, int V, int D, int K>
-static void launch_pass(vl_sim* s, hipStream_t st, const int4* tasks, int n, const typename S::T* A,
+static void launch_pass(vl_sim* s, hipStream_t st, const int4* tasks, int n, int nj, const typename S::T* A,
                         const PassOut<typename S::T>& po) {
-  const unsigned grid = (unsigned)((n + s->tb_waves - 1) / s->tb_waves);
   if (s->tb_waves == 8)
-    hipLaunchKernelGGL((k_pass<S, M, 8, V, D, K>), dim3(grid), dim3(8 * TB_LANES), 0, st, A, po, tasks, n, s->g,
-                       s->coll);
+    launch_pass_w<S, M, 8, V, D, K>(s, st, tasks, n, nj, A, po);
   else
-    hipLaunchKernelGGL((k_pass<S, M, 4, V, D, K>), dim3(grid), dim3(4 * TB_LANES), 0, st, A, po, tasks, n, s->g,
-                       s->coll);
+    launch_pass_w<S, M, 4, V, D, K>(s, st, tasks, n, nj, A, po);
 }
 
 template <class S, int M>
-static void launch_pass_tasks(vl_sim* s, hipStream_t st, const int4* tasks, int n, const typename S::T* A,
+static void launch_pass_tasks(vl_sim* s, hipStream_t st, const int4* tasks, int n, int nj, const typename S::T* A,
                               const PassOut<typename S::T>& po) {
   if (n <= 0) return;
   const int K = s->tb_k;
   const bool wide = sizeof(typename S::T) < 8 && s->tb_v == 2;  // V = 2 (D = 2) or V = 1 (D = 3)
   if (K == 5) {
     if constexpr (sizeof(typename S::T) == 4)  // FP32 only (FP64 and FP16 are slower above 4 / 3)
-      launch_pass<S, M, 1, 3, 5>(s, st, tasks, n, A, po);
+      launch_pass<S, M, 1, 3, 5>(s, st, tasks, n, nj, A, po);
   } else if (K == 4) {
     if (wide)
-      launch_pass<S, M, 2, 2, 4>(s, st, tasks, n, A, po);
+      launch_pass<S, M, 2, 2, 4>(s, st, tasks, n, nj, A, po);
     else
-      launch_pass<S, M, 1, 3, 4>(s, st, tasks, n, A, po);
+      launch_pass<S, M, 1, 3, 4>(s, st, tasks, n, nj, A, po);
   } else if (K == 3) {
     if (wide)
-      launch_pass<S, M, 2, 2, 3>(s, st, tasks, n, A, po);
+      launch_pass<S, M, 2, 2, 3>(s, st, tasks, n, nj, A, po);
     else
-      launch_pass<S, M, 1, 3, 3>(s, st, tasks, n, A, po);
+      launch_pass<S, M, 1, 3, 3>(s, st, tasks, n, nj, A, po);
   } else {
     if (wide)
-      launch_pass<S, M, 2, 2, 2>(s, st, tasks, n, A, po);
+      launch_pass<S, M, 2, 2, 2>(s, st, tasks, n, nj, A, po);
     else
-      launch_pass<S, M, 1, 3, 2>(s, st, tasks, n, A, po);
+      launch_pass<S, M, 1, 3, 2>(s, st, tasks, n, nj, A, po);
   }
 }
 
@@ -307,7 +325,7 @@ static vl_status launch_group(vl_sim* s, const int64_t* kbase, int k_local, bool
   for (int j = 1; j < K; ++j) po.ring[j - 1] = b[j];
   po.out = b[K];
   const bool timed = timed_ok && s->timing && s->ev_used + 2 <= (int64_t)s->ev.size();
-  const int nfar = s->tb_overlap ? s->ntasks_far : 0, nnear = s->ntasks - nfar;
+  const int nfar = s->ntasks_far, nnear = s->ntasks - nfar;
   bool first = false;
   if (!s->ov_open) {  // fork the side stream from everything before this group
     VL_HIP(hipEventRecord(s->ev_fork, s->stream));
@@ -318,7 +336,8 @@ static vl_status launch_group(vl_sim* s, const int64_t* kbase, int k_local, bool
   const int p = s->ov_parity;
   // side stream: near pass, phase 1, (event), phases 2..K
   if (!first) VL_HIP(hipStreamWaitEvent(s->side, s->ev_far[p ^ 1], 0));
-  launch_pass_tasks<S, M>(s, s->side, s->dtasks + nfar, nnear, b[0], po);
+  launch_pass_tasks<S, M>(s, s->side, s->dtasks + nfar, nnear, s->tb_joined_near, b[0], po);
+  if (!s->tb_overlap) launch_pass_tasks<S, M>(s, s->side, s->dtasks, nfar, s->tb_joined_far, b[0], po);
   launch_phase<S, M>(s, s->side, b[0], b[1], step_args(s, s->t, kbase, k_local));
   VL_HIP(hipEventRecord(s->ev_near[p], s->side));
   for (int j = 2; j <= K; ++j)
@@ -326,7 +345,7 @@ static vl_status launch_group(vl_sim* s, const int64_t* kbase, int k_local, bool
   // main stream: far pass
   if (!first) VL_HIP(hipStreamWaitEvent(s->stream, s->ev_near[p ^ 1], 0));
   if (timed) VL_HIP(hipEventRecord(s->ev[s->ev_used], s->stream));
-  launch_pass_tasks<S, M>(s, s->stream, s->dtasks, nfar, b[0], po);
+  if (s->tb_overlap) launch_pass_tasks<S, M>(s, s->stream, s->dtasks, nfar, s->tb_joined_far, b[0], po);
   if (timed) {
     VL_HIP(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
     s->ev_used += 2;
@@ -499,26 +518,39 @@ static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 // Task list of K-step passes, from the node flags (0 interior, 1 boundary, 2 solid):
 //  * strips of up to tb_out(K, V) output columns (a multiple of V) start at x = x0,
 //    x0 + tb_out(K, V), ... (V columns per lane);
+//  * with `join` = W > 1, the columns from x0 first take as many joined strips as fit, each
+//    swept by a workgroup of W waves together (64 V W columns, of which tb_halo(K, V) on the
+//    left and K on the right are margin); then single-wave strips follow on the grid, the
+//    first cut short at the joined strips' end;
 //  * a row is usable for a strip when every node of the strip's columns plus K - 1 on each side
 //    is interior and its pulls stay inside the domain without wrapping (rows 1 .. ny-2,
 //    columns 1 .. nx-2);
 //  * a run of usable rows [ya, yb] gives output rows ya+K-1 .. yb-K+1 (the margin of the levels
-//    below K), cut into tasks of at most `rows` rows of near-equal size.
-// Tasks are ordered by (first row, first column) so a workgroup's waves sweep adjacent strips.
-// Every interior node no task covers goes to the single-step list; a covered node is a ring node
-// when one of its eight neighbours is a fluid node that no task covers, and a task marks each of
-// its four edges that holds a ring node.  Far tasks (below) come first; the caller sets the
-// lockstep bits, which depend on how it groups tasks into workgroups.
+//    below K), cut into tasks of at most `rows` rows of near-equal size;
+//  * the rows of a joined strip that its own runs leave out (where a body or wall crosses its
+//    wide footprint) go to single-wave strips over its columns (the grid's strips, cut at the
+//    joined strip's ends), with their own runs; so a joined plan covers every node the
+//    single-wave plan covers.
+// A joined task is W wave tasks in a row (bits 12 and 13 set, each wave's output columns and
+// its offset in its loaded columns in bits 16..23).  Single-wave tasks are ordered by (first
+// row, first column) so a workgroup's waves sweep adjacent strips; joined groups likewise among
+// themselves.  Every interior node no task covers goes to the single-step list; a covered node
+// is a ring node when one of its eight neighbours is a fluid node that no task covers, and a
+// task marks each of its four edges that holds a ring node.  Order: far joined groups, far
+// single-wave tasks, near joined groups, near single-wave tasks (far and near are launched as
+// separate ranges, so each range starts with its joined groups, whole workgroups from its first
+// task).  The caller sets the lockstep bits of the single-wave tasks, which depend on how it
+// groups tasks into workgroups.
 vl_status plan_passes(const std::vector<uint8_t>& fl, int nx, int ny, int64_t pitch, int K, int V, int rows,
-                      int x0_grid, PassPlan& plan) {
+                      int x0_grid, int join, PassPlan& plan) {
   if (nx >= 65536 || ny >= 65536 || pitch * ny >= ((int64_t)1 << 31) || rows < 1 || K < 2 || K > TB_MAX_K ||
-      (V != 1 && V != 2))
+      (V != 1 && V != 2) || join < 0 || join > 16)
     return set_error(VL_EINVAL, "K-step passes need nx, ny < 65536, a plane below 2^31 nodes and rows >= 1");
-  std::vector<int4>& tasks = plan.tasks;
-  tasks.clear();
+  std::vector<int4> tasks, groups;  // single-wave tasks; joined groups (join tasks each)
   std::vector<uint8_t> covered((size_t)nx * ny, 0);
-  std::vector<uint8_t> usable((size_t)ny);
+  std::vector<uint8_t> usable((size_t)ny), joined_row((size_t)ny);
   const int halo = tb_halo(K, V), out = tb_out(K, V);
+  const int gout = join > 1 ? (TB_LANES * V * join - halo - K) / V * V : 0;
   // Columns interior in at least half the rows: the strips' footprints stay within
   // [xa, xb], so that a column of boundary nodes at an inlet or outlet does not disqualify a
   // whole strip.  The strips are the aligned grid from tb_x0, a narrower first strip ending at
@@ -533,21 +565,30 @@ vl_status plan_passes(const std::vector<uint8_t>& fl, int nx, int ny, int64_t pi
     xa = std::max(xa, 1);
     xb = std::min(xb, nx - 2);
   }
-  std::vector<std::pair<int, int>> strips;  // (first output column, output columns)
+  // the next line of the single-wave grid (x0_grid + a multiple of tb_out) after column x
+  auto grid_end = [&](int x) { return x0_grid + ((x - x0_grid) / out + 1) * out; };
+  std::vector<std::tuple<int, int, bool>> strips;  // (first output column, output columns, joined)
   {
     int x_first = std::max(halo, xa + K - 1);
     x_first += (V - (x_first - halo) % V) % V;  // the strip's first column is a multiple of V
     if (x_first < x0_grid) {
       const int w = std::min({out, x0_grid - x_first, xb - (K - 1) - x_first + 1}) / V * V;
-      if (w >= 1) strips.emplace_back(x_first, w);
+      if (w >= 1) strips.emplace_back(x_first, w, false);
     }
-    for (int x0 = std::max(x0_grid, x_first); x0 < nx; x0 += out) {
-      const int w = std::min({out, nx - K - x0, xb - (K - 1) - x0 + 1}) / V * V;
+    int x0 = std::max(x0_grid, x_first);
+    if (gout > 0)
+      for (; x0 + gout <= std::min(nx - K, xb - (K - 1) + 1); x0 += gout) strips.emplace_back(x0, gout, true);
+    // single-wave strips on the grid from x0_grid (the first one ends at the next grid line)
+    while (x0 < nx) {
+      const int w = std::min({grid_end(x0) - x0, nx - K - x0, xb - (K - 1) - x0 + 1}) / V * V;
       if (w < 1) break;
-      strips.emplace_back(x0, w);
+      strips.emplace_back(x0, w, false);
+      x0 += w;
     }
   }
-  for (const auto& [x0, w] : strips) {
+  // The output row segments of a strip: usable runs less their margins, less the rows in
+  // `skip` (if any), cut into pieces of at most `rows`.
+  auto segments = [&](int x0, int w, const std::vector<uint8_t>* skip, auto&& emit) {
     for (int y = 0; y < ny; ++y) {
       bool ok = y >= 1 && y <= ny - 2;
       for (int x = x0 - (K - 1); ok && x <= x0 + w + K - 2; ++x) ok = fl[(size_t)y * nx + x] == 0;
@@ -560,20 +601,65 @@ vl_status plan_passes(const std::vector<uint8_t>& fl, int nx, int ny, int64_t pi
       }
       int yb = y;
       while (yb + 1 < ny && usable[(size_t)yb + 1]) ++yb;
-      const int first = y + K - 1, len = yb - y - 2 * K + 3;
-      if (len >= 1) {
-        const int nseg = (len + rows - 1) / rows;
-        for (int k = 0; k < nseg; ++k) {
-          const int a = first + (int)((int64_t)len * k / nseg), b = first + (int)((int64_t)len * (k + 1) / nseg);
-          tasks.push_back(make_int4(x0, a, b - a, w));
-          for (int yy = a; yy < b; ++yy) std::memset(&covered[(size_t)yy * nx + x0], 1, (size_t)w);
+      for (int a = y + K - 1, end = yb - K + 2; a < end;) {  // output rows [y + K - 1, yb - K + 1]
+        if (skip && (*skip)[(size_t)a]) {
+          ++a;
+          continue;
         }
+        int b = a;
+        while (b < end && !(skip && (*skip)[(size_t)b])) ++b;
+        const int len = b - a, nseg = (len + rows - 1) / rows;
+        for (int k = 0; k < nseg; ++k)
+          emit(a + (int)((int64_t)len * k / nseg), a + (int)((int64_t)len * (k + 1) / nseg));
+        a = b;
       }
       y = yb + 1;
+    }
+  };
+  auto add_task = [&](int x0, int ya, int yb, int w) {
+    tasks.push_back(make_int4(x0, ya, yb - ya, w));
+    for (int yy = ya; yy < yb; ++yy) std::memset(&covered[(size_t)yy * nx + x0], 1, (size_t)w);
+  };
+  for (const auto& [x0, w, joined] : strips) {
+    if (!joined) {
+      segments(x0, w, nullptr, [&](int ya, int yb) { add_task(x0, ya, yb, w); });
+      continue;
+    }
+    std::fill(joined_row.begin(), joined_row.end(), 0);
+    const int xs = x0 - halo;  // the group's first loaded column
+    segments(x0, w, nullptr, [&](int ya, int yb) {
+      for (int j = 0; j < join; ++j) {
+        const int lx = xs + TB_LANES * V * j;
+        const int ox0 = std::max(x0, lx), ox1 = std::min(x0 + w, lx + TB_LANES * V);
+        groups.push_back(make_int4(ox0, ya, yb - ya, (ox1 - ox0) | 3 << 12 | (ox0 - lx) << 16));
+        for (int yy = ya; yy < yb; ++yy) std::memset(&covered[(size_t)yy * nx + ox0], 1, (size_t)(ox1 - ox0));
+      }
+      std::fill(joined_row.begin() + ya, joined_row.begin() + yb, 1);
+    });
+    // the rows it leaves, in single-wave strips on the grid cut at its ends (so they cover at
+    // least what the grid's strips would)
+    for (int nx0 = x0; nx0 < x0 + w;) {
+      const int nw = std::min(grid_end(nx0), x0 + w) - nx0;
+      segments(nx0, nw, &joined_row, [&](int ya, int yb) { add_task(nx0, ya, yb, nw); });
+      nx0 += nw;
     }
   }
   std::sort(tasks.begin(), tasks.end(),
             [](const int4& a, const int4& b) { return a.y != b.y ? a.y < b.y : a.x < b.x; });
+  const int ngroups = join > 1 ? (int)(groups.size() / (size_t)join) : 0;
+  {
+    std::vector<int> order((size_t)ngroups);
+    for (int i = 0; i < ngroups; ++i) order[(size_t)i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+      const int4 &ta = groups[(size_t)a * join], &tb = groups[(size_t)b * join];
+      return ta.y != tb.y ? ta.y < tb.y : ta.x < tb.x;
+    });
+    std::vector<int4> sorted;
+    sorted.reserve(groups.size());
+    for (int i : order)
+      for (int j = 0; j < join; ++j) sorted.push_back(groups[(size_t)i * join + j]);
+    groups.swap(sorted);
+  }
   std::vector<uint32_t>& listed = plan.listed;
   listed.clear();
   std::vector<uint8_t> ring((size_t)nx * ny, 0);
@@ -595,18 +681,19 @@ vl_status plan_passes(const std::vector<uint8_t>& fl, int nx, int ny, int64_t pi
         }
       ring[k] = r;
     }
-  for (int4& tk : tasks) {
-    const int xa = tk.x, xb = tk.x + tk.w - 1, ya = tk.y, yb = tk.y + tk.z - 1;
-    int edges = 0;
-    for (int yy = ya; yy <= yb; ++yy)
-      for (int xx = xa; xx <= xb; ++xx) {
-        if (!ring[(size_t)yy * nx + xx]) continue;
-        if (yy != ya && yy != yb && xx != xa && xx != xb)
-          return set_error(VL_EINVAL, "internal: a ring node inside a pass task");
-        edges |= (yy == ya ? 1 : 0) | (yy == yb ? 2 : 0) | (xx == xa ? 4 : 0) | (xx == xb ? 8 : 0);
-      }
-    tk.w |= edges << 8;
-  }
+  for (std::vector<int4>* list : {&tasks, &groups})
+    for (int4& tk : *list) {
+      const int xa = tk.x, xb = tk.x + (tk.w & 0xff) - 1, ya = tk.y, yb = tk.y + tk.z - 1;
+      int edges = 0;
+      for (int yy = ya; yy <= yb; ++yy)
+        for (int xx = xa; xx <= xb; ++xx) {
+          if (!ring[(size_t)yy * nx + xx]) continue;
+          if (yy != ya && yy != yb && xx != xa && xx != xb)
+            return set_error(VL_EINVAL, "internal: a ring node inside a pass task");
+          edges |= (yy == ya ? 1 : 0) | (yy == yb ? 2 : 0) | (xx == xa ? 4 : 0) | (xx == xb ? 8 : 0);
+        }
+      tk.w |= edges << 8;
+    }
   // Far and near tasks.  A task is near when the region its outputs depend on (K nodes around
   // them) holds a fluid node that no task covers: its pass must wait for the single steps of
   // those nodes.  Far tasks depend on covered nodes only, so the next group's far pass can run
@@ -625,21 +712,37 @@ vl_status plan_passes(const std::vector<uint8_t>& fl, int nx, int ny, int64_t pi
     return unc[(size_t)(yb + 1) * (nx + 1) + xb + 1] - unc[(size_t)ya * (nx + 1) + xb + 1] -
            unc[(size_t)(yb + 1) * (nx + 1) + xa] + unc[(size_t)ya * (nx + 1) + xa];
   };
-  std::stable_partition(tasks.begin(), tasks.end(), [&](const int4& tk) {
+  auto is_far = [&](const int4& tk) {
     const int w = tk.w & 0xff;
     return uncovered_in(tk.x - K, tk.y - K, tk.x + w - 1 + K, tk.y + tk.z - 1 + K) == 0;
-  });
-  int nfar = 0;
+  };
+  std::stable_partition(tasks.begin(), tasks.end(), is_far);
+  // a joined group is far when all its waves are
+  std::vector<int4> gfar, gnear;
+  for (int g = 0; g < ngroups; ++g) {
+    bool far = true;
+    for (int j = 0; j < join; ++j) far = far && is_far(groups[(size_t)g * join + j]);
+    std::vector<int4>& dst = far ? gfar : gnear;
+    dst.insert(dst.end(), groups.begin() + (int64_t)g * join, groups.begin() + (int64_t)(g + 1) * join);
+  }
+  int nfar1 = 0;  // far single-wave tasks
+  while (nfar1 < (int)tasks.size() && is_far(tasks[(size_t)nfar1])) ++nfar1;
+  std::vector<int4>& all = plan.tasks;
+  all.clear();
+  all.insert(all.end(), gfar.begin(), gfar.end());
+  all.insert(all.end(), tasks.begin(), tasks.begin() + nfar1);
+  all.insert(all.end(), gnear.begin(), gnear.end());
+  all.insert(all.end(), tasks.begin() + nfar1, tasks.end());
+  const int nfar = (int)gfar.size() + nfar1;
   int64_t far_nodes = 0;
-  while (nfar < (int)tasks.size()) {
-    const int4& tk = tasks[(size_t)nfar];
-    const int w = tk.w & 0xff;
-    if (uncovered_in(tk.x - K, tk.y - K, tk.x + w - 1 + K, tk.y + tk.z - 1 + K) != 0) break;
+  for (int i = 0; i < nfar; ++i) {
+    const int4& tk = all[(size_t)i];
     if ((tk.w >> 8) & 15) return set_error(VL_EINVAL, "internal: a ring node in a far task");
-    far_nodes += (int64_t)w * tk.z;
-    ++nfar;
+    far_nodes += (int64_t)(tk.w & 0xff) * tk.z;
   }
   plan.nfar = nfar;
+  plan.njoined_far = (int)gfar.size();
+  plan.njoined_near = (int)gnear.size();
   plan.far_nodes = far_nodes;
   plan.covered = ncov;
   return VL_OK;
@@ -653,15 +756,22 @@ static bool lockstep_group_ok(const std::vector<int4>& tasks, const int32_t* own
   return true;
 }
 
+static bool is_joined(const int4& t) { return (t.w >> 13) & 1; }
+
 void mark_lockstep(std::vector<int4>& tasks, const int32_t* owner, const TaskRanges& ranges, int W) {
-  for (int4& t : tasks) t.w &= ~(1 << 12);
+  for (int4& t : tasks)
+    if (!is_joined(t)) t.w &= ~(1 << 12);
   for (const auto& [begin, end] : ranges)
-    for (int64_t g0 = begin; g0 + W <= end; g0 += W)
-      if (lockstep_group_ok(tasks, owner, g0, W))
+    for (int64_t g0 = begin; g0 + W <= end; g0 += W) {
+      bool any_joined = false;
+      for (int64_t i = g0; i < g0 + W; ++i) any_joined = any_joined || is_joined(tasks[(size_t)i]);
+      if (!any_joined && lockstep_group_ok(tasks, owner, g0, W))
         for (int64_t i = g0; i < g0 + W; ++i) tasks[(size_t)i].w |= 1 << 12;
+    }
 }
 
-vl_status check_lockstep(const std::vector<int4>& tasks, const int32_t* owner, const TaskRanges& ranges, int W) {
+vl_status check_lockstep(const std::vector<int4>& tasks, const int32_t* owner, const TaskRanges& ranges, int W,
+                         int wave_cols) {
   if (W < 1) return set_error(VL_EINVAL, "internal: lockstep workgroups of no waves");
   std::vector<uint8_t> launched(tasks.size(), 0);
   for (const auto& [begin, end] : ranges) {
@@ -669,14 +779,30 @@ vl_status check_lockstep(const std::vector<int4>& tasks, const int32_t* owner, c
       return set_error(VL_EINVAL, "internal: a pass launch range outside the task list");
     for (int64_t g0 = begin; g0 < end; g0 += W) {
       const int64_t g1 = std::min<int64_t>(g0 + W, end);
-      int marked = 0;
+      int marked = 0, joined = 0;
       for (int64_t i = g0; i < g1; ++i) {
         marked += (tasks[(size_t)i].w >> 12) & 1;
+        joined += is_joined(tasks[(size_t)i]);
         launched[(size_t)i] = 1;
       }
-      if (marked == 0) continue;
+      if (marked == 0 && joined == 0) continue;
       if (marked != W || g1 - g0 != W || !lockstep_group_ok(tasks, owner, g0, W))
         return set_error(VL_EINVAL, "internal: a lockstep workgroup whose waves would meet different barriers");
+      if (joined == 0) continue;
+      if (joined != W || wave_cols == 0)
+        return set_error(VL_EINVAL, "internal: joined pass tasks that do not fill a workgroup");
+      // wave i loads the wave_cols columns after wave i - 1's: first loaded column = first
+      // output column less the offset in bits 16..23 (wave_cols < 0: 64 or 128, the same for
+      // every wave of the group)
+      auto loaded = [](const int4& t) { return t.x - ((t.w >> 16) & 0xff); };
+      const int step = wave_cols > 0 ? wave_cols : loaded(tasks[(size_t)g0 + 1]) - loaded(tasks[(size_t)g0]);
+      if (step != TB_LANES && step != 2 * TB_LANES)
+        return set_error(VL_EINVAL, "internal: joined pass tasks whose waves are not adjacent strips");
+      for (int64_t i = g0 + 1; i < g1; ++i) {
+        const int4 &a = tasks[(size_t)i - 1], &b = tasks[(size_t)i];
+        if (loaded(b) - loaded(a) != step || b.x != a.x + (a.w & 0xff))
+          return set_error(VL_EINVAL, "internal: joined pass tasks whose waves are not adjacent strips");
+      }
     }
   }
   for (size_t i = 0; i < tasks.size(); ++i)
@@ -685,24 +811,23 @@ vl_status check_lockstep(const std::vector<int4>& tasks, const int32_t* owner, c
   return VL_OK;
 }
 
-// The launch ranges of a domain's task list: far tasks, then near tasks, when the far pass runs
-// on its own stream; otherwise the whole list in one launch.
-static TaskRanges pass_ranges(int64_t ntasks, int64_t nfar, bool overlap) {
-  if (!overlap) return {{0, ntasks}};
-  return {{0, nfar}, {nfar, ntasks}};
-}
+// The launch ranges of a domain's task list: far tasks, then near tasks (two launches, on two
+// streams when the far pass overlaps the single steps, else one after the other).
+static TaskRanges pass_ranges(int64_t ntasks, int64_t nfar) { return {{0, nfar}, {nfar, ntasks}}; }
 
 static vl_status build_passes(vl_sim* s, int rows) {
   PassPlan plan;
-  VL_TRY(plan_passes(s->tab.flag, s->g.nx, s->g.ny, s->g.pitch, s->tb_k, s->tb_v, rows, s->tb_x0, plan));
+  VL_TRY(plan_passes(s->tab.flag, s->g.nx, s->g.ny, s->g.pitch, s->tb_k, s->tb_v, rows, s->tb_x0,
+                     s->tb_join ? s->tb_waves : 0, plan));
   std::vector<int4>& tasks = plan.tasks;
   const int nfar = plan.nfar;
   s->tb_far_nodes = plan.far_nodes;
-  // Lockstep bit, over the ranges that launch_group launches (far and near tasks separately
-  // when they overlap, else the whole list).
-  const TaskRanges ranges = pass_ranges((int64_t)tasks.size(), nfar, s->tb_overlap);
+  s->tb_joined_far = plan.njoined_far;
+  s->tb_joined_near = plan.njoined_near;
+  // Lockstep bit, over the ranges that launch_group launches (far and near tasks separately).
+  const TaskRanges ranges = pass_ranges((int64_t)tasks.size(), nfar);
   if (s->tb_sync) mark_lockstep(tasks, nullptr, ranges, s->tb_waves);
-  VL_TRY(check_lockstep(tasks, nullptr, ranges, s->tb_waves));
+  VL_TRY(check_lockstep(tasks, nullptr, ranges, s->tb_waves, TB_LANES * s->tb_v));
   s->tb_rows = rows;
   s->ntasks = (int)tasks.size();
   s->ntasks_far = nfar;
@@ -876,6 +1001,12 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
     if (const char* e = std::getenv("VL_TB_WAVES")) s->tb_waves = std::atoi(e) == 4 ? 4 : 8;
     if (const char* e = std::getenv("VL_TB_SYNC")) s->tb_sync = e[0] != '0';
     if (const char* e = std::getenv("VL_TB_OVERLAP")) s->tb_overlap = e[0] != '0';
+    // Joined strips (VL_TB_JOIN=1): a workgroup's waves sweep one strip and hand each other
+    // their edge values through LDS, one barrier a level, instead of each recomputing a halo.
+    // Bit-identical, but slower on MI355X at every layout measured (8192 x 4096 FP32 far pass
+    // 835 against 743 us, FP64 1459 against 1298, FP16 16384 x 8192 2044 against 1760): the
+    // barriers stall more than the halo costs.  Off by default.
+    if (const char* e = std::getenv("VL_TB_JOIN")) s->tb_join = e[0] != '0';
     s->tb_v = precision == VL_F16 ? 2 : 1;
     if (const char* e = std::getenv("VL_TB_V"))
       if (precision != VL_F64) s->tb_v = std::atoi(e) == 2 ? 2 : 1;
@@ -1146,25 +1277,27 @@ vl_status vl_sim_create(vl_builder* b, int precision, vl_sim** out) {
 void vl_sim_destroy(vl_sim* s) { free_sim(s); }
 
 vl_status vl_plan_passes(const uint8_t* flags, int nx, int ny, int k, int v, int rows, int x0, int waves,
-                         int overlap, int32_t* tasks, int64_t max_tasks, uint32_t* listed, int64_t max_listed,
+                         int join, int32_t* tasks, int64_t max_tasks, uint32_t* listed, int64_t max_listed,
                          int64_t* counts) {
   if (!flags || !counts || nx < 3 || ny < 3 || k < 2 || k > TB_MAX_K || (v != 1 && v != 2) || x0 < tb_halo(k, v) ||
-      (x0 - tb_halo(k, v)) % v != 0 || waves < 0)
+      (x0 - tb_halo(k, v)) % v != 0 || waves < 0 || waves > 16 || (join && waves < 2))
     return set_error(VL_EINVAL,
                      "vl_plan_passes: flags, counts, nx, ny >= 3, k in 2..5, v in 1..2, x0 = halo + a multiple of v, "
-                     "waves >= 0");
+                     "waves in 0..16 (at least 2 with join)");
   PassPlan plan;
   std::vector<uint8_t> fl(flags, flags + (size_t)nx * ny);
-  VL_TRY(plan_passes(fl, nx, ny, round_up(nx, 64), k, v, rows, x0, plan));
+  VL_TRY(plan_passes(fl, nx, ny, round_up(nx, 64), k, v, rows, x0, join ? waves : 0, plan));
   if (waves > 0) {  // the lockstep bits as vl_sim_create sets them
-    const TaskRanges ranges = pass_ranges((int64_t)plan.tasks.size(), plan.nfar, overlap != 0);
+    const TaskRanges ranges = pass_ranges((int64_t)plan.tasks.size(), plan.nfar);
     mark_lockstep(plan.tasks, nullptr, ranges, waves);
-    VL_TRY(check_lockstep(plan.tasks, nullptr, ranges, waves));
+    VL_TRY(check_lockstep(plan.tasks, nullptr, ranges, waves, TB_LANES * v));
   }
   counts[0] = (int64_t)plan.tasks.size();
   counts[1] = plan.nfar;
   counts[2] = (int64_t)plan.listed.size();
   counts[3] = plan.covered;
+  counts[4] = plan.njoined_far;
+  counts[5] = plan.njoined_near;
   if (tasks)
     for (int64_t i = 0; i < std::min<int64_t>(max_tasks, counts[0]); ++i) {
       const int4& t = plan.tasks[(size_t)i];
@@ -1193,7 +1326,7 @@ vl_status vl_pass_lockstep(int32_t* tasks, const int32_t* owner, int64_t ntasks,
     mark_lockstep(t, owner, r, waves);
     for (int64_t i = 0; i < ntasks; ++i) tasks[4 * i + 3] = t[(size_t)i].w;
   }
-  return check_lockstep(t, owner, r, waves);
+  return check_lockstep(t, owner, r, waves, -1);
 }
 
 vl_status vl_sim_info_get(vl_sim* s, vl_sim_info* o) {
@@ -1219,6 +1352,7 @@ vl_status vl_sim_info_get(vl_sim* s, vl_sim_info* o) {
   o->pass_tasks = s->ntasks;
   o->listed_nodes = s->nlisted;
   o->pass_rows = s->tb_rows;
+  o->pass_joined_tasks = s->tb_joined_far + s->tb_joined_near;
   o->pass_far_nodes = s->tb_far_nodes;
   o->pass_far_tasks = s->tb_overlap ? s->ntasks_far : 0;
   return VL_OK;

@@ -64,12 +64,14 @@ struct PassPlan {
   std::vector<int4> tasks;
   std::vector<uint32_t> listed;
   int nfar = 0;
+  int njoined_far = 0, njoined_near = 0;  // tasks of joined groups at the start of each range
   int64_t covered = 0, far_nodes = 0;
 };
 // flags: nx * ny bytes, 0 interior, 1 boundary, 2 solid; K steps per pass, V columns per lane,
-// tasks of at most `rows` output rows, strips aligned from column x0_grid.
+// tasks of at most `rows` output rows, strips aligned from column x0_grid; `join` = W > 1
+// plans joined strips for workgroups of W waves (0: single-wave strips only).
 vl_status plan_passes(const std::vector<uint8_t>& flags, int nx, int ny, int64_t pitch, int K, int V, int rows,
-                      int x0_grid, PassPlan& plan);
+                      int x0_grid, int join, PassPlan& plan);
 
 // Lockstep bit (1 << 12 in a task's w) of a pass task list.  The list is launched as one or
 // more ranges [begin, end), each as workgroups of W consecutive tasks from `begin` (the last one
@@ -77,10 +79,14 @@ vl_status plan_passes(const std::vector<uint8_t>& flags, int nx, int ny, int64_t
 // must only be set where every wave of the workgroup executes the same barriers:
 // mark_lockstep sets it on exactly the full workgroups of some range whose W tasks have equal
 // first row and row count (and, with `owner`, belong to one case), and clears it elsewhere;
-// check_lockstep verifies that property for every range and returns VL_EINVAL otherwise.
+// it leaves joined tasks (bit 13, set by the planner with bit 12) as they are.  check_lockstep
+// verifies that property for every range and returns VL_EINVAL otherwise; for joined tasks it
+// also requires that they fill whole workgroups whose waves load adjacent runs of `wave_cols`
+// columns (64 V; 0: no joined task allowed; -1: 64 or 128).
 using TaskRanges = std::vector<std::pair<int64_t, int64_t>>;
 void mark_lockstep(std::vector<int4>& tasks, const int32_t* owner, const TaskRanges& ranges, int W);
-vl_status check_lockstep(const std::vector<int4>& tasks, const int32_t* owner, const TaskRanges& ranges, int W);
+vl_status check_lockstep(const std::vector<int4>& tasks, const int32_t* owner, const TaskRanges& ranges, int W,
+                         int wave_cols = 0);
 
 }  // namespace vl
 
@@ -118,6 +124,8 @@ struct vl_sim {
   int ntasks = 0;                    // one task per wave: far tasks first, then near ones
   int ntasks_far = 0;
   int64_t tb_far_nodes = 0;          // covered nodes of the far tasks
+  bool tb_join = false;              // joined strips (a workgroup's waves sweep one strip)
+  int tb_joined_far = 0, tb_joined_near = 0;  // joined tasks at the start of the far / near range
   // Overlap of a group's single steps with the next group's far pass (side stream): whether the
   // side stream has been forked since the last join, and the parity of the group's events.
   bool tb_overlap = true;

@@ -618,7 +618,11 @@ __host__ __device__ constexpr int tb_out(int k, int v) { return (TB_LANES * v - 
 // that point out of the task across that edge, the ones its uncovered neighbours pull (values
 // the single steps would also produce, so the extra ones are harmless).  Lockstep: the workgroup's tasks
 // cover the same rows, and its waves sweep in step (one barrier a row), so that together they
-// read and write whole lines.
+// read and write whole lines.  Joined (bit 13): the workgroup's W waves sweep one strip of
+// 64 V W columns together, wave j loading the 64 V columns from the strip's first plus 64 V j,
+// and they hand each other the x-neighbours of their edge lanes through LDS at every level
+// (one barrier a level), so no wave recomputes a halo of its own; bits 16..23 then hold the
+// wave's first output column less its first loaded column.
 template <class C>
 __device__ inline C from_left(C v) {  // lane l receives lane l - 1's value
   return __shfl_up(v, 1, TB_LANES);
@@ -737,11 +741,22 @@ struct Rows3 {
   }
 };
 
-template <class S, int M, int V, int D, int K>
+// Edge values a joined wave hands its neighbours at one level: lane 63's x + 1 pulls (f1, f5,
+// f8 of its last column) for the wave on its right, lane 0's x - 1 pulls (f3, f6, f7 of its
+// first column) for the wave on its left.  Two sets, used in turn, so that a wave writing the
+// next level's values never meets a neighbour still reading this level's.
+constexpr int TB_XCH = 6;
+
+// JOIN: the task is joined (bit 13); then xch holds 2 nw TB_XCH values of LDS, wid is the wave's
+// index in the workgroup and nw the workgroup's waves.  The two forms are separate code paths
+// (each with its own register allocation).
+template <class S, int M, int V, int D, int K, bool JOIN = false>
 __device__ __attribute__((always_inline)) inline void multi_step_strip(const int4 task,
                                                                        const typename S::T* __restrict__ A,
                                                                        const PassOut<typename S::T> po,
-                                                                       const Geom& g, const Coll& c) {
+                                                                       const Geom& g, const Coll& c,
+                                                                       typename S::C* xch = nullptr, int wid = 0,
+                                                                       int nw = 1) {
   using T = typename S::T;
   using C = typename S::C;
   using PT = typename Pack<T, V>::type;
@@ -750,7 +765,7 @@ __device__ __attribute__((always_inline)) inline void multi_step_strip(const int
   const int x0 = task.x, y0 = task.y, rows = task.z, w = task.w & 0xff;
   const int edges = (task.w >> 8) & 15;
   const bool lockstep = (task.w >> 12) & 1;
-  const int xs = x0 - tb_halo(K, V);                   // first column of the strip (a multiple of V)
+  const int xs = x0 - (JOIN ? (task.w >> 16) & 0xff : tb_halo(K, V));  // first column (a multiple of V)
   const int xc = xs + V * lane;                        // this lane's first column
   const int xl = min(xc, g.pitch - V);                 // clamped for loads (lanes past the row)
   const bool out_lane = xc >= x0 && xc + V <= x0 + w;  // all V columns are output columns
@@ -784,6 +799,36 @@ __device__ __attribute__((always_inline)) inline void multi_step_strip(const int
     ring_l[j] = out_lane && (edges & 4) && xc + j == x0;
     ring_r[j] = out_lane && (edges & 8) && xc + j == x0 + w - 1;
   }
+  // Joined waves: the newest row's shifted populations (fr) take their edge lanes' neighbours
+  // from the adjacent waves (u: the row before the shift).
+  int xi = 0;  // exchanges so far, choosing the LDS set
+  const bool take_l = lane == 0 && wid > 0, take_r = lane == TB_LANES - 1 && wid + 1 < nw;
+  const int wl = wid > 0 ? wid - 1 : wid, wr = wid + 1 < nw ? wid + 1 : wid;  // the neighbours (or itself)
+  auto edge_swap = [&](C (&fr)[Q][V], const C (&u1)[V], const C (&u5)[V], const C (&u8)[V], const C (&u3)[V],
+                       const C (&u6)[V], const C (&u7)[V]) __attribute__((always_inline)) {
+    C* set = xch + (xi & 1) * nw * TB_XCH;
+    if (lane == TB_LANES - 1) {
+      set[wid * TB_XCH + 0] = u1[V - 1];
+      set[wid * TB_XCH + 1] = u5[V - 1];
+      set[wid * TB_XCH + 2] = u8[V - 1];
+    }
+    if (lane == 0) {
+      set[wid * TB_XCH + 3] = u3[0];
+      set[wid * TB_XCH + 4] = u6[0];
+      set[wid * TB_XCH + 5] = u7[0];
+    }
+    __syncthreads();
+    // every lane reads the neighbours' values (one broadcast read each); the edge lanes take them
+    const C l1 = set[wl * TB_XCH + 0], l5 = set[wl * TB_XCH + 1], l8 = set[wl * TB_XCH + 2];
+    const C r3 = set[wr * TB_XCH + 3], r6 = set[wr * TB_XCH + 4], r7 = set[wr * TB_XCH + 5];
+    fr[1][0] = take_l ? l1 : fr[1][0];
+    fr[5][0] = take_l ? l5 : fr[5][0];
+    fr[8][0] = take_l ? l8 : fr[8][0];
+    fr[3][V - 1] = take_r ? r3 : fr[3][V - 1];
+    fr[6][V - 1] = take_r ? r6 : fr[6][V - 1];
+    fr[7][V - 1] = take_r ? r7 : fr[7][V - 1];
+    ++xi;
+  };
   // Iteration k consumes row r = y0 - K + k: first the row that arrived, then the load of row
   // k + D, then the compute and its stores.  The memory counter that the wait for a load uses
   // also counts stores, and the compiler waits for all of them whenever the accesses in flight
@@ -792,7 +837,7 @@ __device__ __attribute__((always_inline)) inline void multi_step_strip(const int
   // exist); the loop runs in groups of D iterations, and the iterations past the last one keep
   // nothing.
   auto iteration = [&](const int k, PT (&cur)[Q]) __attribute__((always_inline)) {
-    if (lockstep) __syncthreads();
+    if (!JOIN && lockstep) __syncthreads();
     C fr[Q][V];  // the newest row of the level being consumed, x-shifted as the pull needs it
     {
       C u1[V], u3[V], u5[V], u6[V], u7[V], u8[V];
@@ -814,6 +859,7 @@ __device__ __attribute__((always_inline)) inline void multi_step_strip(const int
       right_of(u3, fr[3]);
       right_of(u6, fr[6]);
       right_of(u7, fr[7]);
+      if constexpr (JOIN) edge_swap(fr, u1, u5, u8, u3, u6, u7);
     }
     __builtin_amdgcn_sched_barrier(0);
     {
@@ -896,6 +942,7 @@ __device__ __attribute__((always_inline)) inline void multi_step_strip(const int
         right_of(t3, fr[3]);
         right_of(t6, fr[6]);
         right_of(t7, fr[7]);
+        if constexpr (JOIN) edge_swap(fr, t1, t5, t8, t3, t6, t7);
       }
     }
   };
