@@ -980,9 +980,10 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
   }
   // K-step passes on split-mode domains (VL_PASS_STEPS=1..4, 5 for FP32, overrides; above 1 it
   // also forces split mode).
-  // Steps per pass and rows per task, measured on MI355X at 8192 x 4096: FP32 5 steps, 20 rows;
-  // FP64 4 steps, 16 rows (5 would need more registers than its occupancy allows); FP16 3
-  // steps, 24 rows, 2 columns per lane (its conversions make 4 steps slower).
+  // Steps per pass and rows per task, measured on MI355X (8192 x 4096, FP16 16384 x 8192):
+  // FP32 5 steps, 20 rows (16-40 rows: 1-6 % slower); FP64 4 steps, 24 rows (16: 0.5 % slower;
+  // 5 steps would need more registers than its occupancy allows); FP16 3 steps, 40 rows (24:
+  // 3 % slower), 2 columns per lane (its conversions make 4 steps slower).
   int k_steps = s->split ? (precision == VL_F16 ? 3 : (precision == VL_F32 ? 5 : 4)) : 1;
   if (const char* e = std::getenv("VL_PASS_STEPS")) {
     k_steps = std::min(std::max(std::atoi(e), 1), precision == VL_F32 ? TB_MAX_K : 4);
@@ -994,7 +995,7 @@ static vl_status create(vl::Domain& d, int precision, vl_sim** out) {
     // grid from x = 32.
     // VL_TB_ROWS, VL_TB_WAVES (4 | 8), VL_TB_SYNC (0 | 1), VL_TB_X0 (>= the halo) and VL_TB_V
     // (columns per lane, 1 | 2; FP32 / FP16 only) override.
-    int rows = precision == VL_F16 ? 24 : (precision == VL_F32 ? 20 : 16);
+    int rows = precision == VL_F16 ? 40 : (precision == VL_F32 ? 20 : 24);
     if (const char* e = std::getenv("VL_TB_ROWS")) rows = std::atoi(e);
     // FP64 needs about 170 VGPRs: 4-wave workgroups fit 3 to a CU, 8-wave ones only 1
     s->tb_waves = precision == VL_F64 ? 4 : 8;
