@@ -190,6 +190,37 @@ def test_batch_passes_equal_single_case_runs(gpu, monkeypatch, prec, k, rows):
         ch.close()
 
 
+@pytest.mark.parametrize("env", [{}, {"VL_BATCH_REGION": "1"}, {"VL_BATCH_REGION": "1", "VL_BATCH_OVERLAP": "0"},
+                                 {"VL_BATCH_REGION": "1", "VL_BATCH_OVERLAP": "0", "VL_BATCH_REGION0": "after"},
+                                 {"VL_BATCH_REGION": "1", "VL_BATCH_REGION_T": "256"},
+                                 {"VL_BATCH_REGION": "1", "VL_BATCH_REGION_T": "1024"},
+                                 {"VL_BATCH_REGION": "1", "VL_BATCH_SERIAL": "1"},
+                                 {"VL_BATCH_REGION": "1", "VL_GRAPHS": "0"}])
+@pytest.mark.parametrize("prec", [api.F32, api.F64])
+def test_batch_group_schedules_equal_single_case_runs(gpu, monkeypatch, env, prec):
+    """Every schedule of a group of K steps gives the bits of the same case run alone: the
+    boundary and listed kernels once per step (the default), and the per-case region kernel
+    (VL_BATCH_REGION=1) beside the next group's far pass or after the whole pass
+    (VL_BATCH_OVERLAP=0, its step 0 beside the pass or after it), at 256, 512 and 1024 threads,
+    on one stream, and without graphs."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    cases = wide_cases(seed=41)
+    print(f"[batch] schedule {env} prec={prec} steps={[c.steps for c in cases]}", flush=True)
+    b = run_batch(gpu, cases, prec)
+    assert b.info().pass_steps == (5 if prec == api.F32 else 3)
+    for k in env:
+        monkeypatch.delenv(k)
+    for j, c in enumerate(cases):
+        coef, done, status = b.result(j)
+        assert status == 0 and done == c.steps
+        ch = api.Channel(gpu, c.spec, prec)
+        ref = ch.run(c.steps, c.d)
+        assert np.array_equal(coef, ref), f"case {j}: coefficient history differs"
+        assert np.array_equal(b.read_case(j, c.spec.nx, c.spec.ny), ch.sim.read()), f"case {j}: populations differ"
+        ch.close()
+
+
 def test_batch_passes_edge_cases(gpu, monkeypatch):
     """5-step passes with a case too small for any pass task (all its interior listed), cases of
     3 and 5 steps (fewer than, and exactly, one group) and a case of a few groups plus a

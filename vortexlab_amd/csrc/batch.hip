@@ -59,6 +59,11 @@ __device__ inline double ramp_dev(uint64_t t, uint64_t steps) {  // src/cases/mo
   return s * s * (3.0 - 2.0 * s);
 }
 
+// Boundary tiles (256 boundary nodes of one case each) of one step: the tiles of the cases still
+// running, [first, act[t]), where first = 0 in single steps; in step `jpos` of a group of K steps
+// that starts at t - jpos, first = first_at[t - jpos] skips the tiles of the cases whose boundary
+// nodes the region kernel advances.  The first tile of a case also reduces the previous step's
+// force terms into its history.
 template <class S>
 __global__ __launch_bounds__(BT) void k_batch_boundary(const typename S::T* __restrict__ A,
                                                        typename S::T* __restrict__ B,
@@ -68,13 +73,15 @@ __global__ __launch_bounds__(BT) void k_batch_boundary(const typename S::T* __re
                                                        const int32_t* __restrict__ act, const int64_t* tbase,
                                                        int t_local, const double* __restrict__ kick_tab,
                                                        double* __restrict__ hist, int32_t* __restrict__ status,
-                                                       int64_t* __restrict__ done, double sqrt_cs2) {
+                                                       int64_t* __restrict__ done, double sqrt_cs2,
+                                                       const int32_t* __restrict__ first_at, int jpos) {
   __shared__ double red[BT];
   __shared__ double fsum[2];
   __shared__ int skip;
   const int64_t t = *tbase + t_local;
   const int nact = act[t];
-  for (int i = blockIdx.x; i < nact; i += gridDim.x) {
+  const int first = first_at ? first_at[t - jpos] : 0;
+  for (int i = first + (int)blockIdx.x; i < nact; i += gridDim.x) {
     const int ci = tcase[i];
     if (threadIdx.x == 0) skip = status[ci];
     __syncthreads();
@@ -240,6 +247,97 @@ __global__ __launch_bounds__(BT) void k_batch_listed(const typename S::T* __rest
   }
 }
 
+// The K + 1 population pools of a group of K steps, in time order (pool j holds time t + j).
+template <class T>
+struct PoolSet {
+  T* p[TB_MAX_K + 1];
+};
+
+// Steps j0 .. j1 - 1 of a group of K steps from t, for the boundary nodes and the listed interior
+// nodes of the cases that run the whole group in K-step passes: one workgroup per case (the
+// cases with steps >= t + K, rcase[0 .. nreg_at[t])), a barrier between steps, so the steps of a
+// case need no launch each.  Step j reads pool j and writes pool j + 1; from j = 1 on it reads
+// the ring values of level j that the pass writes, so such a launch follows the pass.  Per node
+// the arithmetic is the boundary and listed kernels', and the per-step work of the boundary
+// kernel's first tile (the previous step's force reduction, the history, the divergence check,
+// the controls and the outlet controller) happens once per step with the same fixed-order
+// trees, so a case gives the same bits as in single steps.
+template <class S, int RT>
+__global__ __launch_bounds__(RT) void k_batch_region(PoolSet<typename S::T> ps, const BatchCase* __restrict__ cases,
+                                                     const int32_t* __restrict__ rcase,
+                                                     const int32_t* __restrict__ nreg_at,
+                                                     const int32_t* __restrict__ clfirst,
+                                                     const int32_t* __restrict__ clcount,
+                                                     const uint32_t* __restrict__ lnodes, const int64_t* tbase,
+                                                     int t_local, int j0, int j1, const double* __restrict__ kick_tab,
+                                                     double* __restrict__ hist, int32_t* __restrict__ status,
+                                                     int64_t* __restrict__ done, double sqrt_cs2) {
+  __shared__ double red[STEP_T];
+  __shared__ double fsum[2];
+  __shared__ int skip;
+  const int64_t t = *tbase + t_local;  // the group's first step
+  const int n = nreg_at[t];
+  for (int r = blockIdx.x; r < n; r += gridDim.x) {
+    const int ci = rcase[r];
+    const BatchCase& cs = cases[ci];
+    const int64_t ns2 = (int64_t)max(cs.t.nslots, 1) * 2;
+    const int64_t np1 = max(cs.t.np, 1);
+    const uint32_t* ln = lnodes + clfirst[ci];
+    const int nln = clcount[ci];
+    for (int j = j0; j < j1; ++j) {
+      const int64_t tj = t + j;  // < cs.steps: the case runs the whole group
+      if (threadIdx.x == 0) skip = status[ci];
+      __syncthreads();
+      const int sk = skip;
+      __syncthreads();
+      if (sk) break;
+      // force terms of step tj - 1 -> coefficient history (and the divergence check)
+      if (tj >= 1) {
+        double cd = 0.0, cl = 0.0;
+        if (cs.t.ngroups > 0) {
+          reduce_forces<RT>(cs.t.fcontrib + ((tj + 1) & 1) * ns2, cs.t.gfirst, 1, fsum, red);
+          cd = cs.coef_k * fsum[0];
+          cl = cs.coef_k * fsum[1];
+        }
+        if (threadIdx.x == 0) {
+          hist[cs.hist_off + 2 * (tj - 1)] = cd;
+          hist[cs.hist_off + 2 * (tj - 1) + 1] = cl;
+          done[ci] = tj;
+          if (!isfinite(cd)) status[ci] = 1;
+        }
+      }
+      StepCtl ctl;
+      const double rmp = ramp_dev((uint64_t)tj, cs.ramp_steps);
+      ctl.vscale[0] = 1.0;
+      ctl.vscale[1] = rmp;
+      ctl.vscale[2] = (tj >= cs.kick_start && tj < cs.kick_start + cs.kick_dur) ? kick_tab[cs.kick_off + tj - cs.kick_start]
+                                                                                : 0.0;
+      ctl.vscale[3] = 1.0;
+      ctl.target = cs.u_ref * rmp;
+      ctl.has_target = cs.absorbing && cs.t.np > 0;
+      double drho = 0.0;
+      if (ctl.has_target) drho = outlet_offset<S, RT>(cs.t.pvel + (tj & 1) * np1, cs.t.np, ctl.target, sqrt_cs2, red);
+      if (threadIdx.x == 0) *cs.t.drho = drho;
+      const typename S::T* a = ps.p[j] + cs.pop_off;
+      typename S::T* bb = ps.p[j + 1] + cs.pop_off;
+      double* fc = cs.t.fcontrib + (tj & 1) * ns2;
+      double* pvn = cs.t.pvel + ((tj + 1) & 1) * np1;
+      // (only the first npb boundary nodes have links that use drho: the outlet-controller
+      // nodes come first)
+      if (cs.c.model == MODEL_BGK) {
+        for (int b = (int)threadIdx.x; b < cs.t.nb; b += RT)
+          boundary_node<S, MODEL_BGK>(a, bb, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
+        for (int k = (int)threadIdx.x; k < nln; k += RT) listed_node<S, MODEL_BGK>(k, nln, ln, a, bb, cs.g, cs.c);
+      } else {
+        for (int b = (int)threadIdx.x; b < cs.t.nb; b += RT)
+          boundary_node<S, MODEL_TRT>(a, bb, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
+        for (int k = (int)threadIdx.x; k < nln; k += RT) listed_node<S, MODEL_TRT>(k, nln, ln, a, bb, cs.g, cs.c);
+      }
+      __syncthreads();  // step tj written before step tj + 1 reads it
+    }
+  }
+}
+
 // Initial state of every case: rest, or the inflow profile when the case has no ramp
 // (Channel::new, src/cases/channel.rs:126-128); non-fluid cells at rest (src/sim.rs:362).
 template <class S>
@@ -321,7 +419,7 @@ struct vl_batch {
   double setup_ms[4] = {};      // set-up time: cases and tables, pass plans, packing and uploads, autotuning
   // device
   hipStream_t stream = nullptr, side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_mid = nullptr;
   // population pools: 2, or K + 1 with K-step passes (pool j + 1 follows pool j in time)
   void* pool[TB_MAX_K + 1] = {};
   int nbuf = 2;
@@ -340,13 +438,28 @@ struct vl_batch {
     int32_t* n_at = nullptr;
     int n = 0;
     std::vector<int32_t> h_n_at;
-  } tasks;
+  } tasks, tasks_far, tasks_near;
+  // with the region kernel: the region kernel of a group runs beside the next group's far pass
+  // (VL_BATCH_OVERLAP=0: after the group's whole pass, before the next group)
+  bool overlap = true;
+  bool ov_open = false;  // the side stream runs ahead of the main stream (joined by join_side)
   int32_t *d_kfirst_at = nullptr;
   int32_t *d_lcase = nullptr, *d_lfirst = nullptr, *d_lcount = nullptr, *d_ln_at = nullptr;
   uint32_t* d_lnodes = nullptr;
   // per group start t: listed tiles of the cases with steps >= t + K, and the bulk tiles of
   // those cases (the bulk kernel skips them in a group)
   std::vector<int32_t> h_ln_at, h_kfirst_at;
+  // Region kernel (VL_BATCH_REGION=1, with K-step passes; by default the boundary and listed
+  // kernels run once per step): one workgroup of region_threads per case in passes advances its
+  // boundary and listed nodes through the group's steps.  rcase = the cases in descending step
+  // order; per group start t, rn_at[t] = the cases with steps >= t + K, bfirst_at[t] = their
+  // boundary tiles (a prefix, which the boundary kernel then skips).  clfirst / clcount: each
+  // case's listed nodes in d_lnodes.  region0_side: step 0 of the region kernel beside the pass.
+  bool region = false, region0_side = true;
+  int region_threads = 512;
+  int32_t *d_rcase = nullptr, *d_rn_at = nullptr, *d_bfirst_at = nullptr, *d_clfirst = nullptr,
+          *d_clcount = nullptr;
+  std::vector<int32_t> h_rn_at, h_bfirst_at, h_bact;
   uint8_t* dflags = nullptr;
   BatchCase* dcases = nullptr;
   std::vector<void*> allocs;  // table pools etc.
@@ -367,8 +480,8 @@ struct vl_batch {
   std::vector<int32_t> ladder;  // grid sizes used when bulk_mult == 0
   int cur = 0;
   int64_t t = 0;
-  // (first pool, bulk grid, pass grid, listed grid) -> window
-  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+  // (first pool, bulk grid, pass grid, listed or region grid, boundary grid, near pass grid) -> window
+  std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> graphs;
   bool use_graphs = true;
   double autotune_ms[3][4] = {};
   double last_run_seconds = 0.0;
@@ -472,7 +585,7 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, int 
   if (ev) VL_HIP(hipEventRecord(ev->bs, bs));
   hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, bs, A, B, b->dcases,
                      b->d_bcase, b->d_bb0, b->d_bact, tbase, t_local, b->d_kick, b->d_hist, b->d_status, b->d_done,
-                     SQRT_CS2_B);
+                     SQRT_CS2_B, (const int32_t*)nullptr, 0);
   if (ev) VL_HIP(hipEventRecord(ev->be, bs));
   if (!b->serial) VL_HIP(hipEventRecord(b->ev_join, b->side));
   if (ev) VL_HIP(hipEventRecord(ev->ks, b->stream));
@@ -488,15 +601,16 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, int 
   return VL_OK;
 }
 
-// Grids of a group of K steps (workgroups): the pass, the listed tiles, the bulk tiles.
+// Grids of a group of K steps (workgroups): the pass, the listed tiles (or the region kernel's
+// cases), the bulk tiles, and with the region kernel the boundary tiles of the cases that end
+// inside the group.
 struct GroupGrids {
-  int pass, listed, bulk;
+  int pass, listed, bulk, bnd, pass_near;
 };
 
 template <class S, int W, int V, int D, int K>
-void launch_batch_pass(vl_batch* b, hipStream_t st, const typename S::T* A, PassOut<typename S::T> po,
-                       const int64_t* tbase, int t_local, int grid) {
-  const vl_batch::TaskList& tl = b->tasks;
+void launch_batch_pass(vl_batch* b, const vl_batch::TaskList& tl, hipStream_t st, const typename S::T* A,
+                       PassOut<typename S::T> po, const int64_t* tbase, int t_local, int grid) {
   hipLaunchKernelGGL((k_batch_pass<S, W, V, D, K>), dim3((unsigned)grid), dim3(W * TB_LANES), 0, st, A, po,
                      tl.task, tl.tcase, tl.n_at, tbase, t_local, b->dcases, b->d_status);
 }
@@ -504,24 +618,24 @@ void launch_batch_pass(vl_batch* b, hipStream_t st, const typename S::T* A, Pass
 // The pass kernel of the batch's K and precision: FP32 one column per lane (K = 2..5), FP16
 // two (K = 2..4), FP64 one, in 4-wave workgroups (K = 2..4); 8 waves per workgroup otherwise.
 template <class S>
-vl_status launch_pass_k(vl_batch* b, hipStream_t st, const typename S::T* A, PassOut<typename S::T> po,
-                        const int64_t* tbase, int t_local, int grid) {
+vl_status launch_pass_k(vl_batch* b, const vl_batch::TaskList& tl, hipStream_t st, const typename S::T* A,
+                        PassOut<typename S::T> po, const int64_t* tbase, int t_local, int grid) {
   const int K = b->pass_k;
   if constexpr (std::is_same<S, StoreF16>::value) {
-    if (K == 2) launch_batch_pass<S, 8, 2, 2, 2>(b, st, A, po, tbase, t_local, grid);
-    else if (K == 3) launch_batch_pass<S, 8, 2, 2, 3>(b, st, A, po, tbase, t_local, grid);
-    else if (K == 4) launch_batch_pass<S, 8, 2, 2, 4>(b, st, A, po, tbase, t_local, grid);
+    if (K == 2) launch_batch_pass<S, 8, 2, 2, 2>(b, tl, st, A, po, tbase, t_local, grid);
+    else if (K == 3) launch_batch_pass<S, 8, 2, 2, 3>(b, tl, st, A, po, tbase, t_local, grid);
+    else if (K == 4) launch_batch_pass<S, 8, 2, 2, 4>(b, tl, st, A, po, tbase, t_local, grid);
     else return set_error(VL_EINVAL, "FP16 batch passes take 2..4 steps");
   } else if constexpr (std::is_same<S, StoreF64>::value) {
-    if (K == 2) launch_batch_pass<S, 4, 1, 3, 2>(b, st, A, po, tbase, t_local, grid);
-    else if (K == 3) launch_batch_pass<S, 4, 1, 3, 3>(b, st, A, po, tbase, t_local, grid);
-    else if (K == 4) launch_batch_pass<S, 4, 1, 3, 4>(b, st, A, po, tbase, t_local, grid);
+    if (K == 2) launch_batch_pass<S, 4, 1, 3, 2>(b, tl, st, A, po, tbase, t_local, grid);
+    else if (K == 3) launch_batch_pass<S, 4, 1, 3, 3>(b, tl, st, A, po, tbase, t_local, grid);
+    else if (K == 4) launch_batch_pass<S, 4, 1, 3, 4>(b, tl, st, A, po, tbase, t_local, grid);
     else return set_error(VL_EINVAL, "FP64 batch passes take 2..4 steps");
   } else {
-    if (K == 2) launch_batch_pass<S, 8, 1, 3, 2>(b, st, A, po, tbase, t_local, grid);
-    else if (K == 3) launch_batch_pass<S, 8, 1, 3, 3>(b, st, A, po, tbase, t_local, grid);
-    else if (K == 4) launch_batch_pass<S, 8, 1, 3, 4>(b, st, A, po, tbase, t_local, grid);
-    else if (K == 5) launch_batch_pass<S, 8, 1, 3, 5>(b, st, A, po, tbase, t_local, grid);
+    if (K == 2) launch_batch_pass<S, 8, 1, 3, 2>(b, tl, st, A, po, tbase, t_local, grid);
+    else if (K == 3) launch_batch_pass<S, 8, 1, 3, 3>(b, tl, st, A, po, tbase, t_local, grid);
+    else if (K == 4) launch_batch_pass<S, 8, 1, 3, 4>(b, tl, st, A, po, tbase, t_local, grid);
+    else if (K == 5) launch_batch_pass<S, 8, 1, 3, 5>(b, tl, st, A, po, tbase, t_local, grid);
     else return set_error(VL_EINVAL, "FP32 batch passes take 2..5 steps");
   }
   return VL_OK;
@@ -535,7 +649,7 @@ void launch_group_phase(vl_batch* b, hipStream_t bs, hipStream_t ms, const typen
                         const int64_t* tbase, int t_local, int j, const GroupGrids& gg) {
   hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, bs, A, B, b->dcases, b->d_bcase,
                      b->d_bb0, b->d_bact, tbase, t_local + j, b->d_kick, b->d_hist, b->d_status, b->d_done,
-                     SQRT_CS2_B);
+                     SQRT_CS2_B, (const int32_t*)nullptr, 0);
   hipLaunchKernelGGL(k_batch_listed<S>, dim3((unsigned)gg.listed), dim3(BT), 0, ms, A, B, b->dcases, b->d_lcase,
                      b->d_lfirst, b->d_lcount, b->d_lnodes, b->d_ln_at, tbase, t_local + j, j, b->d_status);
   hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)gg.bulk), dim3(BT), 0, ms, A, B, b->dflags, b->dcases,
@@ -557,22 +671,144 @@ void launch_group_phase(vl_batch* b, hipStream_t bs, hipStream_t ms, const typen
 // covered nodes only, as the single-domain path does, since the batch's single-step kernels
 // slow the passes more than they hide; and one pass kernel per collision model, side by side
 // or one after the other, since each case's tasks then no longer run together.)
+// The region kernel over steps j0 .. j1 - 1 of the group.
+template <class S, int RT>
+void launch_region_rt(vl_batch* b, hipStream_t st, const PoolSet<typename S::T>& ps, const int64_t* tbase,
+                      int t_local, int j0, int j1, int grid) {
+  hipLaunchKernelGGL((k_batch_region<S, RT>), dim3((unsigned)grid), dim3(RT), 0, st, ps, b->dcases, b->d_rcase,
+                     b->d_rn_at, b->d_clfirst, b->d_clcount, b->d_lnodes, tbase, t_local, j0, j1, b->d_kick,
+                     b->d_hist, b->d_status, b->d_done, SQRT_CS2_B);
+}
 template <class S>
-vl_status launch_batch_group(vl_batch* b, const int64_t* tbase, int t_local, const GroupGrids& gg) {
+void launch_region(vl_batch* b, hipStream_t st, const PoolSet<typename S::T>& ps, const int64_t* tbase, int t_local,
+                   int j0, int j1, int grid) {
+  if (b->region_threads == 256) launch_region_rt<S, 256>(b, st, ps, tbase, t_local, j0, j1, grid);
+  else if (b->region_threads == 1024) launch_region_rt<S, 1024>(b, st, ps, tbase, t_local, j0, j1, grid);
+  else launch_region_rt<S, 512>(b, st, ps, tbase, t_local, j0, j1, grid);
+}
+
+// Step j of the cases that end inside the group (single steps of all their nodes; the region
+// kernel and the pass take the other cases): the boundary tiles past bfirst_at[t], then the bulk
+// tiles past kfirst_at[t].
+template <class S>
+void launch_finishing_step(vl_batch* b, hipStream_t st, const typename S::T* A, typename S::T* B,
+                           const int64_t* tbase, int t_local, int j, const GroupGrids& gg) {
+  hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)gg.bnd), dim3(BT), 0, st, A, B, b->dcases, b->d_bcase,
+                     b->d_bb0, b->d_bact, tbase, t_local + j, b->d_kick, b->d_hist, b->d_status, b->d_done,
+                     SQRT_CS2_B, (const int32_t*)b->d_bfirst_at, j);
+  hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)gg.bulk), dim3(BT), 0, st, A, B, b->dflags, b->dcases,
+                     b->d_kcase, b->d_ky0, b->d_kx0, b->d_kact, tbase, t_local + j, b->tw_log2, b->d_status,
+                     (const int32_t*)b->d_kfirst_at, j);
+}
+
+// A group of K steps with the region kernel.  Main stream: the pass, then the region kernel's
+// steps 1 .. K-1 (or 0 .. K-1).  Side stream: the region kernel's step 0 beside the pass (it reads
+// pool cur only), then the K single steps of the cases that end inside the group.  Joined at the
+// end.
+template <class S>
+vl_status launch_batch_group_region(vl_batch* b, const int64_t* tbase, int t_local, const GroupGrids& gg) {
   using T = typename S::T;
   const int K = b->pass_k, nb = b->nbuf;
   auto P = [&](int j) { return (T*)b->pool[(b->cur + j) % nb]; };
   PassOut<T> po{};
   for (int L = 1; L < K; ++L) po.ring[L - 1] = P(L);
   po.out = P(K);
+  PoolSet<T> ps{};
+  for (int j = 0; j <= K; ++j) ps.p[j] = P(j);
   if (b->serial) {
-    VL_TRYB((launch_pass_k<S>(b, b->stream, P(0), po, tbase, t_local, gg.pass)));
+    VL_TRYB((launch_pass_k<S>(b, b->tasks, b->stream, P(0), po, tbase, t_local, gg.pass)));
+    launch_region<S>(b, b->stream, ps, tbase, t_local, 0, K, gg.listed);
+    for (int j = 0; j < K; ++j) launch_finishing_step<S>(b, b->stream, P(j), P(j + 1), tbase, t_local, j, gg);
+    VL_HIP(hipGetLastError());
+  } else {
+    VL_HIP(hipEventRecord(b->ev_fork, b->stream));
+    VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+    VL_TRYB((launch_pass_k<S>(b, b->tasks, b->stream, P(0), po, tbase, t_local, gg.pass)));
+    if (b->region0_side) {
+      launch_region<S>(b, b->side, ps, tbase, t_local, 0, 1, gg.listed);
+      VL_HIP(hipEventRecord(b->ev_mid, b->side));
+    }
+    for (int j = 0; j < K; ++j) launch_finishing_step<S>(b, b->side, P(j), P(j + 1), tbase, t_local, j, gg);
+    VL_HIP(hipEventRecord(b->ev_join, b->side));
+    if (b->region0_side) VL_HIP(hipStreamWaitEvent(b->stream, b->ev_mid, 0));
+    launch_region<S>(b, b->stream, ps, tbase, t_local, b->region0_side ? 1 : 0, K, gg.listed);
+    VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+    VL_HIP(hipGetLastError());
+  }
+  b->cur = (b->cur + K) % nb;
+  b->t += K;
+  return VL_OK;
+}
+
+// A group of K steps with the region kernel beside the next group's far pass.
+//  * Main stream: the far tasks' pass, then (after the previous group's region kernel) the near
+//    tasks' pass.  A far task's outputs lie more than K nodes from every uncovered node, so its
+//    pass reads covered nodes only (of the previous group's level K, written by both passes
+//    before it in stream order) and writes neither ring values nor anything the region kernel
+//    touches; a near task's region holds uncovered nodes, whose level K the previous group's
+//    region kernel writes.
+//  * Side stream: the single steps of the cases that end inside the group, then (after the near
+//    pass, which writes every ring value) the region kernel's K steps.
+// So the region kernel of group g runs beside the far pass of group g + 1.  Each event is
+// waited for before it is recorded again; join_side joins the side stream back at the end of a
+// window or a run.
+template <class S>
+vl_status launch_batch_group_overlap(vl_batch* b, const int64_t* tbase, int t_local, const GroupGrids& gg) {
+  using T = typename S::T;
+  const int K = b->pass_k, nb = b->nbuf;
+  auto P = [&](int j) { return (T*)b->pool[(b->cur + j) % nb]; };
+  PassOut<T> po{};
+  for (int L = 1; L < K; ++L) po.ring[L - 1] = P(L);
+  po.out = P(K);
+  PoolSet<T> ps{};
+  for (int j = 0; j <= K; ++j) ps.p[j] = P(j);
+  bool first = false;
+  if (!b->ov_open) {
+    VL_HIP(hipEventRecord(b->ev_fork, b->stream));
+    VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+    b->ov_open = true;
+    first = true;
+  }
+  VL_TRYB((launch_pass_k<S>(b, b->tasks_far, b->stream, P(0), po, tbase, t_local, gg.pass)));
+  if (!first) VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+  VL_TRYB((launch_pass_k<S>(b, b->tasks_near, b->stream, P(0), po, tbase, t_local, gg.pass_near)));
+  VL_HIP(hipEventRecord(b->ev_mid, b->stream));
+  for (int j = 0; j < K; ++j) launch_finishing_step<S>(b, b->side, P(j), P(j + 1), tbase, t_local, j, gg);
+  VL_HIP(hipStreamWaitEvent(b->side, b->ev_mid, 0));
+  launch_region<S>(b, b->side, ps, tbase, t_local, 0, K, gg.listed);
+  VL_HIP(hipEventRecord(b->ev_join, b->side));
+  VL_HIP(hipGetLastError());
+  b->cur = (b->cur + K) % nb;
+  b->t += K;
+  return VL_OK;
+}
+
+// Joins the side stream back into the main stream after overlapped groups.
+vl_status join_side(vl_batch* b) {
+  if (!b->ov_open) return VL_OK;
+  VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+  b->ov_open = false;
+  return VL_OK;
+}
+
+template <class S>
+vl_status launch_batch_group(vl_batch* b, const int64_t* tbase, int t_local, const GroupGrids& gg) {
+  using T = typename S::T;
+  if (b->region && b->overlap && !b->serial) return launch_batch_group_overlap<S>(b, tbase, t_local, gg);
+  if (b->region) return launch_batch_group_region<S>(b, tbase, t_local, gg);
+  const int K = b->pass_k, nb = b->nbuf;
+  auto P = [&](int j) { return (T*)b->pool[(b->cur + j) % nb]; };
+  PassOut<T> po{};
+  for (int L = 1; L < K; ++L) po.ring[L - 1] = P(L);
+  po.out = P(K);
+  if (b->serial) {
+    VL_TRYB((launch_pass_k<S>(b, b->tasks, b->stream, P(0), po, tbase, t_local, gg.pass)));
     for (int j = 0; j < K; ++j) launch_group_phase<S>(b, b->stream, b->stream, P(j), P(j + 1), tbase, t_local, j, gg);
     VL_HIP(hipGetLastError());
   } else {
     VL_HIP(hipEventRecord(b->ev_fork, b->stream));
     VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
-    VL_TRYB((launch_pass_k<S>(b, b->stream, P(0), po, tbase, t_local, gg.pass)));
+    VL_TRYB((launch_pass_k<S>(b, b->tasks, b->stream, P(0), po, tbase, t_local, gg.pass)));
     launch_group_phase<S>(b, b->side, b->side, P(0), P(1), tbase, t_local, 0, gg);
     VL_HIP(hipEventRecord(b->ev_join, b->side));
     VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
@@ -645,15 +881,21 @@ GroupGrids group_grids_at(const vl_batch* b, int64_t t, int groups) {
   const int64_t last = (int64_t)b->h_ln_at.size() - 1;
   auto at = [&](const std::vector<int32_t>& v, int64_t i) { return v[(size_t)std::min(std::max<int64_t>(i, 0), last)]; };
   GroupGrids gg;
-  gg.pass = ladder_ceil((at(b->tasks.h_n_at, t) + W - 1) / W, (b->tasks.n + W - 1) / W);
-  gg.listed = ladder_ceil(at(b->h_ln_at, t), b->nlisted_tiles);
-  int64_t bulk = 1;
+  const bool split = b->region && b->overlap && !b->serial;  // far and near tasks in launches of their own
+  const vl_batch::TaskList& first_list = split ? b->tasks_far : b->tasks;
+  gg.pass = ladder_ceil((at(first_list.h_n_at, t) + W - 1) / W, (first_list.n + W - 1) / W);
+  gg.pass_near = split ? ladder_ceil((at(b->tasks_near.h_n_at, t) + W - 1) / W, (b->tasks_near.n + W - 1) / W) : 0;
+  gg.listed = b->region ? ladder_ceil(at(b->h_rn_at, t), (int64_t)b->hcases.size())
+                        : ladder_ceil(at(b->h_ln_at, t), b->nlisted_tiles);
+  int64_t bulk = 1, bnd = 1;
   for (int g = 0; g < groups; ++g)
     for (int j = 0; j < K; ++j) {
       const int64_t tg = t + (int64_t)g * K;
       bulk = std::max<int64_t>(bulk, (int64_t)at(b->h_kact, tg + j) - at(b->h_kfirst_at, tg));
+      if (b->region) bnd = std::max<int64_t>(bnd, (int64_t)at(b->h_bact, tg + j) - at(b->h_bfirst_at, tg));
     }
   gg.bulk = ladder_ceil(bulk, b->nbulk_tiles);
+  gg.bnd = b->region ? ladder_ceil(bnd, b->nbnd_tiles) : 0;
   return gg;
 }
 
@@ -692,7 +934,8 @@ vl_status launch_trial_step(vl_batch* b, int grid, int tw_log2, const int32_t* k
     VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
   }
   hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, bs, A, B, b->dcases, b->d_bcase,
-                     b->d_bb0, b->d_bact, t0, 0, b->d_kick, b->d_hist, b->d_status, b->d_done, SQRT_CS2_B);
+                     b->d_bb0, b->d_bact, t0, 0, b->d_kick, b->d_hist, b->d_status, b->d_done, SQRT_CS2_B,
+                     (const int32_t*)nullptr, 0);
   if (!b->serial) VL_HIP(hipEventRecord(b->ev_join, b->side));
   hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)grid), dim3(BT), 0, b->stream, A, B, b->dflags, b->dcases, kc,
                      ky, kx, kact, t0, 0, tw_log2, b->d_status, (const int32_t*)nullptr, 0);
@@ -726,6 +969,7 @@ void free_batch(vl_batch* b) {
       if (h) (void)hipEventDestroy(h);
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+  if (b->ev_mid) (void)hipEventDestroy(b->ev_mid);
   if (b->side) (void)hipStreamDestroy(b->side);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -747,6 +991,7 @@ vl_status build(vl_batch* b) {
   VL_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
   VL_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
   VL_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+  VL_HIP(hipEventCreateWithFlags(&b->ev_mid, hipEventDisableTiming));
   // ---- set-up of every case: on the device when every body is a polygon or a circle (the set-up
   // kernels evaluate those predicates themselves), else on host threads; VL_BATCH_SETUP=host
   // forces the host builder ----
@@ -881,12 +1126,14 @@ vl_status build(vl_batch* b) {
   // ---- K-step passes: the plan of every case (its strips' tasks and listed nodes) ----
   // Defaults: FP32 5 steps per pass (the 512-case sweep on MI355X: 93 GLUPS against 87 at 4 and
   // 67 in single steps), FP64 3 (the 4-step FP64 pass of both collides needs all 256 registers),
-  // FP16 3; tasks of 20 (FP32), 16 (FP64) or 24 (FP16) rows.  VL_BATCH_PASS_STEPS (1 = single
-  // steps only) and VL_BATCH_TB_ROWS override.
+  // FP16 3; tasks of 12 (FP32), 16 (FP64) or 24 (FP16) rows.  FP32 on the sweep, cases/s by task
+  // height: 8 rows 25.9, 10 26.1, 12 26.3, 14 26.2, 16 25.9, 20 25.4, 28 24.4, 36 23.7 (shorter
+  // tasks re-read more halo rows but balance better over the sweep's short, narrow cases).
+  // VL_BATCH_PASS_STEPS (1 = single steps only) and VL_BATCH_TB_ROWS override.
   b->pass_k = b->precision == VL_F32 ? 5 : 3;
   if (const char* e = std::getenv("VL_BATCH_PASS_STEPS"))
     b->pass_k = std::min(std::max(std::atoi(e), 1), b->precision == VL_F32 ? TB_MAX_K : 4);
-  b->pass_rows = b->precision == VL_F16 ? 24 : (b->precision == VL_F32 ? 20 : 16);
+  b->pass_rows = b->precision == VL_F16 ? 24 : (b->precision == VL_F32 ? 12 : 16);
   if (const char* e = std::getenv("VL_BATCH_TB_ROWS")) b->pass_rows = std::max(1, std::atoi(e));
   // 8 waves per workgroup (FP64: 4); on the 512-case sweep FP32 in 4-wave workgroups, which
   // find more lockstep groups among narrow cases, ran 8 % slower, and tasks of 32 rows 7 %.
@@ -1125,6 +1372,7 @@ vl_status build(vl_batch* b) {
   VL_TRYB(dupload(b, b->d_bcase, bc));
   VL_TRYB(dupload(b, b->d_bb0, bb0));
   VL_TRYB(dupload(b, b->d_bact, bact));
+  b->h_bact = bact;
   VL_TRYB(dupload(b, b->dcases, b->hcases));
   VL_TRYB(dalloc(b, b->d_status, nc));
   VL_TRYB(dalloc(b, b->d_done, nc));
@@ -1254,26 +1502,43 @@ vl_status build(vl_batch* b) {
   b->h_kact = std::move(kact);
   // ---- pass tasks and listed tiles, cases in descending step order ----
   {
-    std::vector<int4> task;
-    std::vector<int32_t> tcase, lcase, lfirst, lcount;
+    // three task lists: all tasks (one launch per group), and the far and near tasks (two
+    // launches, when the region kernel overlaps the next group's far pass); in each, a case's
+    // tasks in (first row, first column) order, so that a workgroup's waves sweep adjacent
+    // strips of the same rows
+    std::vector<int4> task, ftask, ntask;
+    std::vector<int32_t> tcase, ftcase, ntcase, lcase, lfirst, lcount;
     std::vector<uint32_t> lnodes;
-    std::vector<int32_t> tend(nc, 0), lend(nc, 0);
+    std::vector<int32_t> tend(nc, 0), ftend(nc, 0), ntend(nc, 0), lend(nc, 0), clfirst(nc, 0), clcount(nc, 0);
+    auto by_row = [](const int4& a, const int4& c) { return a.y != c.y ? a.y < c.y : a.x < c.x; };
     if (b->pass_k > 1)
       for (int ci : order) {
         PassPlan& pl = plans[(size_t)ci];
-        // one launch takes far and near tasks alike: (first row, first column) order, so that
-        // a workgroup's waves sweep adjacent strips of the same rows
-        std::sort(pl.tasks.begin(), pl.tasks.end(),
-                  [](const int4& a, const int4& c) { return a.y != c.y ? a.y < c.y : a.x < c.x; });
+        std::vector<int4> far(pl.tasks.begin(), pl.tasks.begin() + pl.nfar), near(pl.tasks.begin() + pl.nfar, pl.tasks.end());
+        std::sort(pl.tasks.begin(), pl.tasks.end(), by_row);
+        std::sort(far.begin(), far.end(), by_row);
+        std::sort(near.begin(), near.end(), by_row);
         for (const int4& tk : pl.tasks) {
           task.push_back(tk);
           tcase.push_back(ci);
         }
+        for (const int4& tk : far) {
+          ftask.push_back(tk);
+          ftcase.push_back(ci);
+        }
+        for (const int4& tk : near) {
+          ntask.push_back(tk);
+          ntcase.push_back(ci);
+        }
+        ftend[(size_t)ci] = (int32_t)ftask.size();
+        ntend[(size_t)ci] = (int32_t)ntask.size();
         for (size_t k = 0; k < pl.listed.size(); k += BT) {
           lcase.push_back(ci);
           lfirst.push_back((int32_t)(lnodes.size() + k));
           lcount.push_back((int32_t)std::min<size_t>(BT, pl.listed.size() - k));
         }
+        clfirst[(size_t)ci] = (int32_t)lnodes.size();
+        clcount[(size_t)ci] = (int32_t)pl.listed.size();
         lnodes.insert(lnodes.end(), pl.listed.begin(), pl.listed.end());
         tend[(size_t)ci] = (int32_t)task.size();
         lend[(size_t)ci] = (int32_t)lcase.size();
@@ -1284,33 +1549,48 @@ vl_status build(vl_batch* b) {
     const size_t nt = (size_t)b->max_steps + 2;
     vl_batch::TaskList& tl = b->tasks;
     tl.h_n_at.assign(nt, 0);
+    b->tasks_far.h_n_at.assign(nt, 0);
+    b->tasks_near.h_n_at.assign(nt, 0);
     b->h_ln_at.assign(nt, 0);
     b->h_kfirst_at.assign(nt, 0);
+    b->h_rn_at.assign(nt, 0);
+    b->h_bfirst_at.assign(nt, 0);
     if (b->pass_k > 1)
-      for (size_t tt = 0; tt < nt; ++tt)
+      for (size_t tt = 0; tt < nt; ++tt) {
+        int32_t ncase = 0;
         for (int ci : order) {
           if (b->hcases[(size_t)ci].steps < (int64_t)tt + b->pass_k) break;
           tl.h_n_at[tt] = tend[(size_t)ci];
+          b->tasks_far.h_n_at[tt] = ftend[(size_t)ci];
+          b->tasks_near.h_n_at[tt] = ntend[(size_t)ci];
           b->h_ln_at[tt] = lend[(size_t)ci];
           b->h_kfirst_at[tt] = kend[(size_t)ci];
+          b->h_bfirst_at[tt] = bnd_end[(size_t)ci];
+          b->h_rn_at[tt] = ++ncase;
         }
+      }
     // Lockstep: the waves of a workgroup sweep in step when its tasks belong to one case and
     // cover the same rows (so they meet the same barriers, and a retired case stops them all).
-    // A group's pass launches the prefix of h_n_at[t] tasks from task 0, in workgroups of W.
-    {
+    // A group's pass launches the prefix of h_n_at[t] tasks of a list from its task 0, in
+    // workgroups of W.
+    auto finish_list = [&](vl_batch::TaskList& l, std::vector<int4>& tk, std::vector<int32_t>& tc) -> vl_status {
       TaskRanges ranges;
-      std::vector<int32_t> ends(tl.h_n_at.begin(), tl.h_n_at.end());
-      ends.push_back((int32_t)task.size());
+      std::vector<int32_t> ends(l.h_n_at.begin(), l.h_n_at.end());
+      ends.push_back((int32_t)tk.size());
       std::sort(ends.begin(), ends.end());
       ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
       for (int32_t e : ends) ranges.emplace_back(0, e);
-      mark_lockstep(task, tcase.data(), ranges, b->pass_waves);
-      VL_TRYB(check_lockstep(task, tcase.data(), ranges, b->pass_waves));
-    }
-    tl.n = (int)task.size();
-    VL_TRYB(dupload(b, tl.task, task));
-    VL_TRYB(dupload(b, tl.tcase, tcase));
-    VL_TRYB(dupload(b, tl.n_at, tl.h_n_at));
+      mark_lockstep(tk, tc.data(), ranges, b->pass_waves);
+      VL_TRYB(check_lockstep(tk, tc.data(), ranges, b->pass_waves));
+      l.n = (int)tk.size();
+      VL_TRYB(dupload(b, l.task, tk));
+      VL_TRYB(dupload(b, l.tcase, tc));
+      VL_TRYB(dupload(b, l.n_at, l.h_n_at));
+      return VL_OK;
+    };
+    VL_TRYB(finish_list(tl, task, tcase));
+    VL_TRYB(finish_list(b->tasks_far, ftask, ftcase));
+    VL_TRYB(finish_list(b->tasks_near, ntask, ntcase));
     b->nlisted_tiles = (int)lcase.size();
     VL_TRYB(dupload(b, b->d_lcase, lcase));
     VL_TRYB(dupload(b, b->d_lfirst, lfirst));
@@ -1318,7 +1598,23 @@ vl_status build(vl_batch* b) {
     VL_TRYB(dupload(b, b->d_lnodes, lnodes));
     VL_TRYB(dupload(b, b->d_ln_at, b->h_ln_at));
     VL_TRYB(dupload(b, b->d_kfirst_at, b->h_kfirst_at));
+    std::vector<int32_t> rcase(order.begin(), order.end());
+    VL_TRYB(dupload(b, b->d_rcase, rcase));
+    VL_TRYB(dupload(b, b->d_rn_at, b->h_rn_at));
+    VL_TRYB(dupload(b, b->d_bfirst_at, b->h_bfirst_at));
+    VL_TRYB(dupload(b, b->d_clfirst, clfirst));
+    VL_TRYB(dupload(b, b->d_clcount, clcount));
   }
+  // The region kernel is bit-identical but measured slower on the 512-case sweep (22.8 cases/s
+  // after the pass, 21.7 beside the next far pass, against 25.4 with the per-step kernels): one
+  // workgroup per case cannot hide the partial-line traffic of the nodes beside the inlet and
+  // outlet, and beside the far pass that traffic slows the pass more than the overlap hides.
+  b->region = false;
+  if (const char* e = std::getenv("VL_BATCH_REGION")) b->region = b->pass_k > 1 && e[0] == '1';
+  if (const char* e = std::getenv("VL_BATCH_REGION_T")) b->region_threads = std::atoi(e);
+  if (b->region_threads != 256 && b->region_threads != 1024) b->region_threads = 512;
+  if (const char* e = std::getenv("VL_BATCH_REGION0")) b->region0_side = std::string(e) != "after";
+  if (const char* e = std::getenv("VL_BATCH_OVERLAP")) b->overlap = e[0] != '0';
   b->use_graphs = true;
   if (const char* e = std::getenv("VL_GRAPHS")) b->use_graphs = e[0] != '0';
   lap(3);
@@ -1332,9 +1628,9 @@ int window_steps(const vl_batch* b) { return b->pass_k > 1 ? GRAPH_W / b->pass_k
 vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
   const int wsteps = window_steps(b);
   const int K = b->pass_k;
-  GroupGrids gg{0, 0, 0};
+  GroupGrids gg{0, 0, 0, 0, 0};
   const int grid = K > 1 ? (gg = group_grids_at(b, b->t, wsteps / K)).bulk : bulk_grid_at(b, b->t);
-  const auto key = std::make_tuple(b->cur, grid, gg.pass, gg.listed);
+  const auto key = std::make_tuple(b->cur, grid, gg.pass, gg.listed, gg.bnd, gg.pass_near);
   auto found = b->graphs.find(key);
   if (found != b->graphs.end()) {
     *out = found->second;
@@ -1345,14 +1641,16 @@ vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
   hipGraph_t g = nullptr;
   VL_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   vl_status st = VL_OK;
-  if (K > 1)
+  if (K > 1) {
     for (int k = 0; k < wsteps && st == VL_OK; k += K) st = batch_group(b, b->d_tbase, k, gg);
-  else
+    if (st == VL_OK) st = join_side(b);
+  } else
     for (int k = 0; k < wsteps && st == VL_OK; ++k) st = batch_step(b, b->d_tbase, k, grid);
   if (st == VL_OK) hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(1), 0, b->stream, b->d_tbase, (int64_t)wsteps);
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   b->cur = cur0;
   b->t = t0;
+  b->ov_open = false;
   if (st != VL_OK) {
     if (g) (void)hipGraphDestroy(g);
     return st;
@@ -1450,9 +1748,11 @@ vl_status vl_batch_run(vl_batch* b, int64_t max_steps) {
     VL_HIP(hipMemcpyAsync(b->d_tbase, &b->t, sizeof(int64_t), hipMemcpyHostToDevice, b->stream));
     VL_HIP(hipStreamSynchronize(b->stream));
     int kk = 0;
-    if (b->pass_k > 1)
+    if (b->pass_k > 1) {
       for (; k + b->pass_k <= todo; k += b->pass_k, kk += b->pass_k)
         VL_TRYB(batch_group(b, b->d_tbase, kk, group_grids_at(b, b->t, 1)));
+      VL_TRYB(join_side(b));
+    }
     for (; k < todo; ++k, ++kk) VL_TRYB(batch_step(b, b->d_tbase, kk, bulk_grid_at(b, b->t)));
   }
   VL_HIP(hipStreamSynchronize(b->stream));

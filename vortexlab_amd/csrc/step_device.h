@@ -178,22 +178,28 @@ __global__ __launch_bounds__(256) void k_outlet_prime(const typename S::T* __res
 
 // Density offset of the outlet controller from the nodes' outward velocities `pv` (which the
 // previous step's boundary update left behind): serial left-to-right sum for FP64 storage,
-// fixed-order tree otherwise.  Every thread of the workgroup must call this.
-template <class S>
+// fixed-order tree otherwise.  Every thread of the workgroup must call this.  NT: threads of the
+// workgroup (at least STEP_T); the first STEP_T take the sums, so the association is the same
+// for every NT.
+template <class S, int NT = STEP_T>
 __device__ inline double outlet_offset(const double* __restrict__ pv, int np, double target, double sqrt_cs2,
                                 double* red) {
+  static_assert(NT >= STEP_T, "workgroup narrower than the reduction");
+  const bool part = NT == STEP_T || (int)threadIdx.x < STEP_T;  // takes part in the sums
   double sum = 0.0;  // serial running sum (exact) or this thread's strided partial (tree)
   if constexpr (!S::exact) {
     // this thread's strided partial: the loads are independent, so eight are issued at a time
     // (the addition order stays slot k, k + T, k + 2T, ...)
+    if (part) {
 #pragma unroll 8
-    for (int k = (int)threadIdx.x; k < np; k += STEP_T) sum += pv[k];
+      for (int k = (int)threadIdx.x; k < np; k += STEP_T) sum += pv[k];
+    }
   }
   for (int base = 0; S::exact && base < np; base += STEP_T) {
     const int k = base + (int)threadIdx.x;
     const double v = k < np ? pv[k] : 0.0;
     if constexpr (S::exact) {
-      red[threadIdx.x] = v;
+      if (part) red[threadIdx.x] = v;
       __syncthreads();
       if (threadIdx.x == 0) {
         const int m = min(STEP_T, np - base);
@@ -203,7 +209,7 @@ __device__ inline double outlet_offset(const double* __restrict__ pv, int np, do
     }
   }
   if constexpr (!S::exact) {
-    red[threadIdx.x] = sum;
+    if (part) red[threadIdx.x] = sum;
     __syncthreads();
     for (int w = STEP_T / 2; w > 0; w >>= 1) {
       if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
@@ -220,15 +226,20 @@ __device__ inline double outlet_offset(const double* __restrict__ pv, int np, do
 }
 
 // Force groups: fixed-order FP64 tree.  Thread k sums slots k, k+T, k+2T, ... in order, then
-// a fixed pairwise tree over the threads: the same association every step and every run.
+// a fixed pairwise tree over the threads: the same association every step and every run.  NT:
+// threads of the workgroup (at least STEP_T); the first STEP_T take the sums.
+template <int NT = STEP_T>
 __device__ inline void reduce_forces(const double* __restrict__ fc, const int32_t* __restrict__ gfirst, int ngroups,
                               double* __restrict__ out, double* red) {
+  static_assert(NT >= STEP_T, "workgroup narrower than the reduction");
+  const bool part = NT == STEP_T || (int)threadIdx.x < STEP_T;
   for (int grp = 0; grp < ngroups; ++grp) {
     const int s0 = gfirst[grp], s1 = gfirst[grp + 1];
     for (int comp = 0; comp < 2; ++comp) {
       double a = 0.0;
-      for (int s = s0 + (int)threadIdx.x; s < s1; s += STEP_T) a += fc[2 * s + comp];
-      red[threadIdx.x] = a;
+      if (part)
+        for (int s = s0 + (int)threadIdx.x; s < s1; s += STEP_T) a += fc[2 * s + comp];
+      if (part) red[threadIdx.x] = a;
       __syncthreads();
       for (int w = STEP_T / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
