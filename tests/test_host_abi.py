@@ -60,7 +60,8 @@ SHAPES = [
 @pytest.mark.parametrize("k", range(len(SHAPES)))
 def test_shapes_and_cut_fractions_match_oracle(orc, k):
     g, o = vl.backend(), orc
-    hg, ho = SHAPES[k].make(g), SHAPES[k].make(o)
+    keep = []  # callbacks of the shapes, held while the handles are used
+    hg, ho = SHAPES[k].make(g, keep), SHAPES[k].make(o, keep)
     rng = np.random.default_rng(k)
     pts = rng.uniform(0, 40, size=(2000, 2))
     for x, y in pts:
@@ -82,7 +83,7 @@ def test_shapes_and_cut_fractions_match_oracle(orc, k):
 def test_cut_fraction_closed_form():
     """src/geometry.rs:298-308: the bisection finds the circle crossing."""
     g = vl.backend()
-    c = api.circle(0.0, 0.0, 10.3).make(g)
+    c = api.circle(0.0, 0.0, 10.3).make(g, [])
     assert abs(g.cut_fraction(c, -11.0, 0.0, 1.0, 0.0) - 0.7) < 1e-9
     assert abs(g.cut_fraction(c, 8.0, 8.0, -1.0, -1.0) - (8.0 - 10.3 / math.sqrt(2.0))) < 1e-9
     g.shape_free(c)
@@ -98,7 +99,7 @@ def test_section_areas():
     h = 1.2 / n
     xs = -0.6 + (np.arange(n) + 0.5) * h
     for corner, b, expect in cases:
-        s = api.section(corner, b, 0.0, 0.0, 1.0).make(g)
+        s = api.section(corner, b, 0.0, 0.0, 1.0).make(g, [])
         count = sum(g.shape_inside(s, x, y) for y in xs for x in xs)
         assert abs(count * h * h - expect) < 3e-3, corner
         assert g.shape_inside(s, 0.499, 0.0) and g.shape_inside(s, 0.0, 0.499) and not g.shape_inside(s, 0.501, 0.0)

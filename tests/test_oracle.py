@@ -123,7 +123,7 @@ def test_uniform_stream(orc):
 
 def test_cut_fraction_on_a_circle(orc):
     """src/geometry.rs:298-308."""
-    c = api.circle(0.0, 0.0, 10.3).make(orc)
+    c = api.circle(0.0, 0.0, 10.3).make(orc, [])
     assert abs(orc.cut_fraction(c, -11.0, 0.0, 1.0, 0.0) - 0.7) < 1e-9
     q = orc.cut_fraction(c, 8.0, 8.0, -1.0, -1.0)
     assert abs(q - (8.0 - 10.3 / math.sqrt(2.0))) < 1e-9

@@ -47,7 +47,7 @@ def test_rasterizer_matches_polygon_predicate():
         d = 16.0
         outline = workloads.star_polygon(rng, d, 100.5, 60.0)
         m = surrogate.rasterize(outline, 100.5, 60.0, d)
-        h = api.polygon(outline).make(be)
+        h = api.polygon(outline).make(be, [])
         c = (np.arange(64) + 0.5) / 64 * 1.2 * d - 0.6 * d
         for iy in range(0, 64, 3):
             for ix in range(0, 64, 3):

@@ -61,7 +61,7 @@ def test_star_polygon_predicate_agrees_with_its_radii():
         # the polygon is star-shaped about its centre: that disc reaches at least the inscribed
         # circle of the regular n-gon of the smallest radius
         assert rin >= 0.6 * d / 2 * math.cos(math.pi / len(pts)) * (1 - 1e-12)
-        h = api.polygon(pts).make(be)
+        h = api.polygon(pts).make(be, [])
         for _ in range(200):
             ang = rng.uniform(0, 2 * math.pi)
             rr = rng.uniform(0, 1.3) * rout

@@ -244,7 +244,10 @@ class ShapeSpec:
     args: tuple
     inner: Optional["ShapeSpec"] = None
 
-    def make(self, be: Backend):
+    def make(self, be: Backend, keep: list):
+        """The library handle of this shape.  A Python predicate becomes a ctypes callback, which
+        is appended to `keep`: the caller holds it until the handle has been consumed (shapes
+        are evaluated while a domain is built, never afterwards)."""
         k, a = self.kind, self.args
         if k == "circle":
             return be.shape_circle(*a)
@@ -254,9 +257,9 @@ class ShapeSpec:
         if k == "rounded_square":
             return be.shape_rounded_square(*a)
         if k == "rotated":
-            return be.shape_rotated(self.inner.make(be), *a)
+            return be.shape_rotated(self.inner.make(be, keep), *a)
         if k == "translated":
-            return be.shape_translated(self.inner.make(be), *a)
+            return be.shape_translated(self.inner.make(be, keep), *a)
         if k == "section":
             return be.shape_section(*a)
         if k == "outside_band":
@@ -264,12 +267,9 @@ class ShapeSpec:
         if k == "callback":
             fn = a[0]
             cb = INSIDE_FN(lambda _ctx, x, y: 1 if fn(x, y) else 0)
-            _KEEP.append(cb)
+            keep.append(cb)
             return be.shape_callback(cb, None)
         raise ValueError(k)
-
-
-_KEEP: list = []  # ctypes callbacks must outlive the handles that call them
 
 
 def circle(cx, cy, r):
@@ -323,7 +323,9 @@ def vel_fn(fn: Callable[[float, float], Sequence[float]]):
     return ("fn", fn)
 
 
-def _velocity_desc(v) -> Optional[VelocityDesc]:
+def _velocity_desc(v, keep: list) -> Optional[VelocityDesc]:
+    """The C descriptor of a wall velocity; a Python function becomes a ctypes callback appended
+    to `keep`, which the caller holds until the domain is built."""
     if v is None:
         return None
     d = VelocityDesc()
@@ -341,7 +343,7 @@ def _velocity_desc(v) -> Optional[VelocityDesc]:
             out[0], out[1] = float(u[0]), float(u[1])
 
         cb = VELOCITY_FN(tramp)
-        _KEEP.append(cb)
+        keep.append(cb)
         d.kind, d.fn = VEL_CALLBACK, cb
     else:
         raise ValueError(v)
@@ -366,6 +368,7 @@ class Builder:
         self.collision = TRT
         self.magic = 3.0 / 16.0
         self.accel = (0.0, 0.0)
+        self._keep = []  # ctypes callbacks of this domain's shapes and velocities, until build()
 
     def _kind(self, k: int) -> int:
         if k < 0:
@@ -374,8 +377,8 @@ class Builder:
 
     def add_wall(self, velocity=None, vgroup: int = 0, fgroup: Optional[int] = None,
                  shape: Optional[ShapeSpec] = None) -> int:
-        d = _velocity_desc(velocity)
-        sh = shape.make(self.be) if shape is not None else None
+        d = _velocity_desc(velocity, self._keep)
+        sh = shape.make(self.be, self._keep) if shape is not None else None
         return self._kind(self.be.builder_add_wall(self.h, C.byref(d) if d is not None else None, vgroup,
                                                    -1 if fgroup is None else fgroup, sh))
 
@@ -401,7 +404,10 @@ class Builder:
                                    self.magic, self.accel[0], self.accel[1]))
         out = C.c_void_p()
         h, self.h = self.h, None
-        be.check(be.sim_create(h, arg, C.byref(out)))
+        try:
+            be.check(be.sim_create(h, arg, C.byref(out)))
+        finally:
+            self._keep = []  # the build evaluated (and freed) every shape and velocity
         return Sim(be, out.value)
 
     def __del__(self):
@@ -608,7 +614,9 @@ class Channel:
         self.be = be
         d = _spec_desc(spec)
         out = C.c_void_p()
-        be.check(be.channel_create(C.byref(d), spec.body.make(be), arg, C.byref(out)))
+        keep = []  # callbacks of the body, held until the domain is built
+        be.check(be.channel_create(C.byref(d), spec.body.make(be, keep), arg, C.byref(out)))
+        del keep
         self.h = out.value
         self.u_ref = spec.u
         self.sim = Sim(be, be.channel_sim(self.h), owned=False)
@@ -649,16 +657,21 @@ class Batch:
         be.check(be.batch_create(precision, C.byref(out)))
         self.h = out.value
         self.steps = []
+        self._keep = []  # callbacks of the pending bodies, held until build()
 
     def add(self, spec: ChannelSpec, steps: int, d: float) -> int:
-        k = self.be.batch_add(self.h, C.byref(_spec_desc(spec)), spec.body.make(self.be), int(steps), float(d))
+        k = self.be.batch_add(self.h, C.byref(_spec_desc(spec)), spec.body.make(self.be, self._keep), int(steps),
+                              float(d))
         if k < 0:
             self.be.check(EINVAL)
         self.steps.append(int(steps))
         return k
 
     def build(self):
-        self.be.check(self.be.batch_build(self.h))
+        try:
+            self.be.check(self.be.batch_build(self.h))
+        finally:
+            self._keep = []  # the build evaluated (and freed) every pending body
 
     def run(self, max_steps: int = -1):
         self.be.check(self.be.batch_run(self.h, int(max_steps)))
