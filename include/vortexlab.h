@@ -179,6 +179,13 @@ vl_status vl_sim_run(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force
  * VL_EINVAL also while kernel timing is on.  Every handle on a GPU shares its scratch area: calls
  * that overlap on one GPU race. */
 vl_status vl_sim_run_noalloc(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist);
+/* Instantiates, without running anything, the step graphs a vl_sim_run of nsteps steps from the
+ * current state would replay (no reference counterpart: the reference has no launch graphs).
+ * vl_sim_run replays whole step windows (30 or 32 steps) on its own; the whole groups of K steps
+ * of a shorter run are replayed as one graph only when this call has instantiated it, and
+ * launched one by one otherwise.  Results are the same bits either way.  For callers that time
+ * or repeat short runs. */
+vl_status vl_sim_prepare(vl_sim* s, int64_t nsteps);
 /* Sim::force after the last step (force_groups * 2 doubles). */
 vl_status vl_sim_get_force(vl_sim* s, double* out);
 /* Populations of A (the current state) in the reference layout, as f_i in FP64. */

@@ -136,6 +136,40 @@ def test_joined_pass_equals_single_step(gpu, monkeypatch, k, prec, rows, v):
     ch.close()
 
 
+@pytest.mark.parametrize("k,prec", [(5, api.F32), (4, api.F64), (3, api.F16), (1, api.F32)])
+def test_prepared_short_runs_equal_direct_launches(gpu, monkeypatch, k, prec):
+    """vl_sim_prepare instantiates the graph of a run's whole groups that do not fill a step
+    window; runs that replay it give the bits of the same runs launched group by group (K-step
+    passes, and fused single steps), over pieces of every kind: shorter than a group, a few
+    groups, a window and more, with and without single steps left; preparing leaves the state
+    as it was."""
+    spec = tunnel_spec(d=16, re=100.0)
+    if k == 1:
+        single_step(monkeypatch, "fused")
+    else:
+        passes(monkeypatch, k, rows=12)
+    pieces = (5, 20, 7, 33, 3, 61, 1, 12, 40)
+    print(f"[passes] prepared runs K={k} prec={prec} pieces={pieces}", flush=True)
+
+    def go(prepare):
+        ch = api.Channel(gpu, spec, prec)
+        assert ch.sim.info().pass_steps == k
+        coefs = []
+        for n in pieces:
+            if prepare:
+                t = ch.sim.t
+                ch.sim.prepare(n)
+                assert ch.sim.t == t
+            coefs.append(ch.run(n, 16.0))
+        pops = ch.sim.read()
+        ch.close()
+        return np.concatenate(coefs), pops
+
+    direct, prepared = go(False), go(True)
+    assert np.array_equal(direct[0], prepared[0]), "forces differ"
+    assert np.array_equal(direct[1], prepared[1]), "populations differ"
+
+
 @pytest.mark.parametrize("workload,prec,steps", [("notched_8192x4096", api.F32, 43),
                                                  ("circle_16384x8192", api.F16, 23)])
 def test_pass_full_size_equals_single_step(gpu, monkeypatch, workload, prec, steps):

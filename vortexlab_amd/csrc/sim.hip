@@ -498,8 +498,8 @@ static void free_sim(vl_sim* s) {
   f(s->d_ctl);
   f(s->d_kbase);
   if (s->h_ctl) (void)hipHostFree(s->h_ctl);
-  for (hipGraphExec_t& ge : s->graph)
-    if (ge) (void)hipGraphExecDestroy(ge);
+  for (auto& kv : s->graphs)
+    if (kv.second.first) (void)hipGraphExecDestroy(kv.second.first);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : s->ev_chunk)
     if (e) (void)hipEventDestroy(e);
@@ -1087,44 +1087,68 @@ static vl_status collect_times(vl_sim* s, int64_t steps) {
   return VL_OK;
 }
 
-// A window of GRAPH_K steps captured once per (buffer parity, step parity) and replayed: the
-// kernel arguments of a window do not change between replays (controls and history are
-// indexed through the device counter kbase), so no per-step host work remains.
+// Step graphs: a window of GRAPH_K steps (whole groups) captured once per (current buffer, step
+// parity) and replayed; the kernel arguments of a window do not change between replays
+// (controls and history are indexed through the device counter kbase), so no per-step host
+// work remains.  A run shorter than a window replays a graph of its whole groups when one has
+// been instantiated for that state and length (vl_sim_prepare), and launches them directly
+// otherwise.
 static constexpr int GRAPH_K = 32;
 static int window_steps(const vl_sim* s) { return GRAPH_K / s->tb_k * s->tb_k; }  // whole groups
 
-static vl_status get_window_graph(vl_sim* s, hipGraphExec_t* out) {
-  const int key = s->cur * 2 + (int)(s->t & 1);
-  if (s->graph[key]) {
-    *out = s->graph[key];
+// The graph of `steps` steps (whole groups) from the current state, captured and instantiated
+// on first use; `after` = the current buffer after it.
+static vl_status get_step_graph(vl_sim* s, int steps, hipGraphExec_t* out, int* after) {
+  const std::pair<int, int> key(s->cur * 2 + (int)(s->t & 1), steps);
+  auto found = s->graphs.find(key);
+  if (found != s->graphs.end()) {
+    *out = found->second.first;
+    *after = found->second.second;
     return VL_OK;
   }
   const int cur0 = s->cur;
   const int64_t t0 = s->t;
+  // A graph holds the kernel arguments it was captured with, so it always binds the handle's
+  // own control table and force-history staging (allocated with the simulation and kept for its
+  // lifetime), whatever the current run set: before the first run those pointers are unset, and
+  // vl_sim_run_noalloc points them at its scratch area.
+  if (!s->d_ctl || !s->dhist) return set_error(VL_EINVAL, "internal: step graph without the run buffers");
+  StepCtl* const run_ctl0 = s->run_ctl;
+  double* const run_hist0 = s->run_hist;
+  s->run_ctl = s->d_ctl;
+  s->run_hist = s->dhist;
   hipGraph_t gph = nullptr;
-  VL_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
   vl_status st = VL_OK;
-  const int gs = group_steps(s), win = window_steps(s);
-  for (int k = 0; k < win && st == VL_OK; k += gs) st = some_steps(s, gs, s->d_kbase, k, false);
-  if (st == VL_OK) st = join_overlap(s);
-  if (st == VL_OK) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s->stream, s->d_kbase, (int64_t)win);
-  hipError_t e = hipStreamEndCapture(s->stream, &gph);
-  s->graph_cur[key] = s->cur;
+  if (e == hipSuccess) {
+    const int gs = group_steps(s);
+    for (int k = 0; k < steps && st == VL_OK; k += gs) st = some_steps(s, gs, s->d_kbase, k, false);
+    if (st == VL_OK) st = join_overlap(s);
+    if (st == VL_OK) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s->stream, s->d_kbase, (int64_t)steps);
+    e = hipStreamEndCapture(s->stream, &gph);
+  }
+  const int cur_after = s->cur;
   s->cur = cur0;  // capture recorded the steps without running them
   s->t = t0;
+  s->run_ctl = run_ctl0;
+  s->run_hist = run_hist0;
   if (st != VL_OK) {
     if (gph) (void)hipGraphDestroy(gph);
     return st;
   }
   if (e != hipSuccess) return set_error(VL_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  e = hipGraphInstantiate(&s->graph[key], gph, nullptr, nullptr, 0);
+  hipGraphExec_t ge = nullptr;
+  e = hipGraphInstantiate(&ge, gph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(gph);
-  if (e != hipSuccess) {
-    s->graph[key] = nullptr;
-    return set_error(VL_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-  }
-  *out = s->graph[key];
+  if (e != hipSuccess) return set_error(VL_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  s->graphs[key] = {ge, cur_after};
+  *out = ge;
+  *after = cur_after;
   return VL_OK;
+}
+
+static bool has_step_graph(const vl_sim* s, int steps) {
+  return s->graphs.count(std::pair<int, int>(s->cur * 2 + (int)(s->t & 1), steps)) != 0;
 }
 
 // Device buffers of a run: the control table and the force-history staging of chunks of up to
@@ -1169,15 +1193,26 @@ static vl_status run_steps(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double*
       VL_HIP(hipEventRecord(s->ev_chunk[0], s->stream));
     }
     int64_t k = 0;
-    const int win = window_steps(s);
-    if (graphs && m >= win) {
+    const int win = window_steps(s), gs = group_steps(s);
+    if (graphs && m >= gs) {
       VL_HIP(hipMemsetAsync(s->d_kbase, 0, sizeof(int64_t), s->stream));
+      int after = 0;
       for (; k + win <= m; k += win) {
         hipGraphExec_t ge;
-        VL_TRY(get_window_graph(s, &ge));
+        VL_TRY(get_step_graph(s, win, &ge, &after));
         VL_HIP(hipGraphLaunch(ge, s->stream));
-        s->cur = s->graph_cur[s->cur * 2 + (int)(s->t & 1)];
+        s->cur = after;
         s->t += win;
+      }
+      // the whole groups left, as one prepared graph
+      const int rest = (int)((m - k) / gs * gs);
+      if (rest > 0 && has_step_graph(s, rest)) {
+        hipGraphExec_t ge;
+        VL_TRY(get_step_graph(s, rest, &ge, &after));
+        VL_HIP(hipGraphLaunch(ge, s->stream));
+        s->cur = after;
+        s->t += rest;
+        k += rest;
       }
     }
     while (k < m) {  // remainder, launched directly
@@ -1397,6 +1432,42 @@ vl_status vl_sim_step(vl_sim* s, int64_t nsteps) {
 vl_status vl_sim_run(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist) {
   if (!s || (!ctl && nsteps > 0)) return set_error(VL_EINVAL, "null argument");
   return run_steps(s, nsteps, ctl, force_hist, RunBufs{s->d_ctl, s->dhist, RUN_CHUNK, true});
+}
+
+vl_status vl_sim_prepare(vl_sim* s, int64_t nsteps) {
+  if (!s || nsteps < 0) return set_error(VL_EINVAL, "vl_sim_prepare: null simulation or negative step count");
+  if (!s->use_graphs) return VL_OK;
+  // the state sequence of run_steps over nsteps, without running: whole windows, one graph of
+  // the whole groups left in each chunk, then single steps
+  const int cur0 = s->cur;
+  const int64_t t0 = s->t;
+  const int win = window_steps(s), gs = group_steps(s);
+  vl_status st = VL_OK;
+  for (int64_t k0 = 0; k0 < nsteps && st == VL_OK; k0 += RUN_CHUNK) {
+    const int64_t m = std::min<int64_t>(RUN_CHUNK, nsteps - k0);
+    int64_t k = 0;
+    hipGraphExec_t ge;
+    int after = 0;
+    for (; k + win <= m && st == VL_OK; k += win) {
+      st = get_step_graph(s, win, &ge, &after);
+      s->cur = after;
+      s->t += win;
+    }
+    const int rest = (int)((m - k) / gs * gs);
+    if (rest > 0 && st == VL_OK) {
+      st = get_step_graph(s, rest, &ge, &after);
+      s->cur = after;
+      s->t += rest;
+      k += rest;
+    }
+    for (; k < m; ++k) {  // single steps (launch_step's state change)
+      s->cur = (s->cur + 1) % s->nbuf;
+      s->t += 1;
+    }
+  }
+  s->cur = cur0;
+  s->t = t0;
+  return st;
 }
 
 vl_status vl_sim_run_noalloc(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force_hist) {

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "setup.h"
@@ -151,8 +153,10 @@ struct vl_sim {
   int64_t* d_kbase = nullptr;    // window counter for replayed step graphs
   int64_t* d_kzero = nullptr;    // a device zero (directly launched steps)
   bool use_graphs = true;
-  hipGraphExec_t graph[12] = {};  // by (current buffer, step parity)
-  int graph_cur[12] = {};           // current buffer after the window
+  // Replayed step graphs by (current buffer * 2 + step parity, steps): whole windows, and the
+  // shorter runs of whole groups that vl_sim_prepare instantiates; each with the current buffer
+  // after it.
+  std::map<std::pair<int, int>, std::pair<hipGraphExec_t, int>> graphs;
   int64_t hist_cap = 0;
   std::vector<double> last_force;  // Sim::force after the last step
   int64_t device_bytes = 0;
