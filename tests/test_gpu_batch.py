@@ -190,7 +190,7 @@ def test_batch_passes_equal_single_case_runs(gpu, monkeypatch, prec, k, rows):
         ch.close()
 
 
-@pytest.mark.parametrize("env", [{}, {"VL_BATCH_REGION": "1"}, {"VL_BATCH_REGION": "1", "VL_BATCH_OVERLAP": "0"},
+@pytest.mark.parametrize("env", [{}, {"VL_BATCH_BAND": "0"}, {"VL_BATCH_REGION": "1"}, {"VL_BATCH_REGION": "1", "VL_BATCH_OVERLAP": "0"},
                                  {"VL_BATCH_REGION": "1", "VL_BATCH_OVERLAP": "0", "VL_BATCH_REGION0": "after"},
                                  {"VL_BATCH_REGION": "1", "VL_BATCH_REGION_T": "256"},
                                  {"VL_BATCH_REGION": "1", "VL_BATCH_REGION_T": "1024"},
@@ -199,7 +199,8 @@ def test_batch_passes_equal_single_case_runs(gpu, monkeypatch, prec, k, rows):
 @pytest.mark.parametrize("prec", [api.F32, api.F64])
 def test_batch_group_schedules_equal_single_case_runs(gpu, monkeypatch, env, prec):
     """Every schedule of a group of K steps gives the bits of the same case run alone: the
-    boundary and listed kernels once per step (the default), and the per-case region kernel
+    boundary, listed and band kernels once per step with the bands' intermediate levels in side
+    arrays (the default) or in the pools (VL_BATCH_BAND=0), and the per-case region kernel
     (VL_BATCH_REGION=1) beside the next group's far pass or after the whole pass
     (VL_BATCH_OVERLAP=0, its step 0 beside the pass or after it), at 256, 512 and 1024 threads,
     on one stream, and without graphs."""

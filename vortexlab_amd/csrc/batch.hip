@@ -47,16 +47,64 @@ struct BatchCase {
   int32_t bnd_tile0;    // its first boundary tile
   int32_t absorbing;
   int32_t inflow;       // initial state when ramp_steps == 0
-  int32_t pad_;
+  int32_t band_w;       // columns of each band beside the inlet and outlet in side arrays (0: none)
   double inflow_u, inflow_h;
+  int64_t side_off;     // element offset of the case's band block in each side pool
+  int64_t bidx_off;     // offset of the case's band map
+  int32_t band_sny;     // rows per band column in the side arrays and the band map
+  int32_t pad2_;
 };
 
 constexpr int BT = 256;  // threads of both batch kernels
+
+// The bands of a case in K-step passes: the band_w columns beside the inlet (x = 1 .. band_w) and
+// beside the outlet (x = nx - 1 - band_w .. nx - 2), which no pass task covers.  Their
+// intermediate levels (1 .. K-1) live in a column-major side array of the case ([plane][band
+// column][row], rows padded to band_sny), so that a single step reads and writes whole lines
+// of them; level 0 is read from, and level K written to, the population pools like every other
+// node.  BandView reads or writes through the side array for band nodes and through the pool
+// for every other node; side == null: the pool everywhere.
+template <class T>
+struct BandView {
+  T* main;
+  int64_t plane;
+  int pitch;
+  T* side;
+  int bw, xr, sny;
+  __device__ inline T* at(int i, int x, int y) const {
+    if (side) {
+      const bool lo = x >= 1 && x <= bw, hi = x >= xr && x < xr + bw;
+      if (lo || hi) return side + ((int64_t)(i * 2 * bw + (lo ? x - 1 : bw + x - xr)) * sny + y);
+    }
+    return main + i * plane + ((int64_t)y * pitch + x);
+  }
+};
 
 __device__ inline double ramp_dev(uint64_t t, uint64_t steps) {  // src/cases/mod.rs:11-18
   if (steps == 0 || t >= steps) return 1.0;
   const double s = (double)t / (double)steps;
   return s * s * (3.0 - 2.0 * s);
+}
+
+// Controls of step t of a case (Channel::step, src/cases/channel.rs:136-147): the inflow ramp,
+// the body kick (host-tabulated), the outflow target.
+__device__ inline StepCtl case_controls(const BatchCase& cs, int64_t t, const double* __restrict__ kick_tab) {
+  StepCtl ctl;
+  const double r = ramp_dev((uint64_t)t, cs.ramp_steps);
+  ctl.vscale[0] = 1.0;
+  ctl.vscale[1] = r;
+  ctl.vscale[2] = (t >= cs.kick_start && t < cs.kick_start + cs.kick_dur) ? kick_tab[cs.kick_off + t - cs.kick_start]
+                                                                         : 0.0;
+  ctl.vscale[3] = 1.0;
+  ctl.target = cs.u_ref * r;
+  ctl.has_target = cs.absorbing && cs.t.np > 0;
+  return ctl;
+}
+
+template <class T>
+__device__ inline BandView<T> band_view(const BatchCase& cs, T* pool, T* side) {
+  return BandView<T>{pool + cs.pop_off, cs.g.plane, cs.g.pitch, cs.band_w > 0 && side ? side + cs.side_off : nullptr,
+                     cs.band_w, cs.g.nx - 1 - cs.band_w, cs.band_sny};
 }
 
 // Boundary tiles (256 boundary nodes of one case each) of one step: the tiles of the cases still
@@ -74,13 +122,18 @@ __global__ __launch_bounds__(BT) void k_batch_boundary(const typename S::T* __re
                                                        int t_local, const double* __restrict__ kick_tab,
                                                        double* __restrict__ hist, int32_t* __restrict__ status,
                                                        int64_t* __restrict__ done, double sqrt_cs2,
-                                                       const int32_t* __restrict__ first_at, int jpos) {
+                                                       const int32_t* __restrict__ first_at, int jpos,
+                                                       const int32_t* __restrict__ band_end_at,
+                                                       const typename S::T* __restrict__ side_rd) {
   __shared__ double red[BT];
   __shared__ double fsum[2];
   __shared__ int skip;
   const int64_t t = *tbase + t_local;
   const int nact = act[t];
   const int first = first_at ? first_at[t - jpos] : 0;
+  // band_end_at (steps of a group with side arrays): the tiles below band_end_at[t - jpos] belong
+  // to the cases in passes, whose band nodes the band kernel advances
+  const int band_end = band_end_at ? band_end_at[t - jpos] : 0;
   for (int i = first + (int)blockIdx.x; i < nact; i += gridDim.x) {
     const int ci = tcase[i];
     if (threadIdx.x == 0) skip = status[ci];
@@ -107,30 +160,28 @@ __global__ __launch_bounds__(BT) void k_batch_boundary(const typename S::T* __re
       }
     }
     if (t < cs.steps) {
-      StepCtl ctl;
-      const double r = ramp_dev((uint64_t)t, cs.ramp_steps);
-      ctl.vscale[0] = 1.0;
-      ctl.vscale[1] = r;
-      ctl.vscale[2] = (t >= cs.kick_start && t < cs.kick_start + cs.kick_dur) ? kick_tab[cs.kick_off + t - cs.kick_start]
-                                                                             : 0.0;
-      ctl.vscale[3] = 1.0;
-      ctl.target = cs.u_ref * r;
-      ctl.has_target = cs.absorbing && cs.t.np > 0;
+      const StepCtl ctl = case_controls(cs, t, kick_tab);
       const int b0 = tb0[i];
       double drho = 0.0;
       if (ctl.has_target && b0 < cs.t.npb)
         drho = outlet_offset<S>(cs.t.pvel + (t & 1) * np1, cs.t.np, ctl.target, sqrt_cs2, red);
       if (b0 == 0 && threadIdx.x == 0) *cs.t.drho = drho;
       const int b = b0 + (int)threadIdx.x;
+      const bool band = i < band_end && cs.band_w > 0;
       if (b < cs.t.nb) {
-        const typename S::T* a = A + cs.pop_off;
-        typename S::T* bb = B + cs.pop_off;
-        double* fc = cs.t.fcontrib + (t & 1) * ns2;
-        double* pvn = cs.t.pvel + ((t + 1) & 1) * np1;
-        if (cs.c.model == MODEL_BGK)
-          boundary_node<S, MODEL_BGK>(a, bb, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
-        else
-          boundary_node<S, MODEL_TRT>(a, bb, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
+        using T = typename S::T;
+        const int bx = cs.t.bx[b];
+        const bool skip_node = band && (bx <= cs.band_w || bx >= cs.g.nx - 1 - cs.band_w);
+        if (!skip_node) {
+          const BandView<const T> rv = band_view<const T>(cs, A, band ? side_rd : nullptr);
+          const MainView<T> wv{B + cs.pop_off, cs.g.plane, cs.g.pitch};
+          double* fc = cs.t.fcontrib + (t & 1) * ns2;
+          double* pvn = cs.t.pvel + ((t + 1) & 1) * np1;
+          if (cs.c.model == MODEL_BGK)
+            boundary_node_v<S, MODEL_BGK>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
+          else
+            boundary_node_v<S, MODEL_TRT>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
+        }
       }
     }
     __syncthreads();  // `red` is reused by the next tile
@@ -223,7 +274,8 @@ __global__ __launch_bounds__(W * TB_LANES) void k_batch_pass(const typename S::T
 // One single step of the interior nodes that no task covers, for the cases in K-step passes:
 // listed tile i holds count[i] <= BT nodes of case lcase[i] (packed y << 16 | x from
 // nodes + first[i]); the tiles of the cases that run the group starting at t - jpos are a prefix
-// of nlist_at[t - jpos] tiles.
+// of nlist_at[t - jpos] tiles.  With side arrays the lists hold no band node, and the band
+// neighbours of time t are read from side_rd (null at a group's first step: the pool).
 template <class S>
 __global__ __launch_bounds__(BT) void k_batch_listed(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
                                                      const BatchCase* __restrict__ cases,
@@ -231,19 +283,112 @@ __global__ __launch_bounds__(BT) void k_batch_listed(const typename S::T* __rest
                                                      const int32_t* __restrict__ lcount,
                                                      const uint32_t* __restrict__ nodes,
                                                      const int32_t* __restrict__ nlist_at, const int64_t* tbase,
-                                                     int t_local, int jpos, const int32_t* __restrict__ status) {
+                                                     int t_local, int jpos, const int32_t* __restrict__ status,
+                                                     const typename S::T* __restrict__ side_rd) {
+  using T = typename S::T;
   const int64_t t = *tbase + t_local;
   const int n = nlist_at[t - jpos];
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const int ci = lcase[i];
     if (status[ci]) continue;
     const BatchCase& cs = cases[ci];
-    const typename S::T* a = A + cs.pop_off;
-    typename S::T* b = B + cs.pop_off;
+    const int k = (int)threadIdx.x;
+    if (k >= lcount[i]) continue;
+    const uint32_t v = nodes[lfirst[i] + k];
+    const int x = (int)(v & 0xffffu), y = (int)(v >> 16);
+    const BandView<const T> rv = band_view<const T>(cs, A, side_rd);
+    const MainView<T> wv{B + cs.pop_off, cs.g.plane, cs.g.pitch};
     if (cs.c.model == MODEL_BGK)
-      listed_node<S, MODEL_BGK>((int)threadIdx.x, lcount[i], nodes + lfirst[i], a, b, cs.g, cs.c);
+      interior_node_v<S, MODEL_BGK>(x, y, rv, wv, cs.g, cs.c);
     else
-      listed_node<S, MODEL_TRT>((int)threadIdx.x, lcount[i], nodes + lfirst[i], a, b, cs.g, cs.c);
+      interior_node_v<S, MODEL_TRT>(x, y, rv, wv, cs.g, cs.c);
+  }
+}
+
+// One single step of the band nodes of the cases in K-step passes (see BandView): band tile i =
+// (case, band column, first row) covers BT rows of one band column, a node per thread, so each
+// load and store of a side array runs along a line.  bidx maps a band position to -2 (not a
+// fluid node), -1 (interior) or the boundary node's index.  Step jpos of the group reads the
+// band's time t from side_rd (the pool at the group's first step) and writes time t + 1 to
+// side_wr (the pool at its last step).  A tile of the outlet column computes the outlet
+// controller's offset as the boundary kernel's tiles do.
+template <class S>
+__global__ __launch_bounds__(BT) void k_batch_band(const typename S::T* __restrict__ A, typename S::T* __restrict__ B,
+                                                   const typename S::T* __restrict__ side_rd,
+                                                   typename S::T* __restrict__ side_wr,
+                                                   const BatchCase* __restrict__ cases, const int4* __restrict__ tiles,
+                                                   const int32_t* __restrict__ nband_at,
+                                                   const int32_t* __restrict__ bidx, const int64_t* tbase,
+                                                   int t_local, int jpos, const double* __restrict__ kick_tab,
+                                                   const int32_t* __restrict__ status, double sqrt_cs2) {
+  using T = typename S::T;
+  __shared__ double red[BT];
+  __shared__ int skip;
+  const int64_t t = *tbase + t_local;
+  const int n = nband_at[t - jpos];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int4 tl = tiles[i];  // case, band column, first row
+    const int ci = tl.x;
+    if (threadIdx.x == 0) skip = status[ci];
+    __syncthreads();
+    const int sk = skip;
+    __syncthreads();
+    if (sk) continue;
+    const BatchCase& cs = cases[ci];
+    const int bw = cs.band_w, xr = cs.g.nx - 1 - bw;
+    const int c = tl.y;
+    const int x = c < bw ? 1 + c : xr + (c - bw);
+    const int y = tl.z + (int)threadIdx.x;
+    const StepCtl ctl = case_controls(cs, t, kick_tab);
+    const int64_t np1 = max(cs.t.np, 1);
+    double drho = 0.0;
+    if (ctl.has_target && x == cs.g.nx - 2)  // the outlet column: its nodes use the controller
+      drho = outlet_offset<S>(cs.t.pvel + (t & 1) * np1, cs.t.np, ctl.target, sqrt_cs2, red);
+    const int32_t b = y < cs.band_sny ? bidx[cs.bidx_off + (int64_t)c * cs.band_sny + y] : -2;
+    if (b != -2) {
+      const BandView<const T> rv = band_view<const T>(cs, A, side_rd);
+      const BandView<T> wv = band_view<T>(cs, B, side_wr);
+      if (b >= 0) {
+        const int64_t ns2 = (int64_t)max(cs.t.nslots, 1) * 2;
+        double* fc = cs.t.fcontrib + (t & 1) * ns2;
+        double* pvn = cs.t.pvel + ((t + 1) & 1) * np1;
+        if (cs.c.model == MODEL_BGK)
+          boundary_node_v<S, MODEL_BGK>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
+        else
+          boundary_node_v<S, MODEL_TRT>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
+      } else {
+        if (cs.c.model == MODEL_BGK)
+          interior_node_v<S, MODEL_BGK>(x, y, rv, wv, cs.g, cs.c);
+        else
+          interior_node_v<S, MODEL_TRT>(x, y, rv, wv, cs.g, cs.c);
+      }
+    }
+    __syncthreads();  // `red` is reused by the next tile
+  }
+}
+
+// The band map of every case (built once): -2 for a non-fluid band position, -1 for an interior
+// node, then (second kernel) the index of each boundary node that lies in a band.
+__global__ __launch_bounds__(BT) void k_band_map_init(int32_t* __restrict__ bidx, const uint8_t* __restrict__ flags,
+                                                      const BatchCase* __restrict__ cases, int case0) {
+  const BatchCase& cs = cases[case0 + (int)blockIdx.y];
+  if (cs.band_w <= 0) return;
+  const int64_t n = (int64_t)2 * cs.band_w * cs.band_sny;
+  for (int64_t e = (int64_t)blockIdx.x * BT + threadIdx.x; e < n; e += (int64_t)gridDim.x * BT) {
+    const int c = (int)(e / cs.band_sny), y = (int)(e - (int64_t)c * cs.band_sny);
+    const int x = c < cs.band_w ? 1 + c : cs.g.nx - 1 - cs.band_w + (c - cs.band_w);
+    bidx[cs.bidx_off + e] = y < cs.g.ny && flags[cs.flag_off + (int64_t)y * cs.g.pitch + x] != 2 ? -1 : -2;
+  }
+}
+__global__ __launch_bounds__(BT) void k_band_map_bnd(int32_t* __restrict__ bidx, const BatchCase* __restrict__ cases,
+                                                     int case0) {
+  const BatchCase& cs = cases[case0 + (int)blockIdx.y];
+  if (cs.band_w <= 0) return;
+  const int xr = cs.g.nx - 1 - cs.band_w;
+  for (int b = (int)(blockIdx.x * BT + threadIdx.x); b < cs.t.nb; b += (int)(gridDim.x * BT)) {
+    const int x = cs.t.bx[b], y = cs.t.by[b];
+    const bool lo = x >= 1 && x <= cs.band_w, hi = x >= xr && x < xr + cs.band_w;
+    if (lo || hi) bidx[cs.bidx_off + (int64_t)(lo ? x - 1 : cs.band_w + x - xr) * cs.band_sny + y] = b;
   }
 }
 
@@ -456,6 +601,17 @@ struct vl_batch {
   // boundary tiles (a prefix, which the boundary kernel then skips).  clfirst / clcount: each
   // case's listed nodes in d_lnodes.  region0_side: step 0 of the region kernel beside the pass.
   bool region = false, region0_side = true;
+  // Side arrays of the bands (see BandView; the default with K-step passes, VL_BATCH_BAND=0
+  // keeps every node in the pools): bside[L - 1] holds the bands' level L = 1 .. K-1 of a group,
+  // bidx the band map, band tiles (case, band column, first row) in the cases' descending step
+  // order, nband_at[t] = the band tiles of the cases in passes for the group starting at t.
+  bool band = false;
+  void* bside[TB_MAX_K - 1] = {};
+  int32_t* d_bidx = nullptr;
+  int4* d_btile = nullptr;
+  int32_t* d_nband_at = nullptr;
+  std::vector<int32_t> h_nband_at;
+  int nband_tiles = 0;
   int region_threads = 512;
   int32_t *d_rcase = nullptr, *d_rn_at = nullptr, *d_bfirst_at = nullptr, *d_clfirst = nullptr,
           *d_clcount = nullptr;
@@ -480,8 +636,9 @@ struct vl_batch {
   std::vector<int32_t> ladder;  // grid sizes used when bulk_mult == 0
   int cur = 0;
   int64_t t = 0;
-  // (first pool, bulk grid, pass grid, listed or region grid, boundary grid, near pass grid) -> window
-  std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> graphs;
+  // (first pool, bulk grid, pass grid, listed or region grid, boundary grid, near pass grid,
+  // band grid) -> window
+  std::map<std::tuple<int, int, int, int, int, int, int>, hipGraphExec_t> graphs;
   bool use_graphs = true;
   double autotune_ms[3][4] = {};
   double last_run_seconds = 0.0;
@@ -585,7 +742,7 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, int 
   if (ev) VL_HIP(hipEventRecord(ev->bs, bs));
   hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, bs, A, B, b->dcases,
                      b->d_bcase, b->d_bb0, b->d_bact, tbase, t_local, b->d_kick, b->d_hist, b->d_status, b->d_done,
-                     SQRT_CS2_B, (const int32_t*)nullptr, 0);
+                     SQRT_CS2_B, (const int32_t*)nullptr, 0, (const int32_t*)nullptr, (const typename S::T*)nullptr);
   if (ev) VL_HIP(hipEventRecord(ev->be, bs));
   if (!b->serial) VL_HIP(hipEventRecord(b->ev_join, b->side));
   if (ev) VL_HIP(hipEventRecord(ev->ks, b->stream));
@@ -605,7 +762,7 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, int 
 // cases), the bulk tiles, and with the region kernel the boundary tiles of the cases that end
 // inside the group.
 struct GroupGrids {
-  int pass, listed, bulk, bnd, pass_near;
+  int pass, listed, bulk, bnd, pass_near, band;
 };
 
 template <class S, int W, int V, int D, int K>
@@ -647,11 +804,21 @@ vl_status launch_pass_k(vl_batch* b, const vl_batch::TaskList& tl, hipStream_t s
 template <class S>
 void launch_group_phase(vl_batch* b, hipStream_t bs, hipStream_t ms, const typename S::T* A, typename S::T* B,
                         const int64_t* tbase, int t_local, int j, const GroupGrids& gg) {
+  using T = typename S::T;
+  // the bands' time t + j: the pool at the group's first step, then the side array of level j;
+  // time t + j + 1: the side array of level j + 1, the pool at the group's last step
+  const T* side_rd = b->band && j > 0 ? (const T*)b->bside[j - 1] : nullptr;
+  T* side_wr = b->band && j + 1 < b->pass_k ? (T*)b->bside[j] : nullptr;
   hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, bs, A, B, b->dcases, b->d_bcase,
                      b->d_bb0, b->d_bact, tbase, t_local + j, b->d_kick, b->d_hist, b->d_status, b->d_done,
-                     SQRT_CS2_B, (const int32_t*)nullptr, 0);
+                     SQRT_CS2_B, (const int32_t*)nullptr, j, b->band ? (const int32_t*)b->d_bfirst_at : nullptr,
+                     side_rd);
   hipLaunchKernelGGL(k_batch_listed<S>, dim3((unsigned)gg.listed), dim3(BT), 0, ms, A, B, b->dcases, b->d_lcase,
-                     b->d_lfirst, b->d_lcount, b->d_lnodes, b->d_ln_at, tbase, t_local + j, j, b->d_status);
+                     b->d_lfirst, b->d_lcount, b->d_lnodes, b->d_ln_at, tbase, t_local + j, j, b->d_status, side_rd);
+  if (b->band)
+    hipLaunchKernelGGL(k_batch_band<S>, dim3((unsigned)gg.band), dim3(BT), 0, ms, A, B, side_rd, side_wr, b->dcases,
+                       b->d_btile, b->d_nband_at, b->d_bidx, tbase, t_local + j, j, b->d_kick, b->d_status,
+                       SQRT_CS2_B);
   hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)gg.bulk), dim3(BT), 0, ms, A, B, b->dflags, b->dcases,
                      b->d_kcase, b->d_ky0, b->d_kx0, b->d_kact, tbase, t_local + j, b->tw_log2, b->d_status,
                      (const int32_t*)b->d_kfirst_at, j);
@@ -695,7 +862,8 @@ void launch_finishing_step(vl_batch* b, hipStream_t st, const typename S::T* A, 
                            const int64_t* tbase, int t_local, int j, const GroupGrids& gg) {
   hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)gg.bnd), dim3(BT), 0, st, A, B, b->dcases, b->d_bcase,
                      b->d_bb0, b->d_bact, tbase, t_local + j, b->d_kick, b->d_hist, b->d_status, b->d_done,
-                     SQRT_CS2_B, (const int32_t*)b->d_bfirst_at, j);
+                     SQRT_CS2_B, (const int32_t*)b->d_bfirst_at, j, (const int32_t*)nullptr,
+                     (const typename S::T*)nullptr);
   hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)gg.bulk), dim3(BT), 0, st, A, B, b->dflags, b->dcases,
                      b->d_kcase, b->d_ky0, b->d_kx0, b->d_kact, tbase, t_local + j, b->tw_log2, b->d_status,
                      (const int32_t*)b->d_kfirst_at, j);
@@ -896,6 +1064,7 @@ GroupGrids group_grids_at(const vl_batch* b, int64_t t, int groups) {
     }
   gg.bulk = ladder_ceil(bulk, b->nbulk_tiles);
   gg.bnd = b->region ? ladder_ceil(bnd, b->nbnd_tiles) : 0;
+  gg.band = b->band ? ladder_ceil(at(b->h_nband_at, t), b->nband_tiles) : 0;
   return gg;
 }
 
@@ -935,7 +1104,7 @@ vl_status launch_trial_step(vl_batch* b, int grid, int tw_log2, const int32_t* k
   }
   hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, bs, A, B, b->dcases, b->d_bcase,
                      b->d_bb0, b->d_bact, t0, 0, b->d_kick, b->d_hist, b->d_status, b->d_done, SQRT_CS2_B,
-                     (const int32_t*)nullptr, 0);
+                     (const int32_t*)nullptr, 0, (const int32_t*)nullptr, (const typename S::T*)nullptr);
   if (!b->serial) VL_HIP(hipEventRecord(b->ev_join, b->side));
   hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)grid), dim3(BT), 0, b->stream, A, B, b->dflags, b->dcases, kc,
                      ky, kx, kact, t0, 0, tw_log2, b->d_status, (const int32_t*)nullptr, 0);
@@ -961,6 +1130,8 @@ void free_batch(vl_batch* b) {
     if (p.body) vl_shape_free(p.body);
   for (void* p : b->allocs) (void)hipFree(p);
   for (void* p : b->pool)
+    if (p) (void)hipFree(p);
+  for (void* p : b->bside)
     if (p) (void)hipFree(p);
   for (auto& kv : b->graphs)
     if (kv.second) (void)hipGraphExecDestroy(kv.second);
@@ -1163,6 +1334,39 @@ vl_status build(vl_batch* b) {
       if (st[i] != VL_OK) return set_error(st[i], "case " + std::to_string(i) + ": " + err[i]);
   }
   lap(1);
+  // The region kernel is bit-identical but measured slower on the 512-case sweep (22.8 cases/s
+  // after the pass, 21.7 beside the next far pass, against 25.4 with the per-step kernels): one
+  // workgroup per case cannot hide the partial-line traffic of the nodes beside the inlet and
+  // outlet, and beside the far pass that traffic slows the pass more than the overlap hides.
+  b->region = false;
+  if (const char* e = std::getenv("VL_BATCH_REGION")) b->region = b->pass_k > 1 && e[0] == '1';
+  if (const char* e = std::getenv("VL_BATCH_REGION_T")) b->region_threads = std::atoi(e);
+  if (b->region_threads != 256 && b->region_threads != 1024) b->region_threads = 512;
+  if (const char* e = std::getenv("VL_BATCH_REGION0")) b->region0_side = std::string(e) != "after";
+  if (const char* e = std::getenv("VL_BATCH_OVERLAP")) b->overlap = e[0] != '0';
+  // Band side arrays: with K-step passes and the per-step kernels.  A case's bands are its K
+  // columns beside the inlet and the outlet, which no pass task can cover (a covered node's
+  // K - 1 columns on each side are interior, and column 1 and nx - 2 are boundary columns); a
+  // case too narrow for two separate bands keeps every node in the pools.
+  b->band = b->pass_k > 1 && !b->region;
+  if (const char* e = std::getenv("VL_BATCH_BAND")) b->band = b->band && e[0] != '0';
+  int64_t side_elems = 0, bidx_len = 0;
+  for (size_t i = 0; i < nc; ++i) {
+    BatchCase& c = b->hcases[i];
+    c.band_w = b->band && c.g.nx >= 2 * b->pass_k + 4 ? b->pass_k : 0;
+    c.band_sny = (int32_t)rup(c.g.ny, 64);
+    c.side_off = side_elems;
+    c.bidx_off = bidx_len;
+    if (c.band_w > 0) {
+      side_elems += (int64_t)Q * 2 * c.band_w * c.band_sny;
+      bidx_len += (int64_t)2 * c.band_w * c.band_sny;
+    }
+  }
+  if (b->band)
+    for (int L = 1; L < b->pass_k; ++L) {
+      VL_HIP(hipMalloc(&b->bside[L - 1], (size_t)std::max<int64_t>(side_elems, 1) * b->elem));
+      b->device_bytes += std::max<int64_t>(side_elems, 1) * (int64_t)b->elem;
+    }
   b->nbuf = b->pass_k > 1 ? b->pass_k + 1 : 2;
   for (int j = 0; j < b->nbuf; ++j) VL_HIP(hipMalloc(&b->pool[j], (size_t)pop_elems * b->elem));
   b->device_bytes += b->nbuf * pop_elems * (int64_t)b->elem;
@@ -1422,6 +1626,16 @@ vl_status build(vl_batch* b) {
     }
     VL_HIP(hipGetLastError());
   }
+  if (b->band) {
+    VL_TRYB(dalloc(b, b->d_bidx, (size_t)std::max<int64_t>(bidx_len, 1)));
+    for (size_t c0 = 0; c0 < nc; c0 += 65535) {
+      const unsigned rows = (unsigned)std::min<size_t>(65535, nc - c0);
+      hipLaunchKernelGGL(k_band_map_init, dim3(16, rows), dim3(BT), 0, b->stream, b->d_bidx, b->dflags, b->dcases,
+                         (int)c0);
+      hipLaunchKernelGGL(k_band_map_bnd, dim3(16, rows), dim3(BT), 0, b->stream, b->d_bidx, b->dcases, (int)c0);
+      VL_HIP(hipGetLastError());
+    }
+  }
   VL_HIP(hipStreamSynchronize(b->stream));
   lap(2);
   // ---- autotune the bulk tile shape and the grid policy on the real batch ----
@@ -1509,7 +1723,9 @@ vl_status build(vl_batch* b) {
     std::vector<int4> task, ftask, ntask;
     std::vector<int32_t> tcase, ftcase, ntcase, lcase, lfirst, lcount;
     std::vector<uint32_t> lnodes;
-    std::vector<int32_t> tend(nc, 0), ftend(nc, 0), ntend(nc, 0), lend(nc, 0), clfirst(nc, 0), clcount(nc, 0);
+    std::vector<int32_t> tend(nc, 0), ftend(nc, 0), ntend(nc, 0), lend(nc, 0), clfirst(nc, 0), clcount(nc, 0),
+        bend(nc, 0);
+    std::vector<int4> btile;
     auto by_row = [](const int4& a, const int4& c) { return a.y != c.y ? a.y < c.y : a.x < c.x; };
     if (b->pass_k > 1)
       for (int ci : order) {
@@ -1532,14 +1748,28 @@ vl_status build(vl_batch* b) {
         }
         ftend[(size_t)ci] = (int32_t)ftask.size();
         ntend[(size_t)ci] = (int32_t)ntask.size();
-        for (size_t k = 0; k < pl.listed.size(); k += BT) {
+        // with side arrays the band kernel takes the band columns' nodes: the listed tiles hold
+        // the others
+        const BatchCase& bc_ = b->hcases[(size_t)ci];
+        std::vector<uint32_t> lst;
+        lst.reserve(pl.listed.size());
+        for (uint32_t v : pl.listed) {
+          const int x = (int)(v & 0xffffu);
+          if (bc_.band_w > 0 && (x <= bc_.band_w || x >= bc_.g.nx - 1 - bc_.band_w)) continue;
+          lst.push_back(v);
+        }
+        for (size_t k = 0; k < lst.size(); k += BT) {
           lcase.push_back(ci);
           lfirst.push_back((int32_t)(lnodes.size() + k));
-          lcount.push_back((int32_t)std::min<size_t>(BT, pl.listed.size() - k));
+          lcount.push_back((int32_t)std::min<size_t>(BT, lst.size() - k));
         }
         clfirst[(size_t)ci] = (int32_t)lnodes.size();
-        clcount[(size_t)ci] = (int32_t)pl.listed.size();
-        lnodes.insert(lnodes.end(), pl.listed.begin(), pl.listed.end());
+        clcount[(size_t)ci] = (int32_t)lst.size();
+        lnodes.insert(lnodes.end(), lst.begin(), lst.end());
+        if (bc_.band_w > 0)
+          for (int c = 0; c < 2 * bc_.band_w; ++c)
+            for (int y0 = 0; y0 < bc_.g.ny; y0 += BT) btile.push_back(int4{ci, c, y0, 0});
+        bend[(size_t)ci] = (int32_t)btile.size();
         tend[(size_t)ci] = (int32_t)task.size();
         lend[(size_t)ci] = (int32_t)lcase.size();
         b->pass_nodes += pl.covered;
@@ -1555,6 +1785,7 @@ vl_status build(vl_batch* b) {
     b->h_kfirst_at.assign(nt, 0);
     b->h_rn_at.assign(nt, 0);
     b->h_bfirst_at.assign(nt, 0);
+    b->h_nband_at.assign(nt, 0);
     if (b->pass_k > 1)
       for (size_t tt = 0; tt < nt; ++tt) {
         int32_t ncase = 0;
@@ -1566,6 +1797,7 @@ vl_status build(vl_batch* b) {
           b->h_ln_at[tt] = lend[(size_t)ci];
           b->h_kfirst_at[tt] = kend[(size_t)ci];
           b->h_bfirst_at[tt] = bnd_end[(size_t)ci];
+          b->h_nband_at[tt] = bend[(size_t)ci];
           b->h_rn_at[tt] = ++ncase;
         }
       }
@@ -1604,17 +1836,10 @@ vl_status build(vl_batch* b) {
     VL_TRYB(dupload(b, b->d_bfirst_at, b->h_bfirst_at));
     VL_TRYB(dupload(b, b->d_clfirst, clfirst));
     VL_TRYB(dupload(b, b->d_clcount, clcount));
+    b->nband_tiles = (int)btile.size();
+    VL_TRYB(dupload(b, b->d_btile, btile));
+    VL_TRYB(dupload(b, b->d_nband_at, b->h_nband_at));
   }
-  // The region kernel is bit-identical but measured slower on the 512-case sweep (22.8 cases/s
-  // after the pass, 21.7 beside the next far pass, against 25.4 with the per-step kernels): one
-  // workgroup per case cannot hide the partial-line traffic of the nodes beside the inlet and
-  // outlet, and beside the far pass that traffic slows the pass more than the overlap hides.
-  b->region = false;
-  if (const char* e = std::getenv("VL_BATCH_REGION")) b->region = b->pass_k > 1 && e[0] == '1';
-  if (const char* e = std::getenv("VL_BATCH_REGION_T")) b->region_threads = std::atoi(e);
-  if (b->region_threads != 256 && b->region_threads != 1024) b->region_threads = 512;
-  if (const char* e = std::getenv("VL_BATCH_REGION0")) b->region0_side = std::string(e) != "after";
-  if (const char* e = std::getenv("VL_BATCH_OVERLAP")) b->overlap = e[0] != '0';
   b->use_graphs = true;
   if (const char* e = std::getenv("VL_GRAPHS")) b->use_graphs = e[0] != '0';
   lap(3);
@@ -1628,9 +1853,9 @@ int window_steps(const vl_batch* b) { return b->pass_k > 1 ? GRAPH_W / b->pass_k
 vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
   const int wsteps = window_steps(b);
   const int K = b->pass_k;
-  GroupGrids gg{0, 0, 0, 0, 0};
+  GroupGrids gg{0, 0, 0, 0, 0, 0};
   const int grid = K > 1 ? (gg = group_grids_at(b, b->t, wsteps / K)).bulk : bulk_grid_at(b, b->t);
-  const auto key = std::make_tuple(b->cur, grid, gg.pass, gg.listed, gg.bnd, gg.pass_near);
+  const auto key = std::make_tuple(b->cur, grid, gg.pass, gg.listed, gg.bnd, gg.pass_near, gg.band);
   auto found = b->graphs.find(key);
   if (found != b->graphs.end()) {
     *out = found->second;

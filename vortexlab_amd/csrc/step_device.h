@@ -96,6 +96,34 @@ __device__ inline void collide(typename S::C* f, const Coll& c) {
 
 __device__ inline int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
+// Where a node's populations live: `at(i, x, y)` is the address of population i of node (x, y).
+// MainView: a population array in the pitched layout, as every single-domain kernel reads and
+// writes it.  (The batch also sends the columns beside an inlet and an outlet to a side array,
+// through a view of its own with the same interface.)
+template <class T>
+struct MainView {
+  T* base;
+  int64_t plane;
+  int pitch;
+  __device__ inline T* at(int i, int x, int y) const { return base + i * plane + ((int64_t)y * pitch + x); }
+};
+
+// Pull the nine populations of node (x, y) from their upstream neighbours through a view.
+template <class S, class V>
+__device__ inline void pull_v(const V& v, const Geom& g, int x, int y, typename S::C* f) {
+  const int xm = wrapi(x - 1, g.nx), xp = wrapi(x + 1, g.nx);
+  const int ym = wrapi(y - 1, g.ny), yp = wrapi(y + 1, g.ny);
+  f[0] = S::ld(v.at(0, x, y));
+  f[1] = S::ld(v.at(1, xm, y));
+  f[2] = S::ld(v.at(2, x, ym));
+  f[3] = S::ld(v.at(3, xp, y));
+  f[4] = S::ld(v.at(4, x, yp));
+  f[5] = S::ld(v.at(5, xm, ym));
+  f[6] = S::ld(v.at(6, xp, ym));
+  f[7] = S::ld(v.at(7, xp, yp));
+  f[8] = S::ld(v.at(8, xm, yp));
+}
+
 // Pull the nine populations of node (x, y) from their upstream neighbours in A.
 template <class S>
 __device__ inline void pull(const typename S::T* __restrict__ A, const Geom& g, int x, int y,
@@ -274,21 +302,20 @@ struct LinkFields {
 };
 
 // One boundary node (src/sim.rs:456-507): pull, replace the populations that would come out
-// of non-fluid cells by the link rules in link order, collide, write B; each force link
-// stores its momentum-exchange term.
-template <class S, int M>
-__device__ inline void boundary_node(const typename S::T* __restrict__ A, typename S::T* __restrict__ B, const Geom& g,
-                              const Coll& c, const DevTables& t, const StepCtl& ctl, double drho, double sqrt_cs2,
-                              int b, double* __restrict__ fc, double* __restrict__ pv_next) {
+// of non-fluid cells by the link rules in link order, collide, write; each force link stores
+// its momentum-exchange term.  Reads go through the view rv (time t), writes through wv.
+template <class S, int M, class RV, class WV>
+__device__ inline void boundary_node_v(const RV& rv, const WV& wv, const Geom& g, const Coll& c, const DevTables& t,
+                                       const StepCtl& ctl, double drho, double sqrt_cs2, int b,
+                                       double* __restrict__ fc, double* __restrict__ pv_next) {
   using C = typename S::C;
   const int x = t.bx[b], y = t.by[b];
-  const int64_t idx = (int64_t)y * g.pitch + x;
   C f[Q], own[Q];
-  pull<S>(A, g, x, y, f);
-  // This node's own populations in A (what leaves it along each direction), loaded together
-  // with the pull so that no link waits on a load of its own.
+  pull_v<S>(rv, g, x, y, f);
+  // This node's own populations at time t (what leaves it along each direction), loaded
+  // together with the pull so that no link waits on a load of its own.
 #pragma unroll
-  for (int k = 0; k < Q; ++k) own[k] = S::ld(A + k * g.plane + idx);
+  for (int k = 0; k < Q; ++k) own[k] = S::ld(rv.at(k, x, y));
   // Local velocity for anti-bounce-back links, computed once (the reference recomputes the
   // same value per link).
   bool have_u = false;
@@ -316,7 +343,7 @@ __device__ inline void boundary_node(const typename S::T* __restrict__ A, typena
           back = k * (out - corr) + (1.0 - k) * own[i];
         } else if (L.lflags & 2) {
           const int bxn = wrapi(x - cx(j), g.nx), byn = wrapi(y - cy(j), g.ny);
-          back = 2.0 * q * out + (1.0 - 2.0 * q) * A[j * g.plane + (int64_t)byn * g.pitch + bxn] - corr;
+          back = 2.0 * q * out + (1.0 - 2.0 * q) * S::ld(rv.at(j, bxn, byn)) - corr;
         } else {
           back = out - corr;
         }
@@ -330,8 +357,7 @@ __device__ inline void boundary_node(const typename S::T* __restrict__ A, typena
           back = k * (out - corr) + (1.0f - k) * own[i];
         } else if (L.lflags & 2) {
           const int bxn = wrapi(x - cx(j), g.nx), byn = wrapi(y - cy(j), g.ny);
-          back = 2.0f * qf * out + (1.0f - 2.0f * qf) * S::ld(A + j * g.plane + (int64_t)byn * g.pitch + bxn) -
-                 corr;
+          back = 2.0f * qf * out + (1.0f - 2.0f * qf) * S::ld(rv.at(j, bxn, byn)) - corr;
         } else {
           back = out - corr;
         }
@@ -379,10 +405,20 @@ __device__ inline void boundary_node(const typename S::T* __restrict__ A, typena
   }
   collide<S, M>(f, c);
 #pragma unroll
-  for (int i = 0; i < Q; ++i) S::st(B + i * g.plane + idx, f[i]);
+  for (int i = 0; i < Q; ++i) S::st(wv.at(i, x, y), f[i]);
   if (b < t.npb) {  // pressure-boundary node b is outlet-controller node b: its velocity next step
     pv_next[b] = normal_velocity_of<S>(f, c, t.pnx[b], t.pny[b]);  // f is storage-exact
   }
+}
+
+// The same on population arrays in the pitched layout: reads A (time t), writes B.
+template <class S, int M>
+__device__ inline void boundary_node(const typename S::T* __restrict__ A, typename S::T* __restrict__ B, const Geom& g,
+                                     const Coll& c, const DevTables& t, const StepCtl& ctl, double drho,
+                                     double sqrt_cs2, int b, double* __restrict__ fc, double* __restrict__ pv_next) {
+  using T = typename S::T;
+  boundary_node_v<S, M>(MainView<const T>{A, g.plane, g.pitch}, MainView<T>{B, g.plane, g.pitch}, g, c, t, ctl, drho,
+                        sqrt_cs2, b, fc, pv_next);
 }
 
 // Workgroup `blk` of the boundary part of a step.
@@ -982,6 +1018,16 @@ __device__ inline void listed_node(int k, int n, const uint32_t* __restrict__ no
   const int64_t idx = (int64_t)y * g.pitch + x;
 #pragma unroll
   for (int i = 0; i < Q; ++i) S::st(B + i * g.plane + idx, f[i]);
+}
+
+// The single-step update of interior node (x, y) through views: reads rv (time t), writes wv.
+template <class S, int M, class RV, class WV>
+__device__ inline void interior_node_v(int x, int y, const RV& rv, const WV& wv, const Geom& g, const Coll& c) {
+  typename S::C f[Q];
+  pull_v<S>(rv, g, x, y, f);
+  collide<S, M>(f, c);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) S::st(wv.at(i, x, y), f[i]);
 }
 
 }  // namespace vl
