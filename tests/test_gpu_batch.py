@@ -129,17 +129,21 @@ def test_read_case_mid_run(gpu):
 
 
 def test_partial_runs_resume(gpu):
-    """Running in pieces (any split into windows and single steps) gives the same bits."""
+    """Running in pieces (any split into windows, groups and single steps) gives the same bits:
+    single steps between groups leave the bands' level-0 side arrays behind the pools, so the
+    next group reads its bands from the pools."""
     cases = small_cases(seed=13, n=3)
     whole = run_batch(gpu, cases, api.F32)
     b = api.Batch(gpu, api.F32)
     for c in cases:
         b.add(c.spec, c.steps, c.d)
     b.build()
+    assert b.info().pass_steps == 5
     for piece in (7, 33, 64, 5, 100000):
         b.run(piece)
-    for k in range(len(cases)):
+    for k, c in enumerate(cases):
         assert np.array_equal(whole.result(k)[0], b.result(k)[0])
+        assert np.array_equal(whole.read_case(k, c.spec.nx, c.spec.ny), b.read_case(k, c.spec.nx, c.spec.ny))
 
 
 def wide_cases(seed, n=3):
@@ -190,7 +194,8 @@ def test_batch_passes_equal_single_case_runs(gpu, monkeypatch, prec, k, rows):
         ch.close()
 
 
-@pytest.mark.parametrize("env", [{}, {"VL_BATCH_BAND": "0"}, {"VL_BATCH_REGION": "1"}, {"VL_BATCH_REGION": "1", "VL_BATCH_OVERLAP": "0"},
+@pytest.mark.parametrize("env", [{}, {"VL_BATCH_BAND": "0"}, {"VL_BATCH_THIRD": "0"}, {"VL_BATCH_SERIAL": "1"},
+                                 {"VL_GRAPHS": "0"}, {"VL_BATCH_REGION": "1"}, {"VL_BATCH_REGION": "1", "VL_BATCH_OVERLAP": "0"},
                                  {"VL_BATCH_REGION": "1", "VL_BATCH_OVERLAP": "0", "VL_BATCH_REGION0": "after"},
                                  {"VL_BATCH_REGION": "1", "VL_BATCH_REGION_T": "256"},
                                  {"VL_BATCH_REGION": "1", "VL_BATCH_REGION_T": "1024"},
@@ -200,7 +205,9 @@ def test_batch_passes_equal_single_case_runs(gpu, monkeypatch, prec, k, rows):
 def test_batch_group_schedules_equal_single_case_runs(gpu, monkeypatch, env, prec):
     """Every schedule of a group of K steps gives the bits of the same case run alone: the
     boundary, listed and band kernels once per step with the bands' intermediate levels in side
-    arrays (the default) or in the pools (VL_BATCH_BAND=0), and the per-case region kernel
+    arrays (the default: the band kernel on a stream of its own after a group's first step, or
+    with VL_BATCH_THIRD=0 on the main stream) or in the pools (VL_BATCH_BAND=0), on one stream
+    and without graphs, and the per-case region kernel
     (VL_BATCH_REGION=1) beside the next group's far pass or after the whole pass
     (VL_BATCH_OVERLAP=0, its step 0 beside the pass or after it), at 256, 512 and 1024 threads,
     on one stream, and without graphs."""

@@ -392,6 +392,40 @@ __global__ __launch_bounds__(BT) void k_band_map_bnd(int32_t* __restrict__ bidx,
   }
 }
 
+// The bands' level K of a group, which its last step wrote into the level-0 side array, copied
+// into the pool (the next group's passes read it there as halo, and single steps outside groups
+// read every node there).  Copy tile i = (case, first row) covers BT / (2 band_w) rows of all
+// 2 band_w band columns, one node per thread with the columns of a row on adjacent threads, so
+// a plane's row segment on each side is one store instruction rather than one per column.
+template <class T>
+__global__ __launch_bounds__(BT) void k_band_to_pool(T* __restrict__ B, const T* __restrict__ side0,
+                                                     const BatchCase* __restrict__ cases, const int2* __restrict__ tiles,
+                                                     const int32_t* __restrict__ ncopy_at,
+                                                     const int32_t* __restrict__ bidx, const int64_t* tbase,
+                                                     int t_local, int jpos, const int32_t* __restrict__ status) {
+  const int64_t t = *tbase + t_local;
+  const int n = ncopy_at[t - jpos];
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const int2 tl = tiles[i];  // case, first row
+    const int ci = tl.x;
+    if (status[ci]) continue;  // uniform over the workgroup; no barrier in the loop
+    const BatchCase& cs = cases[ci];
+    const int bw = cs.band_w, bw2 = 2 * bw;
+    const int r = (int)threadIdx.x / bw2, c = (int)threadIdx.x - r * bw2;
+    const int y = tl.y + r;
+    if (r >= BT / bw2 || y >= cs.g.ny) continue;
+    if (bidx[cs.bidx_off + (int64_t)c * cs.band_sny + y] == -2) continue;  // not a fluid node
+    const int x = c < bw ? 1 + c : cs.g.nx - 1 - bw + (c - bw);
+    const T* src = side0 + cs.side_off + (int64_t)c * cs.band_sny + y;
+    T* dst = B + cs.pop_off + (int64_t)y * cs.g.pitch + x;
+    T v[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) v[k] = src[(int64_t)k * bw2 * cs.band_sny];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) dst[(int64_t)k * cs.g.plane] = v[k];
+  }
+}
+
 // The K + 1 population pools of a group of K steps, in time order (pool j holds time t + j).
 template <class T>
 struct PoolSet {
@@ -605,13 +639,25 @@ struct vl_batch {
   // keeps every node in the pools): bside[L - 1] holds the bands' level L = 1 .. K-1 of a group,
   // bidx the band map, band tiles (case, band column, first row) in the cases' descending step
   // order, nband_at[t] = the band tiles of the cases in passes for the group starting at t.
+  // bside0 holds the bands' level K of the last group (the next group's level 0), written
+  // beside the pool at a group's last step; side0_ok says that it matches the pools (no single
+  // step ran since), so a group's first step reads its bands there.  Copy tiles (case, first
+  // row; ncopy_at[t] per group start like nband_at) write level K into the pool row by row.
   bool band = false;
   void* bside[TB_MAX_K - 1] = {};
+  void* bside0 = nullptr;
+  bool side0_ok = false;
   int32_t* d_bidx = nullptr;
   int4* d_btile = nullptr;
   int32_t* d_nband_at = nullptr;
   std::vector<int32_t> h_nband_at;
   int nband_tiles = 0;
+  int2* d_ctile = nullptr;
+  int32_t* d_ncopy_at = nullptr;
+  std::vector<int32_t> h_ncopy_at;
+  int ncopy_tiles = 0;
+  hipStream_t third = nullptr;  // the band kernel's stream in the steps after a group's first
+  hipEvent_t ev_join3 = nullptr;
   int region_threads = 512;
   int32_t *d_rcase = nullptr, *d_rn_at = nullptr, *d_bfirst_at = nullptr, *d_clfirst = nullptr,
           *d_clcount = nullptr;
@@ -637,8 +683,8 @@ struct vl_batch {
   int cur = 0;
   int64_t t = 0;
   // (first pool, bulk grid, pass grid, listed or region grid, boundary grid, near pass grid,
-  // band grid) -> window
-  std::map<std::tuple<int, int, int, int, int, int, int>, hipGraphExec_t> graphs;
+  // band grid, copy grid, side0_ok) -> window
+  std::map<std::tuple<int, int, int, int, int, int, int, int, int>, hipGraphExec_t> graphs;
   bool use_graphs = true;
   double autotune_ms[3][4] = {};
   double last_run_seconds = 0.0;
@@ -755,6 +801,7 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, int 
   VL_HIP(hipGetLastError());
   b->cur = (b->cur + 1) % b->nbuf;
   b->t += 1;
+  b->side0_ok = false;  // the pools' bands moved on; the level-0 side array did not
   return VL_OK;
 }
 
@@ -762,7 +809,7 @@ vl_status launch_batch_step(vl_batch* b, const int64_t* tbase, int t_local, int 
 // cases), the bulk tiles, and with the region kernel the boundary tiles of the cases that end
 // inside the group.
 struct GroupGrids {
-  int pass, listed, bulk, bnd, pass_near, band;
+  int pass, listed, bulk, bnd, pass_near, band, copy;
 };
 
 template <class S, int W, int V, int D, int K>
@@ -800,25 +847,34 @@ vl_status launch_pass_k(vl_batch* b, const vl_batch::TaskList& tl, hipStream_t s
 
 // The single-step part of step j of a group (reads pool cur + j, writes pool cur + j + 1): the
 // boundary kernel on stream `bs`, the listed nodes and the bulk tiles of the cases that end inside
-// the group on stream `ms`.
+// the group on stream `ms`, the band nodes on stream `ts`.
 template <class S>
-void launch_group_phase(vl_batch* b, hipStream_t bs, hipStream_t ms, const typename S::T* A, typename S::T* B,
-                        const int64_t* tbase, int t_local, int j, const GroupGrids& gg) {
+void launch_group_phase(vl_batch* b, hipStream_t bs, hipStream_t ms, hipStream_t ts, const typename S::T* A,
+                        typename S::T* B, const int64_t* tbase, int t_local, int j, const GroupGrids& gg) {
   using T = typename S::T;
-  // the bands' time t + j: the pool at the group's first step, then the side array of level j;
-  // time t + j + 1: the side array of level j + 1, the pool at the group's last step
-  const T* side_rd = b->band && j > 0 ? (const T*)b->bside[j - 1] : nullptr;
-  T* side_wr = b->band && j + 1 < b->pass_k ? (T*)b->bside[j] : nullptr;
+  const int K = b->pass_k;
+  // the bands' time t + j: at the group's first step the level-0 side array (the pool when a
+  // single step ran since the last group), then the side array of level j; time t + j + 1: the
+  // side array of level j + 1, at the group's last step the level-0 side array, copied into
+  // the pool after the band kernel
+  const T* side_rd = !b->band ? nullptr
+                     : j > 0  ? (const T*)b->bside[j - 1]
+                              : (b->side0_ok ? (const T*)b->bside0 : nullptr);
+  T* side_wr = !b->band ? nullptr : j + 1 < K ? (T*)b->bside[j] : (T*)b->bside0;
   hipLaunchKernelGGL(k_batch_boundary<S>, dim3((unsigned)b->grid_bnd), dim3(BT), 0, bs, A, B, b->dcases, b->d_bcase,
                      b->d_bb0, b->d_bact, tbase, t_local + j, b->d_kick, b->d_hist, b->d_status, b->d_done,
                      SQRT_CS2_B, (const int32_t*)nullptr, j, b->band ? (const int32_t*)b->d_bfirst_at : nullptr,
                      side_rd);
   hipLaunchKernelGGL(k_batch_listed<S>, dim3((unsigned)gg.listed), dim3(BT), 0, ms, A, B, b->dcases, b->d_lcase,
                      b->d_lfirst, b->d_lcount, b->d_lnodes, b->d_ln_at, tbase, t_local + j, j, b->d_status, side_rd);
-  if (b->band)
-    hipLaunchKernelGGL(k_batch_band<S>, dim3((unsigned)gg.band), dim3(BT), 0, ms, A, B, side_rd, side_wr, b->dcases,
+  if (b->band) {
+    hipLaunchKernelGGL(k_batch_band<S>, dim3((unsigned)gg.band), dim3(BT), 0, ts, A, B, side_rd, side_wr, b->dcases,
                        b->d_btile, b->d_nband_at, b->d_bidx, tbase, t_local + j, j, b->d_kick, b->d_status,
                        SQRT_CS2_B);
+    if (j + 1 == K)
+      hipLaunchKernelGGL(k_band_to_pool<T>, dim3((unsigned)gg.copy), dim3(BT), 0, ts, B, (const T*)b->bside0,
+                         b->dcases, b->d_ctile, b->d_ncopy_at, b->d_bidx, tbase, t_local + j, j, b->d_status);
+  }
   hipLaunchKernelGGL(k_batch_bulk<S>, dim3((unsigned)gg.bulk), dim3(BT), 0, ms, A, B, b->dflags, b->dcases,
                      b->d_kcase, b->d_ky0, b->d_kx0, b->d_kact, tbase, t_local + j, b->tw_log2, b->d_status,
                      (const int32_t*)b->d_kfirst_at, j);
@@ -971,26 +1027,36 @@ vl_status launch_batch_group(vl_batch* b, const int64_t* tbase, int t_local, con
   po.out = P(K);
   if (b->serial) {
     VL_TRYB((launch_pass_k<S>(b, b->tasks, b->stream, P(0), po, tbase, t_local, gg.pass)));
-    for (int j = 0; j < K; ++j) launch_group_phase<S>(b, b->stream, b->stream, P(j), P(j + 1), tbase, t_local, j, gg);
+    for (int j = 0; j < K; ++j)
+      launch_group_phase<S>(b, b->stream, b->stream, b->stream, P(j), P(j + 1), tbase, t_local, j, gg);
     VL_HIP(hipGetLastError());
   } else {
     VL_HIP(hipEventRecord(b->ev_fork, b->stream));
     VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
     VL_TRYB((launch_pass_k<S>(b, b->tasks, b->stream, P(0), po, tbase, t_local, gg.pass)));
-    launch_group_phase<S>(b, b->side, b->side, P(0), P(1), tbase, t_local, 0, gg);
+    launch_group_phase<S>(b, b->side, b->side, b->side, P(0), P(1), tbase, t_local, 0, gg);
     VL_HIP(hipEventRecord(b->ev_join, b->side));
     VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+    // steps 1 .. K-1: the band kernel on a third stream when there are bands, so that the band,
+    // listed and boundary nodes, each bound by latency, run side by side
+    const bool third = b->band && b->third;
     for (int j = 1; j < K; ++j) {
       VL_HIP(hipEventRecord(b->ev_fork, b->stream));
       VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
-      launch_group_phase<S>(b, b->side, b->stream, P(j), P(j + 1), tbase, t_local, j, gg);
+      if (third) VL_HIP(hipStreamWaitEvent(b->third, b->ev_fork, 0));
+      launch_group_phase<S>(b, b->side, b->stream, third ? b->third : b->stream, P(j), P(j + 1), tbase, t_local, j,
+                            gg);
       VL_HIP(hipEventRecord(b->ev_join, b->side));
+      if (third) VL_HIP(hipEventRecord(b->ev_join3, b->third));
       VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+      if (third) VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join3, 0));
     }
     VL_HIP(hipGetLastError());
   }
   b->cur = (b->cur + K) % nb;
   b->t += K;
+  // the last step wrote the bands' level K into the level-0 side array as well
+  b->side0_ok = b->band;
   return VL_OK;
 }
 
@@ -1065,6 +1131,7 @@ GroupGrids group_grids_at(const vl_batch* b, int64_t t, int groups) {
   gg.bulk = ladder_ceil(bulk, b->nbulk_tiles);
   gg.bnd = b->region ? ladder_ceil(bnd, b->nbnd_tiles) : 0;
   gg.band = b->band ? ladder_ceil(at(b->h_nband_at, t), b->nband_tiles) : 0;
+  gg.copy = b->band ? ladder_ceil(at(b->h_ncopy_at, t), b->ncopy_tiles) : 0;
   return gg;
 }
 
@@ -1133,6 +1200,7 @@ void free_batch(vl_batch* b) {
     if (p) (void)hipFree(p);
   for (void* p : b->bside)
     if (p) (void)hipFree(p);
+  if (b->bside0) (void)hipFree(b->bside0);
   for (auto& kv : b->graphs)
     if (kv.second) (void)hipGraphExecDestroy(kv.second);
   for (auto& e : b->tev)
@@ -1141,6 +1209,8 @@ void free_batch(vl_batch* b) {
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->ev_join) (void)hipEventDestroy(b->ev_join);
   if (b->ev_mid) (void)hipEventDestroy(b->ev_mid);
+  if (b->ev_join3) (void)hipEventDestroy(b->ev_join3);
+  if (b->third) (void)hipStreamDestroy(b->third);
   if (b->side) (void)hipStreamDestroy(b->side);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1163,6 +1233,11 @@ vl_status build(vl_batch* b) {
   VL_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
   VL_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
   VL_HIP(hipEventCreateWithFlags(&b->ev_mid, hipEventDisableTiming));
+  // the band kernel's own stream (VL_BATCH_THIRD=0: the band kernel on the main stream)
+  if (const char* e = std::getenv("VL_BATCH_THIRD"); !e || e[0] != '0') {
+    VL_HIP(hipStreamCreateWithFlags(&b->third, hipStreamNonBlocking));
+    VL_HIP(hipEventCreateWithFlags(&b->ev_join3, hipEventDisableTiming));
+  }
   // ---- set-up of every case: on the device when every body is a polygon or a circle (the set-up
   // kernels evaluate those predicates themselves), else on host threads; VL_BATCH_SETUP=host
   // forces the host builder ----
@@ -1363,8 +1438,9 @@ vl_status build(vl_batch* b) {
     }
   }
   if (b->band)
-    for (int L = 1; L < b->pass_k; ++L) {
-      VL_HIP(hipMalloc(&b->bside[L - 1], (size_t)std::max<int64_t>(side_elems, 1) * b->elem));
+    for (int L = 0; L < b->pass_k; ++L) {
+      void*& p = L == 0 ? b->bside0 : b->bside[L - 1];
+      VL_HIP(hipMalloc(&p, (size_t)std::max<int64_t>(side_elems, 1) * b->elem));
       b->device_bytes += std::max<int64_t>(side_elems, 1) * (int64_t)b->elem;
     }
   b->nbuf = b->pass_k > 1 ? b->pass_k + 1 : 2;
@@ -1724,8 +1800,9 @@ vl_status build(vl_batch* b) {
     std::vector<int32_t> tcase, ftcase, ntcase, lcase, lfirst, lcount;
     std::vector<uint32_t> lnodes;
     std::vector<int32_t> tend(nc, 0), ftend(nc, 0), ntend(nc, 0), lend(nc, 0), clfirst(nc, 0), clcount(nc, 0),
-        bend(nc, 0);
+        bend(nc, 0), cend(nc, 0);
     std::vector<int4> btile;
+    std::vector<int2> ctile;
     auto by_row = [](const int4& a, const int4& c) { return a.y != c.y ? a.y < c.y : a.x < c.x; };
     if (b->pass_k > 1)
       for (int ci : order) {
@@ -1766,10 +1843,13 @@ vl_status build(vl_batch* b) {
         clfirst[(size_t)ci] = (int32_t)lnodes.size();
         clcount[(size_t)ci] = (int32_t)lst.size();
         lnodes.insert(lnodes.end(), lst.begin(), lst.end());
-        if (bc_.band_w > 0)
+        if (bc_.band_w > 0) {
           for (int c = 0; c < 2 * bc_.band_w; ++c)
             for (int y0 = 0; y0 < bc_.g.ny; y0 += BT) btile.push_back(int4{ci, c, y0, 0});
+          for (int y0 = 0; y0 < bc_.g.ny; y0 += BT / (2 * bc_.band_w)) ctile.push_back(int2{ci, y0});
+        }
         bend[(size_t)ci] = (int32_t)btile.size();
+        cend[(size_t)ci] = (int32_t)ctile.size();
         tend[(size_t)ci] = (int32_t)task.size();
         lend[(size_t)ci] = (int32_t)lcase.size();
         b->pass_nodes += pl.covered;
@@ -1786,6 +1866,7 @@ vl_status build(vl_batch* b) {
     b->h_rn_at.assign(nt, 0);
     b->h_bfirst_at.assign(nt, 0);
     b->h_nband_at.assign(nt, 0);
+    b->h_ncopy_at.assign(nt, 0);
     if (b->pass_k > 1)
       for (size_t tt = 0; tt < nt; ++tt) {
         int32_t ncase = 0;
@@ -1798,6 +1879,7 @@ vl_status build(vl_batch* b) {
           b->h_kfirst_at[tt] = kend[(size_t)ci];
           b->h_bfirst_at[tt] = bnd_end[(size_t)ci];
           b->h_nband_at[tt] = bend[(size_t)ci];
+          b->h_ncopy_at[tt] = cend[(size_t)ci];
           b->h_rn_at[tt] = ++ncase;
         }
       }
@@ -1839,7 +1921,11 @@ vl_status build(vl_batch* b) {
     b->nband_tiles = (int)btile.size();
     VL_TRYB(dupload(b, b->d_btile, btile));
     VL_TRYB(dupload(b, b->d_nband_at, b->h_nband_at));
+    b->ncopy_tiles = (int)ctile.size();
+    VL_TRYB(dupload(b, b->d_ctile, ctile));
+    VL_TRYB(dupload(b, b->d_ncopy_at, b->h_ncopy_at));
   }
+  b->side0_ok = false;
   b->use_graphs = true;
   if (const char* e = std::getenv("VL_GRAPHS")) b->use_graphs = e[0] != '0';
   lap(3);
@@ -1853,9 +1939,11 @@ int window_steps(const vl_batch* b) { return b->pass_k > 1 ? GRAPH_W / b->pass_k
 vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
   const int wsteps = window_steps(b);
   const int K = b->pass_k;
-  GroupGrids gg{0, 0, 0, 0, 0, 0};
+  GroupGrids gg{0, 0, 0, 0, 0, 0, 0};
   const int grid = K > 1 ? (gg = group_grids_at(b, b->t, wsteps / K)).bulk : bulk_grid_at(b, b->t);
-  const auto key = std::make_tuple(b->cur, grid, gg.pass, gg.listed, gg.bnd, gg.pass_near, gg.band);
+  const bool ok0 = b->side0_ok;
+  const auto key =
+      std::make_tuple(b->cur, grid, gg.pass, gg.listed, gg.bnd, gg.pass_near, gg.band, gg.copy, (int)ok0);
   auto found = b->graphs.find(key);
   if (found != b->graphs.end()) {
     *out = found->second;
@@ -1876,6 +1964,7 @@ vl_status window_graph(vl_batch* b, hipGraphExec_t* out) {
   b->cur = cur0;
   b->t = t0;
   b->ov_open = false;
+  b->side0_ok = ok0;
   if (st != VL_OK) {
     if (g) (void)hipGraphDestroy(g);
     return st;
@@ -1964,6 +2053,7 @@ vl_status vl_batch_run(vl_batch* b, int64_t max_steps) {
       VL_HIP(hipGraphLaunch(ge, b->stream));
       b->t += wsteps;
       b->cur = (b->cur + wsteps) % b->nbuf;
+      b->side0_ok = b->band && b->pass_k > 1;  // a window of whole groups ends with a group
     }
   }
   if (k < todo) {
