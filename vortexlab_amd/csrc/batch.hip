@@ -73,8 +73,13 @@ struct BandView {
   int bw, xr, sny;
   __device__ inline T* at(int i, int x, int y) const {
     if (side) {
-      const bool lo = x >= 1 && x <= bw, hi = x >= xr && x < xr + bw;
-      if (lo || hi) return side + ((int64_t)(i * 2 * bw + (lo ? x - 1 : bw + x - xr)) * sny + y);
+      // The inlet and outlet cells (x = 0 and x = nx - 1) are not fluid: every population pulled
+      // from them is replaced by a link rule, so those reads go to the band's first / last
+      // column in the side array, beside the node's other loads, instead of a line of their own
+      // per row and plane (the value read is never used).
+      const int xs = x == 0 ? 1 : (x == xr + bw ? xr + bw - 1 : x);
+      const bool lo = xs >= 1 && xs <= bw, hi = xs >= xr && xs < xr + bw;
+      if (lo || hi) return side + ((int64_t)(i * 2 * bw + (lo ? xs - 1 : bw + xs - xr)) * sny + y);
     }
     return main + i * plane + ((int64_t)y * pitch + x);
   }
@@ -658,6 +663,15 @@ struct vl_batch {
   int ncopy_tiles = 0;
   hipStream_t third = nullptr;  // the band kernel's stream in the steps after a group's first
   hipEvent_t ev_join3 = nullptr;
+  // Single steps beside the next group's far pass (VL_BATCH_STEP_OVERLAP=1, with side arrays and
+  // a third stream): steps 1 .. K-1 of a group run on the side stream, forked onto the third
+  // (band) and fourth (listed and bulk) streams, while the main stream runs the next group's far
+  // tasks.  Bit-identical, but on the 512-case sweep 5 % slower than the default (the single
+  // steps after the group's whole pass): the far pass beside them slows by about as much time
+  // as they take.
+  bool step_overlap = false;
+  hipStream_t fourth = nullptr;
+  hipEvent_t ev_sfork = nullptr, ev_join4 = nullptr;
   int region_threads = 512;
   int32_t *d_rcase = nullptr, *d_rn_at = nullptr, *d_bfirst_at = nullptr, *d_clfirst = nullptr,
           *d_clcount = nullptr;
@@ -1015,10 +1029,66 @@ vl_status join_side(vl_batch* b) {
   return VL_OK;
 }
 
+// A group of K steps whose single steps 1 .. K-1 run beside the next group's far pass.
+//  * Main stream: the far tasks' pass, then (after the previous group's single steps) the near
+//    tasks' pass.  A far task's outputs lie more than K nodes from every uncovered node, so it
+//    reads covered nodes only (level K of the previous group, written by both passes before it
+//    in stream order) and writes no ring value; its level-K output lands in the pool that the
+//    previous group's last single steps still use, but only at far nodes, which no single step
+//    reads or writes.
+//  * Side stream: step 0's single-step part, in stream order after the previous group's single
+//    steps and beside this group's passes (it reads pool cur only and writes uncovered nodes and
+//    side arrays, which the passes do not touch), then, after the near pass (both passes have
+//    then written every ring value), steps 1 .. K-1: the boundary kernel on the side stream, the
+//    band kernel (and the bands' copy into the pool after the last step) on the third stream,
+//    the listed and bulk kernels on the fourth, forked from and joined back into the side
+//    stream per step.  The near pass of the next group waits for the end of these steps.
+// join_side joins the side stream back at the end of a window or a run.
+template <class S>
+vl_status launch_batch_group_steps_overlap(vl_batch* b, const int64_t* tbase, int t_local, const GroupGrids& gg) {
+  using T = typename S::T;
+  const int K = b->pass_k, nb = b->nbuf;
+  auto P = [&](int j) { return (T*)b->pool[(b->cur + j) % nb]; };
+  PassOut<T> po{};
+  for (int L = 1; L < K; ++L) po.ring[L - 1] = P(L);
+  po.out = P(K);
+  bool first = false;
+  if (!b->ov_open) {
+    VL_HIP(hipEventRecord(b->ev_fork, b->stream));
+    VL_HIP(hipStreamWaitEvent(b->side, b->ev_fork, 0));
+    b->ov_open = true;
+    first = true;
+  }
+  VL_TRYB((launch_pass_k<S>(b, b->tasks_far, b->stream, P(0), po, tbase, t_local, gg.pass)));
+  if (!first) VL_HIP(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+  VL_TRYB((launch_pass_k<S>(b, b->tasks_near, b->stream, P(0), po, tbase, t_local, gg.pass_near)));
+  VL_HIP(hipEventRecord(b->ev_mid, b->stream));
+  launch_group_phase<S>(b, b->side, b->side, b->side, P(0), P(1), tbase, t_local, 0, gg);
+  VL_HIP(hipStreamWaitEvent(b->side, b->ev_mid, 0));
+  for (int j = 1; j < K; ++j) {
+    VL_HIP(hipEventRecord(b->ev_sfork, b->side));
+    VL_HIP(hipStreamWaitEvent(b->third, b->ev_sfork, 0));
+    VL_HIP(hipStreamWaitEvent(b->fourth, b->ev_sfork, 0));
+    launch_group_phase<S>(b, b->side, b->fourth, b->third, P(j), P(j + 1), tbase, t_local, j, gg);
+    VL_HIP(hipEventRecord(b->ev_join3, b->third));
+    VL_HIP(hipEventRecord(b->ev_join4, b->fourth));
+    VL_HIP(hipStreamWaitEvent(b->side, b->ev_join3, 0));
+    VL_HIP(hipStreamWaitEvent(b->side, b->ev_join4, 0));
+  }
+  VL_HIP(hipEventRecord(b->ev_join, b->side));
+  VL_HIP(hipGetLastError());
+  b->cur = (b->cur + K) % nb;
+  b->t += K;
+  // the last step wrote the bands' level K into the level-0 side array as well
+  b->side0_ok = b->band;
+  return VL_OK;
+}
+
 template <class S>
 vl_status launch_batch_group(vl_batch* b, const int64_t* tbase, int t_local, const GroupGrids& gg) {
   using T = typename S::T;
   if (b->region && b->overlap && !b->serial) return launch_batch_group_overlap<S>(b, tbase, t_local, gg);
+  if (!b->region && b->step_overlap) return launch_batch_group_steps_overlap<S>(b, tbase, t_local, gg);
   if (b->region) return launch_batch_group_region<S>(b, tbase, t_local, gg);
   const int K = b->pass_k, nb = b->nbuf;
   auto P = [&](int j) { return (T*)b->pool[(b->cur + j) % nb]; };
@@ -1115,7 +1185,8 @@ GroupGrids group_grids_at(const vl_batch* b, int64_t t, int groups) {
   const int64_t last = (int64_t)b->h_ln_at.size() - 1;
   auto at = [&](const std::vector<int32_t>& v, int64_t i) { return v[(size_t)std::min(std::max<int64_t>(i, 0), last)]; };
   GroupGrids gg;
-  const bool split = b->region && b->overlap && !b->serial;  // far and near tasks in launches of their own
+  // far and near tasks in launches of their own
+  const bool split = (b->region && b->overlap && !b->serial) || (!b->region && b->step_overlap);
   const vl_batch::TaskList& first_list = split ? b->tasks_far : b->tasks;
   gg.pass = ladder_ceil((at(first_list.h_n_at, t) + W - 1) / W, (first_list.n + W - 1) / W);
   gg.pass_near = split ? ladder_ceil((at(b->tasks_near.h_n_at, t) + W - 1) / W, (b->tasks_near.n + W - 1) / W) : 0;
@@ -1210,7 +1281,10 @@ void free_batch(vl_batch* b) {
   if (b->ev_join) (void)hipEventDestroy(b->ev_join);
   if (b->ev_mid) (void)hipEventDestroy(b->ev_mid);
   if (b->ev_join3) (void)hipEventDestroy(b->ev_join3);
+  if (b->ev_sfork) (void)hipEventDestroy(b->ev_sfork);
+  if (b->ev_join4) (void)hipEventDestroy(b->ev_join4);
   if (b->third) (void)hipStreamDestroy(b->third);
+  if (b->fourth) (void)hipStreamDestroy(b->fourth);
   if (b->side) (void)hipStreamDestroy(b->side);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1237,6 +1311,9 @@ vl_status build(vl_batch* b) {
   if (const char* e = std::getenv("VL_BATCH_THIRD"); !e || e[0] != '0') {
     VL_HIP(hipStreamCreateWithFlags(&b->third, hipStreamNonBlocking));
     VL_HIP(hipEventCreateWithFlags(&b->ev_join3, hipEventDisableTiming));
+    VL_HIP(hipStreamCreateWithFlags(&b->fourth, hipStreamNonBlocking));
+    VL_HIP(hipEventCreateWithFlags(&b->ev_sfork, hipEventDisableTiming));
+    VL_HIP(hipEventCreateWithFlags(&b->ev_join4, hipEventDisableTiming));
   }
   // ---- set-up of every case: on the device when every body is a polygon or a circle (the set-up
   // kernels evaluate those predicates themselves), else on host threads; VL_BATCH_SETUP=host
@@ -1716,6 +1793,8 @@ vl_status build(vl_batch* b) {
   lap(2);
   // ---- autotune the bulk tile shape and the grid policy on the real batch ----
   if (const char* e = std::getenv("VL_BATCH_SERIAL")) b->serial = e[0] == '1';
+  b->step_overlap = false;
+  if (const char* e = std::getenv("VL_BATCH_STEP_OVERLAP")) b->step_overlap = b->band && b->third && !b->serial && e[0] == '1';
   hipDeviceProp_t prop;
   VL_HIP(hipGetDeviceProperties(&prop, 0));
   const int cus = std::max(1, prop.multiProcessorCount);
