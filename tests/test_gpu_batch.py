@@ -194,7 +194,7 @@ def test_batch_passes_equal_single_case_runs(gpu, monkeypatch, prec, k, rows):
         ch.close()
 
 
-@pytest.mark.parametrize("env", [{}, {"VL_BATCH_STEP_OVERLAP": "1"}, {"VL_BATCH_BAND": "0"}, {"VL_BATCH_THIRD": "0"},
+@pytest.mark.parametrize("env", [{}, {"VL_BATCH_STEP_OVERLAP": "1"}, {"VL_BATCH_LINK_SLOTS": "0"}, {"VL_BATCH_BAND": "0"}, {"VL_BATCH_THIRD": "0"},
                                  {"VL_BATCH_SERIAL": "1"},
                                  {"VL_GRAPHS": "0"}, {"VL_BATCH_REGION": "1"}, {"VL_BATCH_REGION": "1", "VL_BATCH_OVERLAP": "0"},
                                  {"VL_BATCH_REGION": "1", "VL_BATCH_OVERLAP": "0", "VL_BATCH_REGION0": "after"},
@@ -209,8 +209,9 @@ def test_batch_group_schedules_equal_single_case_runs(gpu, monkeypatch, env, pre
     arrays (the default: after the group's whole pass, the band kernel on a stream of its own
     after a group's first step; with VL_BATCH_STEP_OVERLAP=1 steps 1 .. K-1 of a group beside
     the next group's far pass, on streams of their own; with VL_BATCH_THIRD=0 the band kernel on
-    the main stream) or in the pools (VL_BATCH_BAND=0), on one stream and without graphs, and the
-    per-case region kernel
+    the main stream) or in the pools (VL_BATCH_BAND=0), with the links read from their slot-major
+    copy (the default) or from the link tables' columns (VL_BATCH_LINK_SLOTS=0), on one stream
+    and without graphs, and the per-case region kernel
     (VL_BATCH_REGION=1) beside the next group's far pass or after the whole pass
     (VL_BATCH_OVERLAP=0, its step 0 beside the pass or after it), at 256, 512 and 1024 threads,
     on one stream, and without graphs."""

@@ -53,6 +53,7 @@ struct BatchCase {
   int64_t bidx_off;     // offset of the case's band map
   int32_t band_sny;     // rows per band column in the side arrays and the band map
   int32_t pad2_;
+  LinkSlots ls;         // slot-major copy of the case's links (meta null: the link tables' columns)
 };
 
 constexpr int BT = 256;  // threads of both batch kernels
@@ -182,10 +183,16 @@ __global__ __launch_bounds__(BT) void k_batch_boundary(const typename S::T* __re
           const MainView<T> wv{B + cs.pop_off, cs.g.plane, cs.g.pitch};
           double* fc = cs.t.fcontrib + (t & 1) * ns2;
           double* pvn = cs.t.pvel + ((t + 1) & 1) * np1;
-          if (cs.c.model == MODEL_BGK)
+          if (cs.ls.meta) {
+            if (cs.c.model == MODEL_BGK)
+              boundary_node_v<S, MODEL_BGK, true>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn, &cs.ls);
+            else
+              boundary_node_v<S, MODEL_TRT, true>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn, &cs.ls);
+          } else if (cs.c.model == MODEL_BGK) {
             boundary_node_v<S, MODEL_BGK>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
-          else
+          } else {
             boundary_node_v<S, MODEL_TRT>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
+          }
         }
       }
     }
@@ -357,10 +364,16 @@ __global__ __launch_bounds__(BT) void k_batch_band(const typename S::T* __restri
         const int64_t ns2 = (int64_t)max(cs.t.nslots, 1) * 2;
         double* fc = cs.t.fcontrib + (t & 1) * ns2;
         double* pvn = cs.t.pvel + ((t + 1) & 1) * np1;
-        if (cs.c.model == MODEL_BGK)
+        if (cs.ls.meta) {
+          if (cs.c.model == MODEL_BGK)
+            boundary_node_v<S, MODEL_BGK, true>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn, &cs.ls);
+          else
+            boundary_node_v<S, MODEL_TRT, true>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn, &cs.ls);
+        } else if (cs.c.model == MODEL_BGK) {
           boundary_node_v<S, MODEL_BGK>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
-        else
+        } else {
           boundary_node_v<S, MODEL_TRT>(rv, wv, cs.g, cs.c, cs.t, ctl, drho, sqrt_cs2, b, fc, pvn);
+        }
       } else {
         if (cs.c.model == MODEL_BGK)
           interior_node_v<S, MODEL_BGK>(x, y, rv, wv, cs.g, cs.c);
@@ -369,6 +382,36 @@ __global__ __launch_bounds__(BT) void k_batch_band(const typename S::T* __restri
       }
     }
     __syncthreads();  // `red` is reused by the next tile
+  }
+}
+
+// The slot-major copy of every case's links (built once, see LinkSlots): workgroup row y takes
+// case case0 + y, a boundary node per thread.
+__global__ __launch_bounds__(BT) void k_link_slots(const BatchCase* __restrict__ cases, int case0, uint32_t* meta,
+                                                   int32_t* fslot, double* q, double* cu6, double* rho, double* uwx,
+                                                   double* uwy) {
+  const BatchCase& cs = cases[case0 + (int)blockIdx.y];
+  if (!cs.ls.meta) return;
+  const DevTables& t = cs.t;
+  const int64_t stride = cs.ls.stride;
+  const int64_t base = cs.ls.meta - meta;  // the case's first boundary node in the slot arrays
+  for (int b = (int)(blockIdx.x * BT + threadIdx.x); b < t.nb; b += (int)(gridDim.x * BT)) {
+    const int l0 = t.bfirst[b], n = min(t.bfirst[b + 1] - l0, LINK_SLOTS);
+    for (int k = 0; k < max(n, 1); ++k) {
+      const int64_t e = base + (int64_t)k * stride + b;
+      if (k >= n) {  // a node without links: slot 0 says so
+        meta[e] = pack_link_meta(0, 0, 0, 0, 0, 0, 0);
+        continue;
+      }
+      const int l = l0 + k;
+      meta[e] = pack_link_meta(t.dir[l], t.kind[l], t.vgroup[l], t.lflags[l], t.lnx[l], t.lny[l], k == 0 ? n : 0);
+      fslot[e] = t.fslot[l];
+      q[e] = t.q[l];
+      cu6[e] = t.cu6[l];
+      rho[e] = t.rho[l];
+      uwx[e] = t.uwx[l];
+      uwy[e] = t.uwy[l];
+    }
   }
 }
 
@@ -399,35 +442,59 @@ __global__ __launch_bounds__(BT) void k_band_map_bnd(int32_t* __restrict__ bidx,
 
 // The bands' level K of a group, which its last step wrote into the level-0 side array, copied
 // into the pool (the next group's passes read it there as halo, and single steps outside groups
-// read every node there).  Copy tile i = (case, first row) covers BT / (2 band_w) rows of all
-// 2 band_w band columns, one node per thread with the columns of a row on adjacent threads, so
-// a plane's row segment on each side is one store instruction rather than one per column.
+// read every node there).  Copy tile i = (case, first row) covers COPY_ROWS rows of all 2 band_w
+// band columns: the workgroup loads the block of every plane along the side array's columns (the
+// rows of a column on adjacent threads), turns it over in LDS, and stores it along the pool's rows
+// (the columns of a row on adjacent threads, two runs of band_w values per row).  Positions that
+// are not fluid keep their pool values.
+constexpr int COPY_ROWS = 64;
 template <class T>
 __global__ __launch_bounds__(BT) void k_band_to_pool(T* __restrict__ B, const T* __restrict__ side0,
                                                      const BatchCase* __restrict__ cases, const int2* __restrict__ tiles,
                                                      const int32_t* __restrict__ ncopy_at,
                                                      const int32_t* __restrict__ bidx, const int64_t* tbase,
                                                      int t_local, int jpos, const int32_t* __restrict__ status) {
+  constexpr int NE = COPY_ROWS * 2 * TB_MAX_K;  // block elements per plane, at most
+  __shared__ T blk[Q * NE];
+  __shared__ uint8_t fluid[NE];
+  __shared__ int skip;
   const int64_t t = *tbase + t_local;
   const int n = ncopy_at[t - jpos];
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const int2 tl = tiles[i];  // case, first row
     const int ci = tl.x;
-    if (status[ci]) continue;  // uniform over the workgroup; no barrier in the loop
+    // one read of the case's status for the whole workgroup (it meets barriers below)
+    if (threadIdx.x == 0) skip = status[ci];
+    __syncthreads();
+    const int sk = skip;
+    __syncthreads();
+    if (sk) continue;
     const BatchCase& cs = cases[ci];
-    const int bw = cs.band_w, bw2 = 2 * bw;
-    const int r = (int)threadIdx.x / bw2, c = (int)threadIdx.x - r * bw2;
-    const int y = tl.y + r;
-    if (r >= BT / bw2 || y >= cs.g.ny) continue;
-    if (bidx[cs.bidx_off + (int64_t)c * cs.band_sny + y] == -2) continue;  // not a fluid node
-    const int x = c < bw ? 1 + c : cs.g.nx - 1 - bw + (c - bw);
-    const T* src = side0 + cs.side_off + (int64_t)c * cs.band_sny + y;
-    T* dst = B + cs.pop_off + (int64_t)y * cs.g.pitch + x;
-    T v[Q];
+    const int bw = cs.band_w, bw2 = 2 * bw, ne = COPY_ROWS * bw2;
+    // load: element e = column c, row r (rows fastest); the block is stored row-major
+    for (int e = (int)threadIdx.x; e < ne; e += BT) {
+      const int c = e / COPY_ROWS, r = e - c * COPY_ROWS, y = tl.y + r;
+      const int64_t col = (int64_t)c * cs.band_sny + y;
+      const bool ok = y < cs.g.ny && bidx[cs.bidx_off + col] != -2;
+      const int o = r * bw2 + c;
+      fluid[o] = ok;
+      if (ok) {
+        const T* src = side0 + cs.side_off + col;
 #pragma unroll
-    for (int k = 0; k < Q; ++k) v[k] = src[(int64_t)k * bw2 * cs.band_sny];
+        for (int k = 0; k < Q; ++k) blk[k * NE + o] = src[(int64_t)k * bw2 * cs.band_sny];
+      }
+    }
+    __syncthreads();
+    // store: element e = row r, column c (columns fastest)
+    for (int e = (int)threadIdx.x; e < ne; e += BT) {
+      if (!fluid[e]) continue;
+      const int r = e / bw2, c = e - r * bw2, y = tl.y + r;
+      const int x = c < bw ? 1 + c : cs.g.nx - 1 - bw + (c - bw);
+      T* dst = B + cs.pop_off + (int64_t)y * cs.g.pitch + x;
 #pragma unroll
-    for (int k = 0; k < Q; ++k) dst[(int64_t)k * cs.g.plane] = v[k];
+      for (int k = 0; k < Q; ++k) dst[(int64_t)k * cs.g.plane] = blk[k * NE + e];
+    }
+    __syncthreads();  // the block is reused by the next tile
   }
 }
 
@@ -661,6 +728,10 @@ struct vl_batch {
   int32_t* d_ncopy_at = nullptr;
   std::vector<int32_t> h_ncopy_at;
   int ncopy_tiles = 0;
+  // Slot-major copy of the links (LinkSlots) for the boundary and band kernels; VL_BATCH_LINK_SLOTS=0
+  // keeps them on the link tables' columns.  Arrays: meta, fslot, q, cu6, rho, uwx, uwy.
+  bool link_slots = true;
+  void* lslot[7] = {};
   hipStream_t third = nullptr;  // the band kernel's stream in the steps after a group's first
   hipEvent_t ev_join3 = nullptr;
   // Single steps beside the next group's far pass (VL_BATCH_STEP_OVERLAP=1, with side arrays and
@@ -1730,6 +1801,38 @@ vl_status build(vl_batch* b) {
   VL_TRYB(dupload(b, b->d_bb0, bb0));
   VL_TRYB(dupload(b, b->d_bact, bact));
   b->h_bact = bact;
+  // slot-major link copy: LINK_SLOTS slots of every boundary node of every case
+  if (const char* e = std::getenv("VL_BATCH_LINK_SLOTS")) b->link_slots = e[0] != '0';
+  if (b->link_slots) {
+    int64_t nbt = 0;
+    for (size_t i = 0; i < nc; ++i) nbt += b->hcases[i].t.nb;
+    const int64_t n = std::max<int64_t>(nbt, 1) * LINK_SLOTS;
+    uint32_t* m;
+    int32_t* fs;
+    double *q, *cu6, *rho, *uwx, *uwy;
+    VL_TRYB(dalloc(b, m, (size_t)n));
+    VL_TRYB(dalloc(b, fs, (size_t)n));
+    VL_TRYB(dalloc(b, q, (size_t)n));
+    VL_TRYB(dalloc(b, cu6, (size_t)n));
+    VL_TRYB(dalloc(b, rho, (size_t)n));
+    VL_TRYB(dalloc(b, uwx, (size_t)n));
+    VL_TRYB(dalloc(b, uwy, (size_t)n));
+    void* arrs[7] = {m, fs, q, cu6, rho, uwx, uwy};
+    for (int k = 0; k < 7; ++k) b->lslot[k] = arrs[k];
+    int64_t off = 0;
+    for (size_t i = 0; i < nc; ++i) {
+      LinkSlots& ls = b->hcases[i].ls;
+      ls.meta = m + off;
+      ls.fslot = fs + off;
+      ls.q = q + off;
+      ls.cu6 = cu6 + off;
+      ls.rho = rho + off;
+      ls.uwx = uwx + off;
+      ls.uwy = uwy + off;
+      ls.stride = std::max<int64_t>(nbt, 1);
+      off += b->hcases[i].t.nb;
+    }
+  }
   VL_TRYB(dupload(b, b->dcases, b->hcases));
   VL_TRYB(dalloc(b, b->d_status, nc));
   VL_TRYB(dalloc(b, b->d_done, nc));
@@ -1779,6 +1882,14 @@ vl_status build(vl_batch* b) {
     }
     VL_HIP(hipGetLastError());
   }
+  if (b->link_slots)
+    for (size_t c0 = 0; c0 < nc; c0 += 65535) {
+      const unsigned rows = (unsigned)std::min<size_t>(65535, nc - c0);
+      hipLaunchKernelGGL(k_link_slots, dim3(16, rows), dim3(BT), 0, b->stream, b->dcases, (int)c0,
+                         (uint32_t*)b->lslot[0], (int32_t*)b->lslot[1], (double*)b->lslot[2], (double*)b->lslot[3],
+                         (double*)b->lslot[4], (double*)b->lslot[5], (double*)b->lslot[6]);
+      VL_HIP(hipGetLastError());
+    }
   if (b->band) {
     VL_TRYB(dalloc(b, b->d_bidx, (size_t)std::max<int64_t>(bidx_len, 1)));
     for (size_t c0 = 0; c0 < nc; c0 += 65535) {
@@ -1925,7 +2036,7 @@ vl_status build(vl_batch* b) {
         if (bc_.band_w > 0) {
           for (int c = 0; c < 2 * bc_.band_w; ++c)
             for (int y0 = 0; y0 < bc_.g.ny; y0 += BT) btile.push_back(int4{ci, c, y0, 0});
-          for (int y0 = 0; y0 < bc_.g.ny; y0 += BT / (2 * bc_.band_w)) ctile.push_back(int2{ci, y0});
+          for (int y0 = 0; y0 < bc_.g.ny; y0 += COPY_ROWS) ctile.push_back(int2{ci, y0});
         }
         bend[(size_t)ci] = (int32_t)btile.size();
         cend[(size_t)ci] = (int32_t)ctile.size();

@@ -279,10 +279,50 @@ __device__ inline void reduce_forces(const double* __restrict__ fc, const int32_
   }
 }
 
+// The links of boundary nodes in slot-major order: link k of boundary node b (its k-th link in
+// link order) at [k * stride + b].  The small fields go into one word: direction (bits 0-3), kind
+// (4-7), velocity group (8-11), flags (12-15), normal + 1 (16-17 x, 18-19 y) and, in slot 0, the
+// node's link count (24-31).  The lanes of a wave hold consecutive boundary nodes (or consecutive
+// rows of a column of them), so each field loads along a line, where the link tables' columns
+// give every lane a line of its own.  A fluid node has at most one link per moving direction.
+constexpr int LINK_SLOTS = 8;
+struct LinkSlots {
+  const uint32_t* meta = nullptr;
+  const int32_t* fslot = nullptr;
+  const double *q = nullptr, *cu6 = nullptr, *rho = nullptr, *uwx = nullptr, *uwy = nullptr;
+  int64_t stride = 0;
+};
+__host__ __device__ inline uint32_t pack_link_meta(int dir, int kind, int vgroup, int lflags, int lnx, int lny,
+                                                   int count) {
+  return (uint32_t)dir | (uint32_t)kind << 4 | (uint32_t)vgroup << 8 | (uint32_t)lflags << 12 |
+         (uint32_t)(lnx + 1) << 16 | (uint32_t)(lny + 1) << 18 | (uint32_t)count << 24;
+}
+
 // The fields of one link (the per-link columns of the boundary tables).
 struct LinkFields {
   int dir, kind, vgroup, lflags, lnx, lny, fslot;
   double q, cu6, rho, uwx, uwy;
+  int count;  // from slot 0 of a slot-major node: its link count
+  // link k of boundary node b from the slot-major copy
+  __device__ static inline LinkFields slot(const LinkSlots& s, int k, int b) {
+    const int64_t e = (int64_t)k * s.stride + b;
+    const uint32_t m = s.meta[e];
+    LinkFields L;
+    L.dir = (int)(m & 15u);
+    L.kind = (int)((m >> 4) & 15u);
+    L.vgroup = (int)((m >> 8) & 15u);
+    L.lflags = (int)((m >> 12) & 15u);
+    L.lnx = (int)((m >> 16) & 3u) - 1;
+    L.lny = (int)((m >> 18) & 3u) - 1;
+    L.count = (int)(m >> 24);
+    L.fslot = s.fslot[e];
+    L.q = s.q[e];
+    L.cu6 = s.cu6[e];
+    L.rho = s.rho[e];
+    L.uwx = s.uwx[e];
+    L.uwy = s.uwy[e];
+    return L;
+  }
   __device__ static inline LinkFields load(const DevTables& t, int l) {
     LinkFields L;
     L.dir = t.dir[l];
@@ -297,6 +337,7 @@ struct LinkFields {
     L.rho = t.rho[l];
     L.uwx = t.uwx[l];
     L.uwy = t.uwy[l];
+    L.count = 0;
     return L;
   }
 };
@@ -304,10 +345,13 @@ struct LinkFields {
 // One boundary node (src/sim.rs:456-507): pull, replace the populations that would come out
 // of non-fluid cells by the link rules in link order, collide, write; each force link stores
 // its momentum-exchange term.  Reads go through the view rv (time t), writes through wv.
-template <class S, int M, class RV, class WV>
+// SLOTS: the links come from the slot-major copy ls instead of the link tables' columns (the same
+// fields in the same order, so the same bits).
+template <class S, int M, bool SLOTS = false, class RV, class WV>
 __device__ inline void boundary_node_v(const RV& rv, const WV& wv, const Geom& g, const Coll& c, const DevTables& t,
                                        const StepCtl& ctl, double drho, double sqrt_cs2, int b,
-                                       double* __restrict__ fc, double* __restrict__ pv_next) {
+                                       double* __restrict__ fc, double* __restrict__ pv_next,
+                                       const LinkSlots* ls = nullptr) {
   using C = typename S::C;
   const int x = t.bx[b], y = t.by[b];
   C f[Q], own[Q];
@@ -323,12 +367,25 @@ __device__ inline void boundary_node_v(const RV& rv, const WV& wv, const Geom& g
   // Every field of a link is loaded together, unconditionally, and the next link's fields while
   // this one is processed: one memory round trip per node's link list instead of several per
   // link.  Indices past the table end are clamped (those loads are never used).
-  const int l0 = t.bfirst[b], l1 = t.bfirst[b + 1];
-  const int lmax = max(t.nl - 1, 0);
-  LinkFields nxt = LinkFields::load(t, min(l0, lmax));
-  for (int l = l0; l < l1; ++l) {
+  // With slots, slot 0 carries the link count and the next slot loaded is clamped to the last
+  // link.
+  int l0 = 0, n = 0, lmax = 0;
+  LinkFields nxt;
+  if constexpr (SLOTS) {
+    nxt = LinkFields::slot(*ls, 0, b);
+    n = nxt.count;
+  } else {
+    l0 = t.bfirst[b];
+    n = t.bfirst[b + 1] - l0;
+    lmax = max(t.nl - 1, 0);
+    nxt = LinkFields::load(t, min(l0, lmax));
+  }
+  for (int li = 0; li < n; ++li) {
     const LinkFields L = nxt;
-    nxt = LinkFields::load(t, min(l + 1, lmax));
+    if constexpr (SLOTS)
+      nxt = LinkFields::slot(*ls, min(li + 1, n - 1), b);
+    else
+      nxt = LinkFields::load(t, min(l0 + li + 1, lmax));
     const int j = L.dir;
     const int i = opp(j);
     const C out = own[j];
