@@ -1520,14 +1520,15 @@ vl_status build(vl_batch* b) {
   // ---- K-step passes: the plan of every case (its strips' tasks and listed nodes) ----
   // Defaults: FP32 5 steps per pass (the 512-case sweep on MI355X: 93 GLUPS against 87 at 4 and
   // 67 in single steps), FP64 3 (the 4-step FP64 pass of both collides needs all 256 registers),
-  // FP16 3; tasks of 12 (FP32), 16 (FP64) or 24 (FP16) rows.  FP32 on the sweep, cases/s by task
-  // height: 8 rows 25.9, 10 26.1, 12 26.3, 14 26.2, 16 25.9, 20 25.4, 28 24.4, 36 23.7 (shorter
-  // tasks re-read more halo rows but balance better over the sweep's short, narrow cases).
+  // FP16 3; tasks of 14 (FP32), 16 (FP64) or 24 (FP16) rows.  FP32 on the sweep, cases/s by task
+  // height with the bands in side arrays: 10 rows 32.8, 12 32.9, 14 33.2, 16 32.8, 20 32.3
+  // (before them: 8 rows 25.9, 10 26.1, 12 26.3, 14 26.2, 16 25.9, 20 25.4, 28 24.4, 36 23.7);
+  // shorter tasks re-read more halo rows but balance better over the sweep's short, narrow cases.
   // VL_BATCH_PASS_STEPS (1 = single steps only) and VL_BATCH_TB_ROWS override.
   b->pass_k = b->precision == VL_F32 ? 5 : 3;
   if (const char* e = std::getenv("VL_BATCH_PASS_STEPS"))
     b->pass_k = std::min(std::max(std::atoi(e), 1), b->precision == VL_F32 ? TB_MAX_K : 4);
-  b->pass_rows = b->precision == VL_F16 ? 24 : (b->precision == VL_F32 ? 12 : 16);
+  b->pass_rows = b->precision == VL_F16 ? 24 : (b->precision == VL_F32 ? 14 : 16);
   if (const char* e = std::getenv("VL_BATCH_TB_ROWS")) b->pass_rows = std::max(1, std::atoi(e));
   // 8 waves per workgroup (FP64: 4); on the 512-case sweep FP32 in 4-wave workgroups, which
   // find more lockstep groups among narrow cases, ran 8 % slower, and tasks of 32 rows 7 %.
