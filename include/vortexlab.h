@@ -197,6 +197,11 @@ vl_status vl_sim_state_hash(vl_sim* s, uint64_t* out);          /* sim.rs:654-66
 vl_status vl_sim_total_mass(vl_sim* s, double* out);            /* sim.rs:538-548 */
 vl_status vl_sim_total_momentum(vl_sim* s, double* out2);       /* sim.rs:551-563 */
 vl_status vl_sim_max_speed(vl_sim* s, double* out);             /* sim.rs:566-579 */
+/* Sim::vorticity (sim.rs:581-620): dv/dx - du/dy per node, nx*ny floats in node order
+ * (one-sided differences next to non-fluid cells and the domain edge, 0 in solids). */
+vl_status vl_sim_vorticity(vl_sim* s, float* out);
+vl_status vl_sim_is_fluid(vl_sim* s, int64_t x, int64_t y, int* out); /* sim.rs:381-383 */
+vl_status vl_sim_periodic(vl_sim* s, int* px, int* py);          /* sim.rs:670-672 */
 /* Kernel timing with HIP events on the sim's own stream (no reference counterpart; the
  * reference times wall clock only, src/cases/cylinder.rs:224-232).  Switching it on resets
  * the counters; bulk_ms sums the duration of every bulk stream-collide launch, total_ms the

@@ -1156,6 +1156,39 @@ vl_status or_sim_max_speed(or_sim* s, double* out) {  // sim.rs:566-579
   return VL_OK;
 }
 
+vl_status or_sim_vorticity(or_sim* s, float* out) {  // sim.rs:581-620
+  const long nx = (long)s->nx, ny = (long)s->ny;
+  std::vector<std::array<double, 2>> u(s->n, {0.0, 0.0});
+  for (size_t idx = 0; idx < s->n; ++idx)
+    if (s->flag[idx] != 2) s->velocity_at(idx, u[idx][0], u[idx][1]);
+  auto fluid = [&](long x, long y) { return x >= 0 && y >= 0 && x < nx && y < ny && s->flag[y * nx + x] != 2; };
+  for (long y = 0; y < ny; ++y)
+    for (long x = 0; x < nx; ++x) {
+      out[y * nx + x] = 0.0f;
+      if (!fluid(x, y)) continue;
+      auto diff = [&](long dx, long dy, int comp) {
+        auto get = [&](long xx, long yy) { return u[yy * nx + xx][comp]; };
+        const bool p = fluid(x + dx, y + dy), m = fluid(x - dx, y - dy);
+        if (p && m) return 0.5 * (get(x + dx, y + dy) - get(x - dx, y - dy));
+        if (p) return get(x + dx, y + dy) - get(x, y);
+        if (m) return get(x, y) - get(x - dx, y - dy);
+        return 0.0;
+      };
+      out[y * nx + x] = (float)(diff(1, 0, 1) - diff(0, 1, 0));
+    }
+  return VL_OK;
+}
+vl_status or_sim_is_fluid(or_sim* s, int64_t x, int64_t y, int* out) {  // sim.rs:381-383
+  if (x < 0 || y < 0 || x >= (int64_t)s->nx || y >= (int64_t)s->ny) return fail(VL_EINVAL, "node outside the domain");
+  *out = s->flag[y * s->nx + x] != 2;
+  return VL_OK;
+}
+vl_status or_sim_periodic(or_sim* s, int* px, int* py) {  // sim.rs:670-672
+  *px = s->periodic_x;
+  *py = s->periodic_y;
+  return VL_OK;
+}
+
 // channel.rs:89-134
 vl_status or_channel_create(const vl_channel_spec* sp, or_shape* body, int threads, or_channel** out) {
   if (!sp || !body || !out) return fail(VL_EINVAL, "null argument");

@@ -66,6 +66,9 @@ vl_status or_sim_state_hash(or_sim* s, uint64_t* out);
 vl_status or_sim_total_mass(or_sim* s, double* out);
 vl_status or_sim_total_momentum(or_sim* s, double* out2);
 vl_status or_sim_max_speed(or_sim* s, double* out);
+vl_status or_sim_vorticity(or_sim* s, float* out);
+vl_status or_sim_is_fluid(or_sim* s, int64_t x, int64_t y, int* out);
+vl_status or_sim_periodic(or_sim* s, int* px, int* py);
 
 vl_status or_channel_create(const vl_channel_spec* spec, or_shape* body, int threads,
                             or_channel** out);

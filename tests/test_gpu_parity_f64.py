@@ -186,3 +186,23 @@ def test_tunnel_with_radiating_sides_and_kick(gpu, orc):
     assert_pops(g.sim.read(), o.sim.read(), "tunnel")
     assert_force(gc, oc, "tunnel cd/cl")
     assert abs(g.sim.outlet_density_offset - o.sim.outlet_density_offset) == 0.0
+
+
+def test_vorticity_is_fluid_periodic_match_oracle(gpu, orc):
+    """Sim::vorticity, is_fluid and periodic (src/sim.rs:581-620, 381-383, 670-672) on the GPU
+    state after 400 steps of the small-cylinder channel: the same FP64 populations give the same
+    float vorticity bit for bit; the FP32-storage state gives it within its storage error."""
+    g = api.Channel(gpu, small_cylinder_spec(), api.F64)
+    o = api.Channel(orc, small_cylinder_spec(), 2)
+    g.run(400, 10.0)
+    o.run(400, 10.0)
+    wg, wo = g.sim.vorticity(), o.sim.vorticity()
+    assert wg.dtype == np.float32 and wg.shape == (g.sim.ny, g.sim.nx)
+    assert np.array_equal(wg, wo)
+    assert np.max(np.abs(wo)) > 1e-3  # the wake has vorticity
+    for x, y in [(0, 0), (1, 1), (g.sim.nx // 2, g.sim.ny // 2), (g.sim.nx - 1, 5), (40, 20)]:
+        assert g.sim.is_fluid(x, y) == o.sim.is_fluid(x, y)
+    assert g.sim.periodic() == o.sim.periodic()
+    f32 = api.Channel(gpu, small_cylinder_spec(), api.F32)
+    f32.run(400, 10.0)
+    assert np.max(np.abs(f32.sim.vorticity() - wo)) < 1e-5

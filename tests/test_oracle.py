@@ -198,3 +198,34 @@ def test_wake_golden_fixture_is_consistent():
     assert ch.sim.fluid_nodes == gold["fluid_nodes"]
     assert np.array_equal(coef[9::10], rows[:100])
     ch.close()
+
+
+def test_vorticity_closed_forms(orc):
+    """Sim::vorticity (src/sim.rs:581-620) on fields whose differences are known exactly, set as
+    equilibria in the closed box of tests/solver.rs:64-80 (fluid x = 1..32, y = 1..24):
+    * u = (a y, b x): every difference is exact, so the vorticity is b - a at every fluid node;
+    * u = (0, c x^2): central differences give 2 c x inside, the one-sided ones (2 x + 1) c at
+      x = 1 (its left neighbour is a wall) and (2 x - 1) c at x = 32;
+    and 0 in the walls.  Also Sim::is_fluid and Sim::periodic."""
+    s = closed_box(orc, "trt", 1)
+    nx, ny = 34, 26
+    yy, xx = np.mgrid[0:ny, 0:nx].astype(np.float64)
+    a, b = 3e-4, -7e-4
+    s.init_arrays(1.0, a * yy, b * xx)
+    w = s.vorticity()
+    fluid = np.zeros((ny, nx), dtype=bool)
+    fluid[1:25, 1:33] = True
+    assert np.all(w[~fluid] == 0.0)
+    assert np.max(np.abs(w[fluid] - (b - a))) < 1e-9
+    c = 2e-5
+    s.init_arrays(1.0, 0.0, c * xx * xx)
+    w = s.vorticity()
+    want = 2.0 * c * xx
+    want[:, 1] = 3.0 * c
+    want[:, 32] = (2.0 * 32 - 1.0) * c
+    assert np.max(np.abs(w[fluid] - want[fluid])) < 1e-9
+    assert s.is_fluid(1, 1) and s.is_fluid(32, 24) and not s.is_fluid(0, 5) and not s.is_fluid(33, 5)
+    with pytest.raises(api.VortexError):
+        s.is_fluid(34, 0)
+    assert s.periodic() == (False, False)
+    assert periodic_box(orc, "bgk", 1, nx=12, ny=10).periodic() == (True, True)

@@ -139,6 +139,9 @@ _SIGS = [
     ("sim_total_mass", C.c_int, [_VP, _D]),
     ("sim_total_momentum", C.c_int, [_VP, _D]),
     ("sim_max_speed", C.c_int, [_VP, _D]),
+    ("sim_vorticity", C.c_int, [_VP, C.POINTER(C.c_float)]),
+    ("sim_is_fluid", C.c_int, [_VP, C.c_int64, C.c_int64, C.POINTER(C.c_int)]),
+    ("sim_periodic", C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("sim_set_timing", C.c_int, [_VP, C.c_int]),
     ("device_copy_bandwidth", C.c_int, [C.c_size_t, C.c_int, _D]),
     ("sim_get_timing", C.c_int, [_VP, _D, C.POINTER(C.c_int64), _D, C.POINTER(C.c_int64)]),
@@ -537,6 +540,24 @@ class Sim:
         v = C.c_double()
         self.be.check(self.be.sim_max_speed(self.h, C.byref(v)))
         return v.value
+
+    def vorticity(self) -> np.ndarray:
+        """Sim::vorticity (src/sim.rs:581-620): dv/dx - du/dy per node as float32 [ny, nx]."""
+        out = np.zeros((self.ny, self.nx), dtype=np.float32)
+        self.be.check(self.be.sim_vorticity(self.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def is_fluid(self, x: int, y: int) -> bool:
+        """Sim::is_fluid (src/sim.rs:381-383)."""
+        v = C.c_int()
+        self.be.check(self.be.sim_is_fluid(self.h, int(x), int(y), C.byref(v)))
+        return bool(v.value)
+
+    def periodic(self):
+        """Sim::periodic (src/sim.rs:670-672): (periodic in x, periodic in y)."""
+        px, py = C.c_int(), C.c_int()
+        self.be.check(self.be.sim_periodic(self.h, C.byref(px), C.byref(py)))
+        return bool(px.value), bool(py.value)
 
     def set_timing(self, on: bool = True):
         self.be.check(self.be.sim_set_timing(self.h, int(on)))

@@ -1641,6 +1641,61 @@ vl_status vl_sim_total_momentum(vl_sim* s, double* out) {
   return VL_OK;
 }
 
+// Sim::vorticity (src/sim.rs:581-620): dv/dx - du/dy by central differences, one-sided where one
+// neighbour along the axis is not fluid (or lies past the domain edge; the differences do not
+// wrap), 0 where neither is and in non-fluid cells; out: nx * ny values, node (x, y) at y nx + x.
+vl_status vl_sim_vorticity(vl_sim* s, float* out) {
+  if (!s || !out) return set_error(VL_EINVAL, "null argument");
+  std::vector<char> host;
+  VL_TRY(read_raw(s, host));
+  const int64_t nx = s->g.nx, ny = s->g.ny;
+  std::vector<double> u((size_t)(2 * nx * ny), 0.0);
+  for (int64_t y = 0; y < ny; ++y)
+    for (int64_t x = 0; x < nx; ++x) {
+      const int64_t k = y * nx + x;
+      if (s->tab.flag[k] == 2) continue;
+      double f[Q];
+      for (int i = 0; i < Q; ++i) f[i] = host_load(s, host, i, y * s->g.pitch + x);
+      double r, jx, jy;
+      Exact::moments(f, r, jx, jy);
+      u[2 * k] = jx / r - 0.5 * s->accel[0];
+      u[2 * k + 1] = jy / r - 0.5 * s->accel[1];
+    }
+  auto fluid = [&](int64_t x, int64_t y) {
+    return x >= 0 && y >= 0 && x < nx && y < ny && s->tab.flag[y * nx + x] != 2;
+  };
+  // d(component comp)/d(axis (dx, dy)) at (x, y) from whichever neighbours are fluid
+  auto diff = [&](int64_t x, int64_t y, int dx, int dy, int comp) {
+    const auto at = [&](int64_t xx, int64_t yy) { return u[(size_t)(2 * (yy * nx + xx) + comp)]; };
+    const bool p = fluid(x + dx, y + dy), m = fluid(x - dx, y - dy);
+    if (p && m) return 0.5 * (at(x + dx, y + dy) - at(x - dx, y - dy));
+    if (p) return at(x + dx, y + dy) - at(x, y);
+    if (m) return at(x, y) - at(x - dx, y - dy);
+    return 0.0;
+  };
+  for (int64_t y = 0; y < ny; ++y)
+    for (int64_t x = 0; x < nx; ++x)
+      out[y * nx + x] = fluid(x, y) ? (float)(diff(x, y, 1, 0, 1) - diff(x, y, 0, 1, 0)) : 0.0f;
+  return VL_OK;
+}
+
+// Sim::is_fluid (src/sim.rs:381-383); a node outside the domain is an error (the reference
+// panics on the index).
+vl_status vl_sim_is_fluid(vl_sim* s, int64_t x, int64_t y, int* out) {
+  if (!s || !out) return set_error(VL_EINVAL, "null argument");
+  if (x < 0 || y < 0 || x >= s->g.nx || y >= s->g.ny) return set_error(VL_EINVAL, "node outside the domain");
+  *out = s->tab.flag[y * s->g.nx + x] != 2;
+  return VL_OK;
+}
+
+// Sim::periodic (src/sim.rs:670-672).
+vl_status vl_sim_periodic(vl_sim* s, int* px, int* py) {
+  if (!s || !px || !py) return set_error(VL_EINVAL, "null argument");
+  *px = s->g.periodic_x != 0;
+  *py = s->g.periodic_y != 0;
+  return VL_OK;
+}
+
 vl_status vl_sim_max_speed(vl_sim* s, double* out) {
   if (!s || !out) return set_error(VL_EINVAL, "null argument");
   std::vector<char> host;
