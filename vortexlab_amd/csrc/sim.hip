@@ -1141,6 +1141,13 @@ static vl_status get_step_graph(vl_sim* s, int steps, hipGraphExec_t* out, int* 
   e = hipGraphInstantiate(&ge, gph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(gph);
   if (e != hipSuccess) return set_error(VL_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  // upload it now, so that its first launch costs what a replay does (a graph prepared ahead of a
+  // run, vl_sim_prepare, then keeps that cost out of the run)
+  e = hipGraphUpload(ge, s->stream);
+  if (e != hipSuccess) {
+    (void)hipGraphExecDestroy(ge);
+    return set_error(VL_EDEVICE, std::string("hipGraphUpload: ") + hipGetErrorString(e));
+  }
   s->graphs[key] = {ge, cur_after};
   *out = ge;
   *after = cur_after;
