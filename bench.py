@@ -90,15 +90,16 @@ def run_gpu(vl, api, wl, precision, steps, warmup, scale, keep=False, kernel_tim
     info = ch.sim.info()
     log(f"[bench] {PREC_NAME[precision]}: built {info.nx}x{info.ny}, {info.fluid_nodes} fluid nodes, "
         f"{info.link_count} links in {time.time() - t0:.1f} s")
-    coef_w = ch.run(warmup, d) if warmup else np.zeros((0, 2))
     # timed region: `steps` steps through vl_channel_run, in pieces of up to 10000 steps, each
     # followed by a progress line.  Whole step windows (30 steps at 5 steps per pass, 32 at 4)
     # replay captured graphs; the whole groups of K steps left over replay one graph too, which
-    # vl_sim_prepare instantiates here before the clock starts (host-side set-up, like the
-    # warmup: it runs nothing).  The fixed cost of a call (control table upload, force history
-    # readback) stays in the timed region, so a run of a few groups still reads a little below
-    # the full-length rate.
-    ch.sim.prepare(min(10000, steps))
+    # vl_sim_prepare instantiates (host-side set-up: it runs nothing) for the state the timed run
+    # starts from, before the warmup steps, so that the warmup runs right before the clock starts
+    # and the GPU is not left idle in between.  The fixed cost of a call (control table upload,
+    # force history readback) stays in the timed region, so a run of a few groups still reads a
+    # little below the full-length rate.
+    ch.sim.prepare(min(10000, steps), after=warmup)
+    coef_w = ch.run(warmup, d) if warmup else np.zeros((0, 2))
     pieces, done = [], 0
     t0 = time.perf_counter()
     while done < steps:

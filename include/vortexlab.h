@@ -186,6 +186,9 @@ vl_status vl_sim_run_noalloc(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, doubl
  * launched one by one otherwise.  Results are the same bits either way.  For callers that time
  * or repeat short runs. */
 vl_status vl_sim_prepare(vl_sim* s, int64_t nsteps);
+/* The same for a run of nsteps steps that starts skip steps from the current state (no reference
+ * counterpart), so that a caller can prepare a timed run before its warmup steps. */
+vl_status vl_sim_prepare_after(vl_sim* s, int64_t skip, int64_t nsteps);
 /* Sim::force after the last step (force_groups * 2 doubles). */
 vl_status vl_sim_get_force(vl_sim* s, double* out);
 /* Populations of A (the current state) in the reference layout, as f_i in FP64. */

@@ -142,7 +142,8 @@ def test_prepared_short_runs_equal_direct_launches(gpu, monkeypatch, k, prec):
     window; runs that replay it give the bits of the same runs launched group by group (K-step
     passes, and fused single steps), over pieces of every kind: shorter than a group, a few
     groups, a window and more, with and without single steps left; preparing leaves the state
-    as it was."""
+    as it was.  Each piece prepared just before it runs, or ahead, before the piece before it
+    (vl_sim_prepare_after)."""
     spec = tunnel_spec(d=16, re=100.0)
     if k == 1:
         single_step(monkeypatch, "fused")
@@ -155,19 +156,23 @@ def test_prepared_short_runs_equal_direct_launches(gpu, monkeypatch, k, prec):
         ch = api.Channel(gpu, spec, prec)
         assert ch.sim.info().pass_steps == k
         coefs = []
-        for n in pieces:
-            if prepare:
-                t = ch.sim.t
+        for i, n in enumerate(pieces):
+            t = ch.sim.t
+            if prepare == "now":
                 ch.sim.prepare(n)
-                assert ch.sim.t == t
+            elif prepare == "ahead" and i + 1 < len(pieces):
+                ch.sim.prepare(pieces[i + 1], after=n)
+            assert ch.sim.t == t
             coefs.append(ch.run(n, 16.0))
         pops = ch.sim.read()
         ch.close()
         return np.concatenate(coefs), pops
 
-    direct, prepared = go(False), go(True)
-    assert np.array_equal(direct[0], prepared[0]), "forces differ"
-    assert np.array_equal(direct[1], prepared[1]), "populations differ"
+    direct = go(None)
+    for mode in ("now", "ahead"):
+        prepared = go(mode)
+        assert np.array_equal(direct[0], prepared[0]), f"forces differ ({mode})"
+        assert np.array_equal(direct[1], prepared[1]), f"populations differ ({mode})"
 
 
 @pytest.mark.parametrize("workload,prec,steps", [("notched_8192x4096", api.F32, 43),

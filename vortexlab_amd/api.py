@@ -131,6 +131,7 @@ _SIGS = [
     ("sim_run", C.c_int, [_VP, C.c_int64, C.POINTER(Ctl), _D]),
     ("sim_run_noalloc", C.c_int, [_VP, C.c_int64, C.POINTER(Ctl), _D]),
     ("sim_prepare", C.c_int, [_VP, C.c_int64]),
+    ("sim_prepare_after", C.c_int, [_VP, C.c_int64, C.c_int64]),
     ("sim_get_force", C.c_int, [_VP, _D]),
     ("sim_read", C.c_int, [_VP, _D]),
     ("sim_write", C.c_int, [_VP, _D]),
@@ -472,11 +473,11 @@ class Sim:
         self.be.check(self.be.sim_run(self.h, k, arr, _dptr(hist) if want_force else None))
         return hist[:, :ng] if want_force else None
 
-    def prepare(self, steps: int):
-        """Instantiates the step graphs a run of ``steps`` steps from the current state replays
-        (GPU library only; runs shorter than a step window replay their whole groups as one
-        graph only once it is prepared)."""
-        self.be.check(self.be.sim_prepare(self.h, int(steps)))
+    def prepare(self, steps: int, after: int = 0):
+        """Instantiates the step graphs a run of ``steps`` steps replays, the run starting
+        ``after`` steps from the current state (GPU library only; runs shorter than a step window
+        replay their whole groups as one graph only once it is prepared)."""
+        self.be.check(self.be.sim_prepare_after(self.h, int(after), int(steps)))
 
     def init(self, field: Callable[[int, int], tuple]):
         """Sim::init: equilibrium of field(x, y) -> (rho, ux, uy) on every fluid node."""

@@ -1441,13 +1441,19 @@ vl_status vl_sim_run(vl_sim* s, int64_t nsteps, const vl_ctl* ctl, double* force
   return run_steps(s, nsteps, ctl, force_hist, RunBufs{s->d_ctl, s->dhist, RUN_CHUNK, true});
 }
 
-vl_status vl_sim_prepare(vl_sim* s, int64_t nsteps) {
-  if (!s || nsteps < 0) return set_error(VL_EINVAL, "vl_sim_prepare: null simulation or negative step count");
+vl_status vl_sim_prepare(vl_sim* s, int64_t nsteps) { return vl_sim_prepare_after(s, 0, nsteps); }
+
+vl_status vl_sim_prepare_after(vl_sim* s, int64_t skip, int64_t nsteps) {
+  if (!s || nsteps < 0 || skip < 0)
+    return set_error(VL_EINVAL, "vl_sim_prepare: null simulation or negative step count");
   if (!s->use_graphs) return VL_OK;
-  // the state sequence of run_steps over nsteps, without running: whole windows, one graph of
+  // the state sequence of run_steps over nsteps, without running, from the state `skip` steps
+  // ahead (every step of a run moves the current buffer on by one): whole windows, one graph of
   // the whole groups left in each chunk, then single steps
   const int cur0 = s->cur;
   const int64_t t0 = s->t;
+  s->cur = (int)((s->cur + skip) % s->nbuf);
+  s->t += skip;
   const int win = window_steps(s), gs = group_steps(s);
   vl_status st = VL_OK;
   for (int64_t k0 = 0; k0 < nsteps && st == VL_OK; k0 += RUN_CHUNK) {
@@ -1474,6 +1480,9 @@ vl_status vl_sim_prepare(vl_sim* s, int64_t nsteps) {
   }
   s->cur = cur0;
   s->t = t0;
+  // the graphs' uploads run on the stream: wait for them, so that a run timed after this call
+  // starts on a quiet stream
+  if (st == VL_OK) VL_HIP(hipStreamSynchronize(s->stream));
   return st;
 }
 
