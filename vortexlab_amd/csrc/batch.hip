@@ -1924,10 +1924,17 @@ vl_status build(vl_batch* b) {
   // sweep: 1.78-1.82 ms against 2.0-2.6 ms a trial step), so by default only the three tile
   // shapes are timed, one workgroup per tile, over two steps each; VL_BATCH_AUTOTUNE=full times all
   // twelve policies over four.
-  bool full = false;
-  if (const char* e = std::getenv("VL_BATCH_AUTOTUNE")) full = std::string(e) == "full";
+  // With K-step passes the bulk tiles only carry the cases that end inside a group, a small
+  // share of the work, and the shape made no measurable difference, so the trials are skipped
+  // (128 x 2 tiles) unless VL_BATCH_AUTOTUNE asks for them ("on", or "full").
+  bool full = false, tune = b->pass_k == 1;
+  if (const char* e = std::getenv("VL_BATCH_AUTOTUNE")) {
+    full = std::string(e) == "full";
+    tune = full || std::string(e) == "on";
+  }
+  if (!tune) best_s = 1;
   const int nmult = full ? 4 : 1, reps = full ? 4 : 2;
-  for (int si = 0; si < 3; ++si) {
+  for (int si = 0; tune && si < 3; ++si) {
     bulk_tiles(b, order, 1 << shapes[si][0], shapes[si][1], kc, ky, kx, kend);
     int32_t *tkc, *tky, *tkx, *tact;
     std::vector<int32_t> act1(1, (int32_t)kc.size());
