@@ -57,6 +57,7 @@ struct BatchCase {
 };
 
 constexpr int BT = 256;  // threads of both batch kernels
+constexpr int LISTED_NPT = 2;  // listed nodes per thread of k_batch_listed (a tile holds LISTED_NPT * BT)
 
 // The bands of a case in K-step passes: the band_w columns beside the inlet (x = 1 .. band_w) and
 // beside the outlet (x = nx - 1 - band_w .. nx - 2), which no pass task covers.  Their
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(W * TB_LANES) void k_batch_pass(const typename S::T
 }
 
 // One single step of the interior nodes that no task covers, for the cases in K-step passes:
-// listed tile i holds count[i] <= BT nodes of case lcase[i] (packed y << 16 | x from
+// listed tile i holds count[i] <= LISTED_NPT * BT nodes of case lcase[i] (packed y << 16 | x from
 // nodes + first[i]); the tiles of the cases that run the group starting at t - jpos are a prefix
 // of nlist_at[t - jpos] tiles.  With side arrays the lists hold no band node, and the band
 // neighbours of time t are read from side_rd (null at a group's first step: the pool).
@@ -298,22 +299,40 @@ __global__ __launch_bounds__(BT) void k_batch_listed(const typename S::T* __rest
                                                      int t_local, int jpos, const int32_t* __restrict__ status,
                                                      const typename S::T* __restrict__ side_rd) {
   using T = typename S::T;
+  using C = typename S::C;
   const int64_t t = *tbase + t_local;
   const int n = nlist_at[t - jpos];
   for (int i = blockIdx.x; i < n; i += gridDim.x) {
     const int ci = lcase[i];
     if (status[ci]) continue;
     const BatchCase& cs = cases[ci];
-    const int k = (int)threadIdx.x;
-    if (k >= lcount[i]) continue;
-    const uint32_t v = nodes[lfirst[i] + k];
-    const int x = (int)(v & 0xffffu), y = (int)(v >> 16);
+    const int cnt = lcount[i];
     const BandView<const T> rv = band_view<const T>(cs, A, side_rd);
     const MainView<T> wv{B + cs.pop_off, cs.g.plane, cs.g.pitch};
-    if (cs.c.model == MODEL_BGK)
-      interior_node_v<S, MODEL_BGK>(x, y, rv, wv, cs.g, cs.c);
-    else
-      interior_node_v<S, MODEL_TRT>(x, y, rv, wv, cs.g, cs.c);
+    // LISTED_NPT nodes per thread (k, k + BT, ...): every node's pulls are issued before the
+    // first collide, so a wave has that many nodes' loads in flight
+    int x[LISTED_NPT], y[LISTED_NPT];
+    bool on[LISTED_NPT];
+    C f[LISTED_NPT][Q];
+#pragma unroll
+    for (int m = 0; m < LISTED_NPT; ++m) {
+      const int k = (int)threadIdx.x + m * BT;
+      on[m] = k < cnt;
+      const uint32_t v = on[m] ? nodes[lfirst[i] + k] : 0u;
+      x[m] = (int)(v & 0xffffu);
+      y[m] = (int)(v >> 16);
+      if (on[m]) pull_v<S>(rv, cs.g, x[m], y[m], f[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < LISTED_NPT; ++m) {
+      if (!on[m]) continue;
+      if (cs.c.model == MODEL_BGK)
+        collide<S, MODEL_BGK>(f[m], cs.c);
+      else
+        collide<S, MODEL_TRT>(f[m], cs.c);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) S::st(wv.at(q, x[m], y[m]), f[m][q]);
+    }
   }
 }
 
@@ -2033,10 +2052,10 @@ vl_status build(vl_batch* b) {
           if (bc_.band_w > 0 && (x <= bc_.band_w || x >= bc_.g.nx - 1 - bc_.band_w)) continue;
           lst.push_back(v);
         }
-        for (size_t k = 0; k < lst.size(); k += BT) {
+        for (size_t k = 0; k < lst.size(); k += (size_t)LISTED_NPT * BT) {
           lcase.push_back(ci);
           lfirst.push_back((int32_t)(lnodes.size() + k));
-          lcount.push_back((int32_t)std::min<size_t>(BT, lst.size() - k));
+          lcount.push_back((int32_t)std::min<size_t>((size_t)LISTED_NPT * BT, lst.size() - k));
         }
         clfirst[(size_t)ci] = (int32_t)lnodes.size();
         clcount[(size_t)ci] = (int32_t)lst.size();
