@@ -389,7 +389,9 @@ __device__ inline void boundary_node_v(const RV& rv, const WV& wv, const Geom& g
     const int j = L.dir;
     const int i = opp(j);
     const C out = own[j];
-    const double scale = ctl.vscale[L.vgroup];
+    // a select, not an indexed load: an indexed register array goes to the stack
+    const int vg = L.vgroup;
+    const double scale = vg == 0 ? ctl.vscale[0] : (vg == 1 ? ctl.vscale[1] : (vg == 2 ? ctl.vscale[2] : ctl.vscale[3]));
     C back;
     if (L.kind == LINK_WALL) {
       const double q = L.q;
