@@ -4,8 +4,8 @@ The labels come from the batched solver (vl_batch); the network is trained and e
 the HIP kernels behind vl_nn_* (vortexlab_amd/csrc/nn.hip).  A 4-member ensemble answers a
 query when its members agree on the drag (relative spread of the mean Cd at most a threshold
 calibrated on shapes with known answers so that the accepted answers meet a stated 90th-percentile
-error), otherwise the query falls back to the solver, so the ensemble is also the judge of its own
-error.
+error), otherwise the query falls back to the solver, so the ensemble's spread also decides when its
+own answer is trusted.
 
 Usage (one GPU):  python -m vortexlab_amd.surrogate --train 8192 --test 1024 --out report.json
 """
