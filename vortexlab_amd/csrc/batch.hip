@@ -466,7 +466,7 @@ __global__ __launch_bounds__(BT) void k_band_map_bnd(int32_t* __restrict__ bidx,
 // rows of a column on adjacent threads), turns it over in LDS, and stores it along the pool's rows
 // (the columns of a row on adjacent threads, two runs of band_w values per row).  Positions that
 // are not fluid keep their pool values.
-constexpr int COPY_ROWS = 64;
+constexpr int COPY_ROWS = 16;
 template <class T>
 __global__ __launch_bounds__(BT) void k_band_to_pool(T* __restrict__ B, const T* __restrict__ side0,
                                                      const BatchCase* __restrict__ cases, const int2* __restrict__ tiles,
